@@ -1,0 +1,148 @@
+/*
+ * mpfft -- multiprecision split-radix conjugate-pair FFT on AMD MI355X
+ * (gfx950).  C ABI of libmpfft.so.
+ *
+ * Drop-in for the FFT kernels declared in mfrewer/mpFFT's mpFFT.h: the same
+ * names and signatures (forward, out-of-place, split real/imaginary arrays,
+ * N = 2^n).  The multiprecision results are bit-identical to the
+ * reference's MPFR path (every operation correctly rounded at the arrays'
+ * precision, in the reference's operation order); the double results are
+ * bit-identical to the reference's double code.
+ *
+ * Two layers:
+ *   1. the reference entry points (legacy names, void, synchronous); a
+ *      failure is recorded per thread and read with mpfft_last_error();
+ *   2. a plan API (int status) with device-resident execution for callers
+ *      that keep data on the GPU.
+ *
+ * Device data format ("SoA limbs"), per transform of N points at L limbs:
+ *   uint32 words [0, 2N)            hdr: (exp << 1) | sign for a nonzero
+ *                                   value 0.1b..b * 2^exp, INT32_MIN | sign
+ *                                   for a signed zero
+ *   uint32 words [2N + j*2N, ...)   limb j (j = 0 most significant) of the
+ *                                   2N reals; real parts first, then
+ *                                   imaginary parts
+ * i.e. 2N*(L+1) words per transform, transforms back to back.
+ */
+#ifndef MPFFT_H
+#define MPFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <gmp.h>
+#include <mpfr.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define MPFFT_OK            0
+#define MPFFT_ERR_SIZE     -1   /* N is not a power of two, or too large      */
+#define MPFFT_ERR_PREC     -2   /* precision unsupported, or arrays disagree  */
+#define MPFFT_ERR_VALUE    -3   /* NaN / Inf / exponent beyond +-2^29 in input */
+#define MPFFT_ERR_SUBTREE  -4   /* legacy call with s != 1 or sn != 0         */
+#define MPFFT_ERR_HIP      -5   /* HIP runtime error                          */
+#define MPFFT_ERR_ARG      -6   /* null pointer or bad argument               */
+#define MPFFT_ERR_NODEV    -7   /* no usable GPU                              */
+#define MPFFT_ERR_FORM     -8   /* form not valid for this call               */
+
+/* ---- reference entry points -------------------------------------------- */
+
+/* mpFFT.h:38 / :66 (lab/serial/mp/serial_mp_fft3.c:121,
+ * lab/parallel/mp/parallel_mp_fft2.c:121).  Supported call: the top-level
+ * (n = N, s = 1, sn = 0), as every caller of the reference makes it
+ * (mpFFT.c:47/50).  All four arrays must hold N elements of one precision
+ * P (<= 1024 bits); outputs are written into outr/outi in place. */
+void mp_sr24_cp_fft(unsigned long n, unsigned long s, long int sn,
+                    mpfr_t *inr, mpfr_t *ini, mpfr_t *outr, mpfr_t *outi);
+
+/* mpFFT.h:39 (lab/serial/mp/serial_mp_fft4.c:121): shift-indexed twiddles,
+ * same results.  Supported call: (N, log2n = 0, s = 1, sn = 0). */
+void mp_sr24_cp_fft_v2(unsigned long n, unsigned long log2n, unsigned long s,
+                       long int sn, mpfr_t *inr, mpfr_t *ini, mpfr_t *outr,
+                       mpfr_t *outi);
+
+/* mpFFT.h:25 / :53 (lab/serial/double/serial_d_fft6.c:93, the CODE 6 op
+ * order; CODE 5 is reachable through the plan API, MPFFT_FORM_D5). */
+void d_sr24_cp_fft(unsigned long n, unsigned long s, long int sn,
+                   double *inr, double *ini, double *outr, double *outi);
+
+/* mpFFT.h:26 (lab/serial/double/serial_d_fft7.c:93-94): CODE 7, the same
+ * arithmetic as CODE 6.  Supported call: (N, 0, 1, 0). */
+void d_sr24_cp_fft_v2(unsigned long n, unsigned long log2n, unsigned long s,
+                      long int sn, double *inr, double *ini, double *outr,
+                      double *outi);
+
+/* status of the last reference-entry-point call on this thread */
+int mpfft_last_error(void);
+const char *mpfft_strerror(int code);
+
+/* ---- plan API ------------------------------------------------------------ */
+
+typedef struct mpfft_plan_s *mpfft_plan;
+
+#define MPFFT_FORM_MP  0   /* multiprecision, CODE 3 op order (3m3a)     */
+#define MPFFT_FORM_D5  5   /* double, CODE 5 op order (4m2a, 3 butterflies) */
+#define MPFFT_FORM_D6  6   /* double, CODE 6/7 op order (1 butterfly)      */
+
+/* Create a plan for `batch` transforms of size n at precision prec (bits;
+ * ignored for the double forms).  Builds the twiddle table on the host
+ * (MPFR, the reference's lut() sequence) and uploads it. */
+int mpfft_plan_create(mpfft_plan *plan, unsigned long n, long prec, int form,
+                      unsigned int batch);
+int mpfft_plan_destroy(mpfft_plan plan);
+
+/* limbs per real used by the device format, and words per transform */
+int mpfft_plan_info(mpfft_plan plan, long *limbs, size_t *words_per_transform);
+
+/* One transform through mpfr_t arrays (marshalling included). */
+int mpfft_execute_mpfr(mpfft_plan plan, mpfr_t *inr, mpfr_t *ini,
+                       mpfr_t *outr, mpfr_t *outi);
+
+/* `batch` transforms on device-resident SoA-limb buffers (d_in and d_out
+ * hold batch * words_per_transform words each; they must not overlap).
+ * Enqueued on `stream` (a hipStream_t, NULL = the plan's own stream);
+ * returns without waiting. */
+int mpfft_execute_limbs(mpfft_plan plan, const uint32_t *d_in, uint32_t *d_out,
+                        void *stream);
+
+/* Part of the transform for profiling: stage 0 is the leaf pass, stage i
+ * (1 <= i < mpfft_plan_stages) the combine pass of node size 2^(i+2).
+ * Running stages [0, mpfft_plan_stages) equals mpfft_execute_limbs. */
+int mpfft_execute_limbs_range(mpfft_plan plan, const uint32_t *d_in, uint32_t *d_out,
+                              void *stream, int stage_begin, int stage_end);
+int mpfft_plan_stages(mpfft_plan plan);
+
+/* The same from host SoA-limb buffers (copies in, runs, copies out). */
+int mpfft_execute_limbs_host(mpfft_plan plan, const uint32_t *h_in, uint32_t *h_out);
+
+/* Double forms: `batch` transforms of split arrays, re then im, 2N doubles
+ * per transform; device-resident or host. */
+int mpfft_execute_f64(mpfft_plan plan, const double *d_in, double *d_out, void *stream);
+int mpfft_execute_f64_host(mpfft_plan plan, const double *h_in, double *h_out);
+
+/* Wait for the plan's stream. */
+int mpfft_plan_sync(mpfft_plan plan);
+
+/* ---- conversion ----------------------------------------------------------- */
+
+/* mpfr_t arrays (re[n], im[n]) <-> one SoA-limb transform at `limbs` limbs. */
+int mpfft_mpfr_to_limbs(unsigned long n, long limbs, mpfr_t *re, mpfr_t *im,
+                        uint32_t *h_out);
+int mpfft_limbs_to_mpfr(unsigned long n, long limbs, const uint32_t *h_in,
+                        mpfr_t *re, mpfr_t *im);
+
+/* ---- device arithmetic check ---------------------------------------------- */
+
+/* z[i] = x[i] op y[i] correctly rounded at prec bits on the GPU; op 0 add,
+ * 1 sub, 2 mul; x, y, z: host SoA-limb arrays of `count` reals at
+ * mpfft_limbs_for_precision(prec) limbs (hdr[count] then limbs). */
+int mpfft_mp_binop_host(int op, unsigned long count, long prec,
+                        const uint32_t *x, const uint32_t *y, uint32_t *z);
+long mpfft_limbs_for_precision(long prec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPFFT_H */

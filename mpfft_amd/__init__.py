@@ -1,0 +1,156 @@
+"""mpfft_amd: multiprecision split-radix conjugate-pair FFT on AMD MI355X.
+
+The product is the C-ABI library libmpfft.so (include/mpfft.h), built from
+mpfft_amd/csrc by `python -m mpfft_amd.build`.  This module only loads it
+through ctypes for tests and benchmarks; there is no Python or CPU
+fallback: if the library is missing, loading raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmpfft.so")
+
+FORM_MP, FORM_D5, FORM_D6 = 0, 5, 6
+OK = 0
+ERRORS = {
+    -1: "MPFFT_ERR_SIZE", -2: "MPFFT_ERR_PREC", -3: "MPFFT_ERR_VALUE",
+    -4: "MPFFT_ERR_SUBTREE", -5: "MPFFT_ERR_HIP", -6: "MPFFT_ERR_ARG",
+    -7: "MPFFT_ERR_NODEV", -8: "MPFFT_ERR_FORM",
+}
+
+# every symbol include/mpfft.h declares
+EXPORTS = [
+    "mp_sr24_cp_fft", "mp_sr24_cp_fft_v2", "d_sr24_cp_fft", "d_sr24_cp_fft_v2",
+    "mpfft_last_error", "mpfft_strerror", "mpfft_plan_create", "mpfft_plan_destroy",
+    "mpfft_plan_info", "mpfft_execute_mpfr", "mpfft_execute_limbs",
+    "mpfft_execute_limbs_host", "mpfft_execute_limbs_range", "mpfft_plan_stages",
+    "mpfft_execute_f64", "mpfft_execute_f64_host",
+    "mpfft_plan_sync", "mpfft_mpfr_to_limbs", "mpfft_limbs_to_mpfr",
+    "mpfft_mp_binop_host", "mpfft_limbs_for_precision",
+]
+
+_lib = None
+
+
+class MpfftError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        super().__init__(f"{what}: {ERRORS.get(code, code)}")
+
+
+def lib():
+    """Load libmpfft.so (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"{LIB_PATH} not built: run python -m mpfft_amd.build")
+    L = ctypes.CDLL(LIB_PATH)
+    P, U, I, L_, VP = (ctypes.c_void_p, ctypes.c_ulong, ctypes.c_int, ctypes.c_long,
+                       ctypes.c_void_p)
+    L.mpfft_plan_create.argtypes = [ctypes.POINTER(VP), U, L_, I, ctypes.c_uint]
+    L.mpfft_plan_destroy.argtypes = [VP]
+    L.mpfft_plan_info.argtypes = [VP, ctypes.POINTER(L_), ctypes.POINTER(ctypes.c_size_t)]
+    L.mpfft_execute_limbs.argtypes = [VP, P, P, P]
+    L.mpfft_execute_limbs_host.argtypes = [VP, P, P]
+    L.mpfft_execute_limbs_range.argtypes = [VP, P, P, P, I, I]
+    L.mpfft_plan_stages.argtypes = [VP]
+    L.mpfft_execute_f64.argtypes = [VP, P, P, P]
+    L.mpfft_execute_f64_host.argtypes = [VP, P, P]
+    L.mpfft_plan_sync.argtypes = [VP]
+    L.mpfft_mp_binop_host.argtypes = [I, U, L_, P, P, P]
+    L.mpfft_limbs_for_precision.argtypes = [L_]
+    L.mpfft_limbs_for_precision.restype = L_
+    L.mpfft_strerror.restype = ctypes.c_char_p
+    L.mp_sr24_cp_fft.argtypes = [U, U, L_, P, P, P, P]
+    L.mp_sr24_cp_fft_v2.argtypes = [U, U, U, L_, P, P, P, P]
+    L.d_sr24_cp_fft.argtypes = [U, U, L_, P, P, P, P]
+    L.d_sr24_cp_fft_v2.argtypes = [U, U, U, L_, P, P, P, P]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def check(rc, what):
+    if rc != OK:
+        raise MpfftError(rc, what)
+
+
+def limbs_for_precision(P):
+    return int(lib().mpfft_limbs_for_precision(P))
+
+
+class Plan:
+    """A plan of `batch` transforms of size n (mp form at P bits, or a
+    double form)."""
+
+    def __init__(self, n, P=0, form=FORM_MP, batch=1):
+        self.n, self.P, self.form, self.batch = n, P, form, batch
+        h = ctypes.c_void_p()
+        check(lib().mpfft_plan_create(ctypes.byref(h), n, P, form, batch), "plan_create")
+        self.h = h
+        Lc, W = ctypes.c_long(), ctypes.c_size_t()
+        check(lib().mpfft_plan_info(h, ctypes.byref(Lc), ctypes.byref(W)), "plan_info")
+        self.L, self.words = Lc.value, W.value
+
+    def close(self):
+        if self.h:
+            lib().mpfft_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def execute_host(self, x):
+        """x: host array of batch*words (uint32 for mp, float64 for double)."""
+        if self.form == FORM_MP:
+            x = np.ascontiguousarray(x, dtype=np.uint32)
+            y = np.empty_like(x)
+            assert x.size == self.words * self.batch
+            check(lib().mpfft_execute_limbs_host(self.h, _ptr(x), _ptr(y)), "execute_limbs_host")
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            y = np.empty_like(x)
+            assert x.size == self.words * self.batch
+            check(lib().mpfft_execute_f64_host(self.h, _ptr(x), _ptr(y)), "execute_f64_host")
+        return y
+
+    def execute_device(self, d_in, d_out, stream=None):
+        """d_in, d_out: device pointers (ints), e.g. torch tensor data_ptr()."""
+        s = ctypes.c_void_p(stream) if stream else None
+        if self.form == FORM_MP:
+            rc = lib().mpfft_execute_limbs(self.h, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out), s)
+        else:
+            rc = lib().mpfft_execute_f64(self.h, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out), s)
+        check(rc, "execute_device")
+
+    def stages(self):
+        return int(lib().mpfft_plan_stages(self.h))
+
+    def execute_range(self, d_in, d_out, s0, s1, stream=None):
+        s = ctypes.c_void_p(stream) if stream else None
+        check(lib().mpfft_execute_limbs_range(self.h, ctypes.c_void_p(d_in),
+                                              ctypes.c_void_p(d_out), s, s0, s1),
+              "execute_range")
+
+    def sync(self):
+        check(lib().mpfft_plan_sync(self.h), "plan_sync")
+
+
+def mp_binop(op, x, y, count, P):
+    """element-wise correctly rounded op (0 add, 1 sub, 2 mul) on the GPU;
+    x, y: SoA-limb arrays of count reals at limbs_for_precision(P) limbs."""
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    y = np.ascontiguousarray(y, dtype=np.uint32)
+    z = np.empty_like(x)
+    check(lib().mpfft_mp_binop_host(op, count, P, _ptr(x), _ptr(y), _ptr(z)), "mp_binop")
+    return z

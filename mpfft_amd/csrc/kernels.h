@@ -1,0 +1,65 @@
+// Kernel argument blocks and launchers shared by the HIP sources
+// (mp_fft.hip, d_fft.hip) and the host library (mpfft.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpfft {
+
+// ---- multiprecision plain form (mp_fft.hip) ----
+
+struct LevelArgs {
+    uint32_t *data;             // out buffer of transform 0
+    const uint32_t *lut;        // twiddle buffer
+    const uint32_t *node_off;   // output offsets of the size-n nodes
+    uint64_t stride;            // words between consecutive transforms
+    uint32_t N, n, cnt;         // size, node size, number of size-n nodes
+    uint32_t nG, rG;            // general butterflies; padded range (pairs)
+    uint32_t nS, rS;            // k = n/8 butterflies; padded range
+    uint32_t nZ;                // k = 0 butterflies
+    int P;
+};
+
+struct LeafArgs {
+    const uint32_t *in;         // input buffer of transform 0
+    uint32_t *data;             // out buffer of transform 0
+    const int32_t *sn4;         // size-4 leaves: input offset sn
+    const uint32_t *off4;       //                output offset
+    const int32_t *sn2;         // size-2 leaves
+    const uint32_t *off2;
+    uint64_t in_stride, stride;
+    uint32_t N, n4, r4, n2;
+    int P;
+};
+
+// kernels exist for these limb counts; a precision P uses the smallest
+// count >= ceil(P/32)
+int limbs_for_precision(long P);
+
+hipError_t mp_fft_launch(int L, int P, const LeafArgs &la, const LevelArgs *lv,
+                         int nlev, uint32_t batch, hipStream_t st);
+hipError_t mp_binop_launch(int L, int P, int op, uint32_t count, const uint32_t *x,
+                           const uint32_t *y, uint32_t *z, hipStream_t st);
+
+// ---- double path (d_fft.hip) ----
+
+struct DTables {
+    const int32_t *leaf_sn;     // per leaf: input offset
+    const uint32_t *leaf_off;   //           output offset
+    const uint32_t *leaf_n;     //           size (1, 2, 4, 8, 16)
+    uint32_t nleaf;
+    const uint32_t *lvl_off;    // node offsets of all levels, concatenated
+    const uint32_t *lvl_begin;  // lvl_begin[l] .. lvl_begin[l+1]: level 2^l
+    const double *wr, *wi;      // N/4 each
+    double c[5];                // w8, w16_1, w16_2, w16_3, w16_4
+    uint32_t N, lgN, lmin;      // combine levels lmin..lgN
+    int code;                   // 5 or 6
+};
+
+// largest transform run from LDS by one workgroup (2N doubles)
+constexpr uint32_t kDLdsMaxN = 4096;
+
+hipError_t d_fft_launch(const DTables &t, const double *in, double *out, uint64_t stride,
+                        uint32_t batch, hipStream_t st);
+
+}  // namespace mpfft

@@ -1,0 +1,321 @@
+// Correctly rounded P-bit floating-point arithmetic on 32-bit limbs, for
+// gfx950 (CDNA4).  Replaces the mpfr_add / mpfr_sub / mpfr_mul calls of the
+// reference hot loop (lab/serial/mp/serial_mp_fft3.c:141-248) with
+// register-resident limb code: every result equals MPFR's RNDN result at
+// precision P (round to nearest, ties to even, unbounded exponent, signed
+// zeros as in IEEE 754 / MPFR: x - x = +0, (-0) + (-0) = -0).
+//
+// Number layout in registers: m[0] is the most significant limb and has its
+// top bit set for a nonzero value 0.m * 2^e; bits below position P are zero.
+// L = number of limbs (compile time), 32(L-1) < P <= 32L.  FULL means
+// P == 32L (no padding bits), which shortens the rounding step.
+//
+// Every routine is branch-light: the common path is straight-line code over
+// the limb arrays with compile-time indices; data-dependent limb shifts
+// (large exponent gaps, massive cancellation) run as bounded loops of
+// one-limb moves, so no register array is ever indexed dynamically.
+//
+// The header also compiles as plain C++ with MPFFT_CPU_BUILD defined; that
+// build exists only so the arithmetic can be checked against MPFR on a host
+// without a GPU (tests/cpu_arith_check.cpp).
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+#ifdef MPFFT_CPU_BUILD
+#define MPFFT_FN static inline
+#else
+#include <hip/hip_runtime.h>
+#define MPFFT_FN __device__ __forceinline__
+#endif
+
+namespace mpfft {
+
+constexpr int32_t kHdrZero = INT32_MIN;   // hdr of a signed zero (| sign bit)
+
+template <int L>
+struct Mp {
+    uint32_t m[L];
+    int32_t e;        // exponent (value = 0.m * 2^e) when !zero
+    uint32_t neg;     // sign bit
+    uint32_t zero;    // 1 for +-0
+};
+
+MPFFT_FN uint32_t fshr(uint32_t hi, uint32_t lo, uint32_t s)
+{
+    // low 32 bits of (hi:lo) >> s, s in [0, 31]
+#ifdef MPFFT_CPU_BUILD
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> s);
+#else
+    return __builtin_amdgcn_alignbit(hi, lo, s);
+#endif
+}
+
+MPFFT_FN uint32_t clz32(uint32_t v)   // v != 0
+{
+#ifdef MPFFT_CPU_BUILD
+    return (uint32_t)__builtin_clz(v);
+#else
+    return (uint32_t)__clz((int)v);
+#endif
+}
+
+MPFFT_FN uint32_t add_cc(uint32_t a, uint32_t b, uint32_t &c)
+{
+    uint64_t s = (uint64_t)a + b + c;
+    c = (uint32_t)(s >> 32);
+    return (uint32_t)s;
+}
+
+MPFFT_FN uint32_t sub_bb(uint32_t a, uint32_t b, uint32_t &br)
+{
+    uint64_t s = (uint64_t)a - b - br;
+    br = (uint32_t)(s >> 63);
+    return (uint32_t)s;
+}
+
+// Round the normalized window w[0..L] (w[0] has its top bit set) plus a
+// sticky flag (nonzero remainder below w[L]) to P bits, ties to even.
+// On mantissa overflow the result becomes 0.1000.. and e is incremented.
+template <int L, bool FULL>
+MPFFT_FN void round_window(uint32_t (&w)[L + 1], uint32_t sticky,
+                           int32_t &e, int P)
+{
+    if (FULL) {
+        uint32_t rbit = w[L] >> 31;
+        uint32_t rest = (w[L] << 1) | sticky;
+        uint32_t up = rbit & ((rest != 0) | (w[L - 1] & 1));
+        uint32_t c = up;
+#pragma unroll
+        for (int j = L - 1; j >= 0; j--) w[j] = add_cc(w[j], 0, c);
+        if (c) { w[0] = 0x80000000u; e += 1; }
+    } else {
+        // last kept bit: index P-1 from the top -> limb kl, bit kb (from LSB)
+        const int kl = (P - 1) >> 5;
+        const uint32_t kb = 31u - (uint32_t)((P - 1) & 31);
+        // round bit: index P from the top -> limb rl, bit rb
+        const int rl = P >> 5;
+        const uint32_t rb = 31u - (uint32_t)(P & 31);
+        uint32_t rbit = 0, rest = sticky, lsb = 0;
+#pragma unroll
+        for (int j = 0; j <= L; j++) {
+            uint32_t v = w[j];
+            uint32_t below = (rb == 0) ? 0u : (v & ((1u << rb) - 1u));
+            if (j == rl) { rbit = (v >> rb) & 1u; rest |= below; }
+            if (j > rl) rest |= v;
+            if (j == kl) lsb = (v >> kb) & 1u;
+        }
+        uint32_t up = rbit & ((rest != 0) | lsb);
+        uint32_t c = 0;
+#pragma unroll
+        for (int j = L; j >= 0; j--) {
+            uint32_t keep = (j < kl) ? 0xffffffffu
+                          : (j == kl) ? ~((1u << kb) - 1u) : 0u;
+            uint32_t inc = (j == kl && up) ? (1u << kb) : 0u;
+            w[j] = add_cc(w[j] & keep, inc, c);
+        }
+        if (c) { w[0] = 0x80000000u; e += 1; }
+    }
+}
+
+template <int L>
+MPFFT_FN void set_zero(Mp<L> &z, uint32_t neg)
+{
+#pragma unroll
+    for (int j = 0; j < L; j++) z.m[j] = 0;
+    z.e = 0; z.neg = neg; z.zero = 1;
+}
+
+// z = (-1)^fx * x + (-1)^fy * y, correctly rounded to P bits.
+template <int L, bool FULL>
+MPFFT_FN void mp_add(Mp<L> &z, const Mp<L> &x, uint32_t fx,
+                     const Mp<L> &y, uint32_t fy, int P)
+{
+    const uint32_t xs = x.neg ^ fx, ys = y.neg ^ fy;
+    if (x.zero | y.zero) {
+        if (x.zero & y.zero) { set_zero(z, xs & ys); return; }
+        if (x.zero) { z = y; z.neg = ys; return; }
+        z = x; z.neg = xs; return;
+    }
+    // a = operand with the larger exponent, b = the other one
+    const bool sw = y.e > x.e;
+    uint32_t aw[L + 1], bw[L + 1];
+#pragma unroll
+    for (int j = 0; j < L; j++) {
+        aw[j] = sw ? y.m[j] : x.m[j];
+        bw[j] = sw ? x.m[j] : y.m[j];
+    }
+    aw[L] = 0; bw[L] = 0;
+    int32_t e = sw ? y.e : x.e;
+    const uint32_t sa = sw ? ys : xs, sb = sw ? xs : ys;
+    const uint32_t d = (uint32_t)(sw ? (y.e - x.e) : (x.e - y.e));
+
+    // align b: shift right by d bits into the (L+1)-limb window, OR-ing
+    // everything that falls off the window into sticky
+    uint32_t sticky = 0;
+    if (d >= 32u * (L + 1)) {
+#pragma unroll
+        for (int j = 0; j <= L; j++) bw[j] = 0;
+        sticky = 1;
+    } else {
+        const uint32_t q = d >> 5, r = d & 31u;
+        for (uint32_t t = 0; t < q; t++) {
+            sticky |= bw[L];
+#pragma unroll
+            for (int j = L; j >= 1; j--) bw[j] = bw[j - 1];
+            bw[0] = 0;
+        }
+        if (r) {
+            sticky |= bw[L] << (32u - r);
+#pragma unroll
+            for (int j = L; j >= 1; j--) bw[j] = fshr(bw[j - 1], bw[j], r);
+            bw[0] >>= r;
+        }
+        sticky = sticky != 0;
+    }
+
+    uint32_t sign = sa;
+    if (sa == sb) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int j = L; j >= 0; j--) aw[j] = add_cc(aw[j], bw[j], c);
+        if (c) {
+            sticky |= aw[L] & 1u;
+#pragma unroll
+            for (int j = L; j >= 1; j--) aw[j] = fshr(aw[j - 1], aw[j], 1);
+            aw[0] = (aw[0] >> 1) | 0x80000000u;
+            e += 1;
+        }
+    } else {
+        // exact value is aw - bw - delta with 0 < delta < 1 (guard-LSB units)
+        // when sticky: subtract one more unit and keep sticky set
+        uint32_t br = sticky;
+#pragma unroll
+        for (int j = L; j >= 0; j--) aw[j] = sub_bb(aw[j], bw[j], br);
+        if (br) {
+            // only possible for d == 0 (then sticky == 0): |b| > |a|
+            uint32_t c = 1;
+#pragma unroll
+            for (int j = L; j >= 0; j--) aw[j] = add_cc(~aw[j], 0, c);
+            sign ^= 1u;
+        }
+        uint32_t any = sticky;
+#pragma unroll
+        for (int j = 0; j <= L; j++) any |= aw[j];
+        if (!any) { set_zero(z, 0); return; }    // exact cancellation: +0
+        // normalize (more than one bit only when d <= 1, i.e. exact)
+        while (aw[0] == 0) {
+#pragma unroll
+            for (int j = 0; j < L; j++) aw[j] = aw[j + 1];
+            aw[L] = 0;
+            e -= 32;
+        }
+        const uint32_t lz = clz32(aw[0]);
+        if (lz) {
+#pragma unroll
+            for (int j = 0; j < L; j++) aw[j] = fshr(aw[j], aw[j + 1], 32u - lz);
+            aw[L] <<= lz;
+            e -= (int32_t)lz;
+        }
+    }
+    round_window<L, FULL>(aw, sticky, e, P);
+#pragma unroll
+    for (int j = 0; j < L; j++) z.m[j] = aw[j];
+    z.e = e; z.neg = sign; z.zero = 0;
+}
+
+// z = x * y, correctly rounded to P bits (schoolbook product, column-wise
+// from the least significant column; the low L-1 columns only feed sticky).
+template <int L, bool FULL>
+MPFFT_FN void mp_mul(Mp<L> &z, const Mp<L> &x, const Mp<L> &y, int P)
+{
+    const uint32_t sign = x.neg ^ y.neg;
+    if (x.zero | y.zero) { set_zero(z, sign); return; }
+    uint32_t w[L + 1];
+    uint32_t sticky = 0;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int c = 0; c < 2 * L - 1; c++) {
+        uint32_t ov = 0;
+        const int i0 = c - (L - 1) > 0 ? c - (L - 1) : 0;
+        const int i1 = c < L - 1 ? c : L - 1;
+#pragma unroll
+        for (int i = i0; i <= i1; i++) {
+            const uint64_t p = (uint64_t)x.m[L - 1 - i] * (uint64_t)y.m[L - 1 - (c - i)];
+            acc += p;
+            ov += acc < p;
+        }
+        const uint32_t limb = (uint32_t)acc;
+        if (c < L - 1) sticky |= limb;
+        else w[2 * L - 1 - c] = limb;
+        acc = (acc >> 32) | ((uint64_t)ov << 32);
+    }
+    w[0] = (uint32_t)acc;
+    int32_t e = x.e + y.e;
+    if (!(w[0] >> 31)) {
+#pragma unroll
+        for (int j = 0; j < L; j++) w[j] = fshr(w[j], w[j + 1], 31);
+        w[L] <<= 1;
+        e -= 1;
+    }
+    round_window<L, FULL>(w, sticky != 0, e, P);
+#pragma unroll
+    for (int j = 0; j < L; j++) z.m[j] = w[j];
+    z.e = e; z.neg = sign; z.zero = 0;
+}
+
+// o = c ? a : b, element-wise (keeps the arrays in registers)
+template <int L>
+MPFFT_FN void mp_select(Mp<L> &o, uint32_t c, const Mp<L> &a, const Mp<L> &b)
+{
+#pragma unroll
+    for (int j = 0; j < L; j++) o.m[j] = c ? a.m[j] : b.m[j];
+    o.e = c ? a.e : b.e;
+    o.neg = c ? a.neg : b.neg;
+    o.zero = c ? a.zero : b.zero;
+}
+
+// ---- SoA load/store: buffer = hdr int32[M] followed by limb uint32[L][M] ----
+
+template <int L>
+MPFFT_FN void mp_load(Mp<L> &x, const uint32_t *__restrict__ buf,
+                      uint32_t M, uint32_t i)
+{
+    const int32_t h = (int32_t)buf[i];
+    x.neg = (uint32_t)h & 1u;
+    x.zero = (h & ~1) == kHdrZero;
+    x.e = h >> 1;
+    const uint32_t *lb = buf + M;
+#pragma unroll
+    for (int j = 0; j < L; j++) x.m[j] = lb[(size_t)j * M + i];
+}
+
+template <int L>
+MPFFT_FN void mp_store(const Mp<L> &x, uint32_t *__restrict__ buf,
+                       uint32_t M, uint32_t i)
+{
+    buf[i] = x.zero ? ((uint32_t)kHdrZero | x.neg) : (((uint32_t)x.e << 1) | x.neg);
+    uint32_t *lb = buf + M;
+#pragma unroll
+    for (int j = 0; j < L; j++) lb[(size_t)j * M + i] = x.zero ? 0u : x.m[j];
+}
+
+#ifndef MPFFT_CPU_BUILD
+// exchange a number with the partner lane (lane ^ 1) of the same wave
+MPFFT_FN uint32_t swap_pair(uint32_t v)
+{
+    // DPP quad_perm [1,0,3,2]
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+}
+
+template <int L>
+MPFFT_FN void mp_swap_pair(Mp<L> &o, const Mp<L> &x)
+{
+#pragma unroll
+    for (int j = 0; j < L; j++) o.m[j] = swap_pair(x.m[j]);
+    o.e = (int32_t)swap_pair((uint32_t)x.e);
+    o.neg = swap_pair(x.neg);
+    o.zero = swap_pair(x.zero);
+}
+#endif
+
+}  // namespace mpfft

@@ -1,0 +1,620 @@
+// libmpfft host side: plans, twiddle tables, mpfr_t marshalling and the
+// reference entry points (include/mpfft.h).
+//
+// The reference computes one transform per call with depth-first recursion
+// over mpfr_t arrays (lab/serial/mp/serial_mp_fft3.c:121-253).  Here a plan
+// flattens that recursion once into leaf tables and per-size node lists,
+// keeps the twiddles (the reference's lut() sequence, evaluated by MPFR at
+// the plan's precision) resident in HBM, and drives the HIP kernels.
+#include "../../include/mpfft.h"
+#include "kernels.h"
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+static_assert(sizeof(mp_limb_t) == 8, "64-bit GMP limbs expected");
+
+using namespace mpfft;
+
+namespace {
+
+thread_local int g_last_error = MPFFT_OK;
+
+constexpr long kMaxPrec = 1024;          // largest compiled limb count * 32
+constexpr long kExpLimit = 1L << 29;     // accepted input exponent range
+constexpr unsigned long kMaxN = 1UL << 24;
+
+#define HIP_TRY(expr)                                          \
+    do {                                                       \
+        if ((expr) != hipSuccess) return MPFFT_ERR_HIP;        \
+    } while (0)
+
+bool is_pow2(unsigned long n) { return n && !(n & (n - 1)); }
+
+uint32_t pad32(uint32_t v) { return (v + 31u) & ~31u; }
+
+// ---- recursion tree (serial_mp_fft3.c:180-182) ----------------------------
+
+struct Tree {
+    struct Leaf { int32_t sn; uint32_t off, n; };
+    std::vector<Leaf> leaves;
+    std::vector<std::vector<uint32_t>> level;   // level[l]: offsets of size-2^l nodes
+    uint32_t lgN = 0;
+
+    void walk(uint32_t n, uint32_t s, int32_t sn, uint32_t off, uint32_t leaf_max)
+    {
+        if (n <= leaf_max) { leaves.push_back({sn, off, n}); return; }
+        uint32_t l = 0; while ((1u << l) < n) l++;
+        level[l].push_back(off);
+        walk(n / 2, 2 * s, sn, off, leaf_max);
+        walk(n / 4, 4 * s, sn + (int32_t)s, off + n / 2, leaf_max);
+        walk(n / 4, 4 * s, sn - (int32_t)s, off + 3 * n / 4, leaf_max);
+    }
+    Tree(uint32_t N, uint32_t leaf_max)
+    {
+        while ((1u << lgN) < N) lgN++;
+        level.resize(lgN + 2);
+        walk(N, 1, 0, 0, leaf_max);
+    }
+};
+
+// ---- device buffer helper -------------------------------------------------
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t b)
+    {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        bytes = b;
+        if (!b) return MPFFT_OK;
+        return hipMalloc(&p, b) == hipSuccess ? MPFFT_OK : MPFFT_ERR_HIP;
+    }
+    template <class T> int upload(const std::vector<T> &v)
+    {
+        int rc = alloc(sizeof(T) * (v.empty() ? 1 : v.size()));
+        if (rc) return rc;
+        if (!v.empty())
+            HIP_TRY(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+        return MPFFT_OK;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+// ---- mpfr_t <-> SoA limbs --------------------------------------------------
+
+int mpfr_put(mpfr_srcptr x, uint32_t *hdr, uint32_t *limb, size_t M, size_t i, long L)
+{
+    if (mpfr_nan_p(x) || mpfr_inf_p(x)) return MPFFT_ERR_VALUE;
+    const uint32_t neg = mpfr_signbit(x) ? 1u : 0u;
+    if (mpfr_zero_p(x)) {
+        hdr[i] = 0x80000000u | neg;
+        for (long j = 0; j < L; j++) limb[j * M + i] = 0;
+        return MPFFT_OK;
+    }
+    const long e = mpfr_get_exp(x);
+    if (e >= kExpLimit || e <= -kExpLimit) return MPFFT_ERR_VALUE;
+    hdr[i] = ((uint32_t)e << 1) | neg;
+    const mp_limb_t *d = (const mp_limb_t *)mpfr_custom_get_significand(x);
+    const long n64 = (mpfr_get_prec(x) + 63) / 64;
+    for (long j = 0; j < L; j++) {
+        const long t = j >> 1;               // 64-bit limb index from the top
+        uint32_t w = 0;
+        if (t < n64) {
+            const mp_limb_t v = d[n64 - 1 - t];
+            w = (j & 1) ? (uint32_t)v : (uint32_t)(v >> 32);
+        }
+        limb[j * M + i] = w;
+    }
+    return MPFFT_OK;
+}
+
+void mpfr_take(mpfr_ptr x, const uint32_t *hdr, const uint32_t *limb, size_t M, size_t i, long L)
+{
+    const uint32_t h = hdr[i], neg = h & 1u;
+    if ((h & ~1u) == 0x80000000u) { mpfr_set_zero(x, neg ? -1 : 1); return; }
+    const mpfr_prec_t prec = mpfr_get_prec(x);
+    const long n64 = (prec + 63) / 64;
+    mp_limb_t *d = (mp_limb_t *)mpfr_custom_get_significand(x);
+    for (long t = 0; t < n64; t++) {
+        const long j = 2 * t;
+        const uint64_t hi = j < L ? limb[j * M + i] : 0;
+        const uint64_t lo = j + 1 < L ? limb[(j + 1) * M + i] : 0;
+        d[n64 - 1 - t] = (mp_limb_t)((hi << 32) | lo);
+    }
+    const mpfr_exp_t e = (mpfr_exp_t)((int32_t)h >> 1);
+    mpfr_custom_init_set(x, neg ? -MPFR_REGULAR_KIND : MPFR_REGULAR_KIND, e, prec, d);
+}
+
+// ---- twiddles: the reference's lut() (serial_mp_fft3.c:71-82) ---------------
+
+// r, s, d of k = 0..N/4-1 at precision P, as SoA (M = 3N/4)
+void build_mp_lut(unsigned long N, long P, long L, std::vector<uint32_t> &buf)
+{
+    const size_t Q = N / 4, M = 3 * Q;
+    buf.assign(M * (L + 1), 0);
+    uint32_t *hdr = buf.data(), *limb = hdr + M;
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt < 1) nt = 1;
+    if (nt > 16) nt = 16;
+    if (Q < 256) nt = 1;
+    auto work = [&](size_t k0, size_t k1) {
+        mpfr_t step, ang, c, s, t;
+        mpfr_inits2(P, step, ang, c, s, t, (mpfr_ptr)0);
+        // RN(RN(pi) / N): exact division by a power of two
+        mpfr_const_pi(step, MPFR_RNDN);
+        mpfr_div_ui(step, step, N, MPFR_RNDN);
+        for (size_t k = k0; k < k1; k++) {
+            // the angle is the rounded product RN(step * 2k), as in the reference
+            mpfr_mul_ui(ang, step, 2 * k, MPFR_RNDN);
+            mpfr_cos(c, ang, MPFR_RNDN);
+            mpfr_sin(s, ang, MPFR_RNDN);
+            mpfr_put(c, hdr, limb, M, k, L);
+            mpfr_add(t, c, s, MPFR_RNDN);
+            mpfr_put(t, hdr, limb, M, Q + k, L);
+            mpfr_sub(t, c, s, MPFR_RNDN);
+            mpfr_put(t, hdr, limb, M, 2 * Q + k, L);
+        }
+        mpfr_clears(step, ang, c, s, t, (mpfr_ptr)0);
+        mpfr_free_cache();
+    };
+    std::vector<std::thread> th;
+    for (unsigned i = 0; i < nt; i++)
+        th.emplace_back(work, Q * i / nt, Q * (i + 1) / nt);
+    for (auto &x : th) x.join();
+}
+
+}  // namespace
+
+// ---- plan ------------------------------------------------------------------
+
+struct mpfft_plan_s {
+    unsigned long N = 0;
+    long P = 0;
+    int form = 0;
+    unsigned batch = 1;
+    long L = 0;              // device limbs per real (mp form)
+    size_t words = 0;        // uint32 words (mp) or doubles (double forms) per transform
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+
+    // mp form
+    DevBuf lut, sn4, off4, sn2, off2, nodes;
+    LeafArgs leaf{};
+    std::vector<LevelArgs> levels;
+    // double forms
+    DevBuf tw, dleaf_sn, dleaf_off, dleaf_n, dlvl_off, dlvl_begin;
+    DTables dt{};
+    // staging for host / mpfr execution
+    DevBuf d_in, d_out;
+    std::vector<uint32_t> h_stage_in, h_stage_out;
+
+    ~mpfft_plan_s() { if (stream) (void)hipStreamDestroy(stream); }
+};
+
+namespace {
+
+int build_mp_plan(mpfft_plan_s *p)
+{
+    const unsigned long N = p->N;
+    const Tree t((uint32_t)N, 4);
+    std::vector<int32_t> sn4, sn2;
+    std::vector<uint32_t> off4, off2;
+    for (const auto &lf : t.leaves) {
+        if (lf.n == 4) { sn4.push_back(lf.sn); off4.push_back(lf.off); }
+        else if (lf.n == 2) { sn2.push_back(lf.sn); off2.push_back(lf.off); }
+    }
+    int rc;
+    if ((rc = p->sn4.upload(sn4)) || (rc = p->off4.upload(off4)) ||
+        (rc = p->sn2.upload(sn2)) || (rc = p->off2.upload(off2)))
+        return rc;
+    std::vector<uint32_t> all;
+    std::vector<size_t> first(t.level.size(), 0);
+    for (size_t l = 0; l < t.level.size(); l++) {
+        first[l] = all.size();
+        all.insert(all.end(), t.level[l].begin(), t.level[l].end());
+    }
+    if ((rc = p->nodes.upload(all))) return rc;
+    if (N >= 8) {
+        std::vector<uint32_t> lutbuf;
+        build_mp_lut(N, p->P, p->L, lutbuf);
+        if ((rc = p->lut.upload(lutbuf))) return rc;
+    }
+    LeafArgs &la = p->leaf;
+    la.sn4 = p->sn4.as<int32_t>(); la.off4 = p->off4.as<uint32_t>();
+    la.sn2 = p->sn2.as<int32_t>(); la.off2 = p->off2.as<uint32_t>();
+    la.N = (uint32_t)N; la.n4 = (uint32_t)sn4.size(); la.r4 = pad32(la.n4);
+    la.n2 = (uint32_t)sn2.size(); la.P = (int)p->P;
+    la.in_stride = la.stride = p->words;
+    p->levels.clear();
+    for (uint32_t l = 3; l <= t.lgN; l++) {
+        LevelArgs a{};
+        const uint32_t n = 1u << l, cnt = (uint32_t)t.level[l].size();
+        a.lut = p->lut.as<uint32_t>();
+        a.node_off = p->nodes.as<uint32_t>() + first[l];
+        a.stride = p->words;
+        a.N = (uint32_t)N; a.n = n; a.cnt = cnt;
+        a.nG = n >= 16 ? cnt * (n / 4 - 2) : 0; a.rG = pad32(a.nG);
+        a.nS = cnt; a.rS = pad32(cnt);
+        a.nZ = cnt;
+        a.P = (int)p->P;
+        p->levels.push_back(a);
+    }
+    return MPFFT_OK;
+}
+
+int build_d_plan(mpfft_plan_s *p)
+{
+    const unsigned long N = p->N;
+    const int code = p->form == MPFFT_FORM_D5 ? 5 : 6;
+    const Tree t((uint32_t)N, code == 5 ? 4 : 16);
+    std::vector<int32_t> lsn;
+    std::vector<uint32_t> loff, ln;
+    for (const auto &lf : t.leaves) { lsn.push_back(lf.sn); loff.push_back(lf.off); ln.push_back(lf.n); }
+    std::vector<uint32_t> all, begin(t.lgN + 2, 0);
+    for (uint32_t l = 0; l <= t.lgN; l++) {
+        begin[l] = (uint32_t)all.size();
+        all.insert(all.end(), t.level[l].begin(), t.level[l].end());
+    }
+    begin[t.lgN + 1] = (uint32_t)all.size();
+    // twiddles and constants exactly as serial_d_fft5.c:52-62 / serial_d_fft6.c:52-62
+    std::vector<double> tw(2 * (N / 4) + 1, 0.0);
+    for (unsigned long k = 0; k < N / 4; k++) {
+        tw[k] = cos(2. * M_PI * k / N);
+        tw[N / 4 + k] = sin(2. * M_PI * k / N);
+    }
+    int rc;
+    if ((rc = p->dleaf_sn.upload(lsn)) || (rc = p->dleaf_off.upload(loff)) ||
+        (rc = p->dleaf_n.upload(ln)) || (rc = p->dlvl_off.upload(all)) ||
+        (rc = p->dlvl_begin.upload(begin)) || (rc = p->tw.upload(tw)))
+        return rc;
+    DTables &d = p->dt;
+    d.leaf_sn = p->dleaf_sn.as<int32_t>(); d.leaf_off = p->dleaf_off.as<uint32_t>();
+    d.leaf_n = p->dleaf_n.as<uint32_t>(); d.nleaf = (uint32_t)lsn.size();
+    d.lvl_off = p->dlvl_off.as<uint32_t>(); d.lvl_begin = p->dlvl_begin.as<uint32_t>();
+    d.wr = p->tw.as<double>(); d.wi = p->tw.as<double>() + N / 4;
+    d.c[0] = cos(2. * M_PI * 1 / 8);
+    d.c[1] = cos(2. * M_PI * 1 / 16);
+    d.c[2] = cos(2. * M_PI * 1 / 16) - cos(2. * M_PI * 3 / 16);
+    d.c[3] = cos(2. * M_PI * 3 / 16);
+    d.c[4] = cos(2. * M_PI * 1 / 16) + cos(2. * M_PI * 3 / 16);
+    d.N = (uint32_t)N; d.lgN = t.lgN; d.lmin = code == 5 ? 3 : 5; d.code = code;
+    return MPFFT_OK;
+}
+
+// stages [s0, s1): 0 = leaves, i >= 1 = combine level of node size 2^(i+2)
+int run_mp(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t st,
+           int s0 = 0, int s1 = 1 << 30)
+{
+    const int nst = 1 + (int)p->levels.size();
+    if (s1 > nst) s1 = nst;
+    if (s0 < 0 || s0 > s1) return MPFFT_ERR_ARG;
+    if (p->N == 1) {
+        if (s0 == 0 && s1 >= 1)
+            HIP_TRY(hipMemcpyAsync(d_out, d_in, sizeof(uint32_t) * p->words * p->batch,
+                                   hipMemcpyDeviceToDevice, st));
+        return MPFFT_OK;
+    }
+    LeafArgs la = p->leaf;
+    la.in = d_in; la.data = d_out;
+    if (s0 > 0) la.n4 = la.r4 = la.n2 = 0;        // no leaf launch
+    const int l0 = s0 > 0 ? s0 - 1 : 0, l1 = s1 - 1;
+    std::vector<LevelArgs> lv(p->levels.begin() + l0, p->levels.begin() + (l1 > l0 ? l1 : l0));
+    for (auto &a : lv) a.data = d_out;
+    hipError_t e = mp_fft_launch((int)p->L, (int)p->P, la, lv.data(), (int)lv.size(),
+                                 p->batch, st);
+    return e == hipSuccess ? MPFFT_OK : MPFFT_ERR_HIP;
+}
+
+int ensure_stage(mpfft_plan_s *p)
+{
+    const size_t bytes = (p->form == MPFFT_FORM_MP ? sizeof(uint32_t) : sizeof(double)) *
+                         p->words * p->batch;
+    if (p->d_in.bytes != bytes) {
+        int rc;
+        if ((rc = p->d_in.alloc(bytes)) || (rc = p->d_out.alloc(bytes))) return rc;
+    }
+    return MPFFT_OK;
+}
+
+// cache of plans for the reference entry points, keyed by (N, P, form)
+std::mutex g_cache_mu;
+std::map<std::tuple<unsigned long, long, int>, std::unique_ptr<mpfft_plan_s>> g_cache;
+
+mpfft_plan_s *cached_plan(unsigned long N, long P, int form, int *rc)
+{
+    std::lock_guard<std::mutex> g(g_cache_mu);
+    auto key = std::make_tuple(N, P, form);
+    auto it = g_cache.find(key);
+    if (it != g_cache.end()) { *rc = MPFFT_OK; return it->second.get(); }
+    mpfft_plan pl = nullptr;
+    *rc = mpfft_plan_create(&pl, N, P, form, 1);
+    if (*rc) return nullptr;
+    g_cache[key].reset(pl);
+    return pl;
+}
+
+int legacy_mp(unsigned long n, unsigned long s, long sn, mpfr_t *inr, mpfr_t *ini,
+              mpfr_t *outr, mpfr_t *outi)
+{
+    if (!inr || !ini || !outr || !outi) return MPFFT_ERR_ARG;
+    if (s != 1 || sn != 0) return MPFFT_ERR_SUBTREE;
+    if (!is_pow2(n) || n > kMaxN) return MPFFT_ERR_SIZE;
+    const long P = (long)mpfr_get_prec(outr[0]);
+    int rc;
+    mpfft_plan_s *p = cached_plan(n, P, MPFFT_FORM_MP, &rc);
+    if (!p) return rc;
+    return mpfft_execute_mpfr(p, inr, ini, outr, outi);
+}
+
+int legacy_d(unsigned long n, unsigned long s, long sn, const double *inr, const double *ini,
+             double *outr, double *outi)
+{
+    if (!inr || !ini || !outr || !outi) return MPFFT_ERR_ARG;
+    if (s != 1 || sn != 0) return MPFFT_ERR_SUBTREE;
+    if (!is_pow2(n) || n > kMaxN) return MPFFT_ERR_SIZE;
+    int rc;
+    mpfft_plan_s *p = cached_plan(n, 0, MPFFT_FORM_D6, &rc);
+    if (!p) return rc;
+    std::vector<double> hin(2 * n), hout(2 * n);
+    memcpy(hin.data(), inr, sizeof(double) * n);
+    memcpy(hin.data() + n, ini, sizeof(double) * n);
+    if ((rc = mpfft_execute_f64_host(p, hin.data(), hout.data()))) return rc;
+    memcpy(outr, hout.data(), sizeof(double) * n);
+    memcpy(outi, hout.data() + n, sizeof(double) * n);
+    return MPFFT_OK;
+}
+
+}  // namespace
+
+// ---- C ABI -----------------------------------------------------------------
+
+extern "C" {
+
+int mpfft_last_error(void) { return g_last_error; }
+
+const char *mpfft_strerror(int code)
+{
+    switch (code) {
+    case MPFFT_OK: return "ok";
+    case MPFFT_ERR_SIZE: return "transform size is not a supported power of two";
+    case MPFFT_ERR_PREC: return "precision unsupported or inconsistent across arrays";
+    case MPFFT_ERR_VALUE: return "input holds NaN, Inf or an exponent beyond +-2^29";
+    case MPFFT_ERR_SUBTREE: return "only the top-level call (s = 1, sn = 0) is supported";
+    case MPFFT_ERR_HIP: return "HIP runtime error";
+    case MPFFT_ERR_ARG: return "bad argument";
+    case MPFFT_ERR_NODEV: return "no usable GPU";
+    case MPFFT_ERR_FORM: return "form not valid for this call";
+    default: return "unknown error";
+    }
+}
+
+long mpfft_limbs_for_precision(long prec)
+{
+    if (prec < 2 || prec > kMaxPrec) return -1;
+    return limbs_for_precision(prec);
+}
+
+int mpfft_plan_create(mpfft_plan *plan, unsigned long n, long prec, int form, unsigned int batch)
+{
+    if (!plan || batch == 0) return MPFFT_ERR_ARG;
+    *plan = nullptr;
+    if (!is_pow2(n) || n > kMaxN) return MPFFT_ERR_SIZE;
+    if (form != MPFFT_FORM_MP && form != MPFFT_FORM_D5 && form != MPFFT_FORM_D6)
+        return MPFFT_ERR_FORM;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MPFFT_ERR_NODEV;
+    std::unique_ptr<mpfft_plan_s> p(new mpfft_plan_s);
+    p->N = n; p->form = form; p->batch = batch;
+    HIP_TRY(hipGetDevice(&p->device));
+    HIP_TRY(hipStreamCreate(&p->stream));
+    int rc;
+    if (form == MPFFT_FORM_MP) {
+        if (prec < 2 || prec > kMaxPrec) return MPFFT_ERR_PREC;
+        p->P = prec;
+        p->L = limbs_for_precision(prec);
+        p->words = 2 * n * (size_t)(p->L + 1);
+        rc = build_mp_plan(p.get());
+    } else {
+        p->P = 53;
+        p->words = 2 * n;
+        rc = build_d_plan(p.get());
+    }
+    if (rc) return rc;
+    *plan = p.release();
+    return MPFFT_OK;
+}
+
+int mpfft_plan_destroy(mpfft_plan plan)
+{
+    delete plan;
+    return MPFFT_OK;
+}
+
+int mpfft_plan_info(mpfft_plan p, long *limbs, size_t *words)
+{
+    if (!p) return MPFFT_ERR_ARG;
+    if (limbs) *limbs = p->L;
+    if (words) *words = p->words;
+    return MPFFT_OK;
+}
+
+int mpfft_plan_sync(mpfft_plan p)
+{
+    if (!p) return MPFFT_ERR_ARG;
+    HIP_TRY(hipSetDevice(p->device));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return MPFFT_OK;
+}
+
+int mpfft_execute_limbs(mpfft_plan p, const uint32_t *d_in, uint32_t *d_out, void *stream)
+{
+    if (!p || !d_in || !d_out) return MPFFT_ERR_ARG;
+    if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    HIP_TRY(hipSetDevice(p->device));
+    return run_mp(p, d_in, d_out, stream ? (hipStream_t)stream : p->stream);
+}
+
+int mpfft_execute_limbs_range(mpfft_plan p, const uint32_t *d_in, uint32_t *d_out,
+                              void *stream, int stage_begin, int stage_end)
+{
+    if (!p || !d_in || !d_out) return MPFFT_ERR_ARG;
+    if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    HIP_TRY(hipSetDevice(p->device));
+    return run_mp(p, d_in, d_out, stream ? (hipStream_t)stream : p->stream,
+                  stage_begin, stage_end);
+}
+
+int mpfft_plan_stages(mpfft_plan p)
+{
+    if (!p) return MPFFT_ERR_ARG;
+    return p->form == MPFFT_FORM_MP ? 1 + (int)p->levels.size() : MPFFT_ERR_FORM;
+}
+
+int mpfft_execute_limbs_host(mpfft_plan p, const uint32_t *h_in, uint32_t *h_out)
+{
+    if (!p || !h_in || !h_out) return MPFFT_ERR_ARG;
+    if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    std::lock_guard<std::mutex> g(p->mu);
+    HIP_TRY(hipSetDevice(p->device));
+    int rc = ensure_stage(p);
+    if (rc) return rc;
+    const size_t bytes = sizeof(uint32_t) * p->words * p->batch;
+    HIP_TRY(hipMemcpyAsync(p->d_in.p, h_in, bytes, hipMemcpyHostToDevice, p->stream));
+    if ((rc = run_mp(p, p->d_in.as<uint32_t>(), p->d_out.as<uint32_t>(), p->stream))) return rc;
+    HIP_TRY(hipMemcpyAsync(h_out, p->d_out.p, bytes, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return MPFFT_OK;
+}
+
+int mpfft_mpfr_to_limbs(unsigned long n, long limbs, mpfr_t *re, mpfr_t *im, uint32_t *h_out)
+{
+    if (!re || !im || !h_out || limbs < 1) return MPFFT_ERR_ARG;
+    const size_t M = 2 * n;
+    uint32_t *hdr = h_out, *limb = h_out + M;
+    for (unsigned long k = 0; k < n; k++) {
+        int rc = mpfr_put(re[k], hdr, limb, M, k, limbs);
+        if (!rc) rc = mpfr_put(im[k], hdr, limb, M, n + k, limbs);
+        if (rc) return rc;
+    }
+    return MPFFT_OK;
+}
+
+int mpfft_limbs_to_mpfr(unsigned long n, long limbs, const uint32_t *h_in, mpfr_t *re, mpfr_t *im)
+{
+    if (!re || !im || !h_in || limbs < 1) return MPFFT_ERR_ARG;
+    const size_t M = 2 * n;
+    const uint32_t *hdr = h_in, *limb = h_in + M;
+    for (unsigned long k = 0; k < n; k++) {
+        mpfr_take(re[k], hdr, limb, M, k, limbs);
+        mpfr_take(im[k], hdr, limb, M, n + k, limbs);
+    }
+    return MPFFT_OK;
+}
+
+int mpfft_execute_mpfr(mpfft_plan p, mpfr_t *inr, mpfr_t *ini, mpfr_t *outr, mpfr_t *outi)
+{
+    if (!p || !inr || !ini || !outr || !outi) return MPFFT_ERR_ARG;
+    if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    if (p->batch != 1) return MPFFT_ERR_ARG;
+    const unsigned long n = p->N;
+    for (unsigned long k = 0; k < n; k++) {
+        if (mpfr_get_prec(inr[k]) != p->P || mpfr_get_prec(ini[k]) != p->P ||
+            mpfr_get_prec(outr[k]) != p->P || mpfr_get_prec(outi[k]) != p->P)
+            return MPFFT_ERR_PREC;
+    }
+    std::lock_guard<std::mutex> g(p->mu);
+    p->h_stage_in.resize(p->words);
+    p->h_stage_out.resize(p->words);
+    int rc = mpfft_mpfr_to_limbs(n, p->L, inr, ini, p->h_stage_in.data());
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(p->device));
+    if ((rc = ensure_stage(p))) return rc;
+    const size_t bytes = sizeof(uint32_t) * p->words;
+    HIP_TRY(hipMemcpyAsync(p->d_in.p, p->h_stage_in.data(), bytes, hipMemcpyHostToDevice, p->stream));
+    if ((rc = run_mp(p, p->d_in.as<uint32_t>(), p->d_out.as<uint32_t>(), p->stream))) return rc;
+    HIP_TRY(hipMemcpyAsync(p->h_stage_out.data(), p->d_out.p, bytes, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return mpfft_limbs_to_mpfr(n, p->L, p->h_stage_out.data(), outr, outi);
+}
+
+int mpfft_execute_f64(mpfft_plan p, const double *d_in, double *d_out, void *stream)
+{
+    if (!p || !d_in || !d_out) return MPFFT_ERR_ARG;
+    if (p->form == MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    HIP_TRY(hipSetDevice(p->device));
+    hipError_t e = d_fft_launch(p->dt, d_in, d_out, p->words, p->batch,
+                                stream ? (hipStream_t)stream : p->stream);
+    return e == hipSuccess ? MPFFT_OK : MPFFT_ERR_HIP;
+}
+
+int mpfft_execute_f64_host(mpfft_plan p, const double *h_in, double *h_out)
+{
+    if (!p || !h_in || !h_out) return MPFFT_ERR_ARG;
+    if (p->form == MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    std::lock_guard<std::mutex> g(p->mu);
+    HIP_TRY(hipSetDevice(p->device));
+    int rc = ensure_stage(p);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * p->words * p->batch;
+    HIP_TRY(hipMemcpyAsync(p->d_in.p, h_in, bytes, hipMemcpyHostToDevice, p->stream));
+    if ((rc = mpfft_execute_f64(p, p->d_in.as<double>(), p->d_out.as<double>(), p->stream)))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(h_out, p->d_out.p, bytes, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return MPFFT_OK;
+}
+
+int mpfft_mp_binop_host(int op, unsigned long count, long prec, const uint32_t *x,
+                        const uint32_t *y, uint32_t *z)
+{
+    if (!x || !y || !z || op < 0 || op > 2 || count == 0 || count > (1UL << 26))
+        return MPFFT_ERR_ARG;
+    const long L = mpfft_limbs_for_precision(prec);
+    if (L < 1) return MPFFT_ERR_PREC;
+    const size_t bytes = sizeof(uint32_t) * count * (L + 1);
+    DevBuf dx, dy, dz;
+    int rc;
+    if ((rc = dx.alloc(bytes)) || (rc = dy.alloc(bytes)) || (rc = dz.alloc(bytes))) return rc;
+    HIP_TRY(hipMemcpy(dx.p, x, bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dy.p, y, bytes, hipMemcpyHostToDevice));
+    if (mp_binop_launch((int)L, (int)prec, op, (uint32_t)count, dx.as<uint32_t>(),
+                        dy.as<uint32_t>(), dz.as<uint32_t>(), nullptr) != hipSuccess)
+        return MPFFT_ERR_HIP;
+    HIP_TRY(hipMemcpy(z, dz.p, bytes, hipMemcpyDeviceToHost));
+    return MPFFT_OK;
+}
+
+void mp_sr24_cp_fft(unsigned long n, unsigned long s, long int sn, mpfr_t *inr, mpfr_t *ini,
+                    mpfr_t *outr, mpfr_t *outi)
+{
+    g_last_error = legacy_mp(n, s, sn, inr, ini, outr, outi);
+}
+
+void mp_sr24_cp_fft_v2(unsigned long n, unsigned long log2n, unsigned long s, long int sn,
+                       mpfr_t *inr, mpfr_t *ini, mpfr_t *outr, mpfr_t *outi)
+{
+    g_last_error = log2n != 0 ? MPFFT_ERR_SUBTREE : legacy_mp(n, s, sn, inr, ini, outr, outi);
+}
+
+void d_sr24_cp_fft(unsigned long n, unsigned long s, long int sn, double *inr, double *ini,
+                   double *outr, double *outi)
+{
+    g_last_error = legacy_d(n, s, sn, inr, ini, outr, outi);
+}
+
+void d_sr24_cp_fft_v2(unsigned long n, unsigned long log2n, unsigned long s, long int sn,
+                      double *inr, double *ini, double *outr, double *outi)
+{
+    g_last_error = log2n != 0 ? MPFFT_ERR_SUBTREE : legacy_d(n, s, sn, inr, ini, outr, outi);
+}
+
+}  // extern "C"

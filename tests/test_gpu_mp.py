@@ -1,0 +1,227 @@
+"""GPU parity of the multiprecision plain form, through the C ABI.
+
+Bar: bit-exact.  Every check compares the HIP result with the reference's
+own outputs (tests/golden, made by the compiled reference) or with the
+oracle, which tests/test_oracle.py pins to them.
+"""
+import ctypes
+import hashlib
+
+import numpy as np
+import pytest
+
+import mpfft_amd as M
+from mpfft_amd import inputs as I
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+MP_SIZES = [1, 2, 4, 8, 16, 64, 1024]
+MP_PRECS = [53, 64, 128, 256, 1024]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def run_fft(n, P, x, batch=1):
+    """x: SoA-limb transform(s) at ceil(P/32) limbs; result likewise."""
+    Lo = O.limbs(P)
+    plan = M.Plan(n, P, M.FORM_MP, batch)
+    xs = np.concatenate([I.relimb(t, n, Lo, plan.L)
+                         for t in np.split(np.asarray(x, dtype=np.uint32), batch)])
+    y = plan.execute_host(xs)
+    plan.close()
+    return np.concatenate([I.relimb(t, n, plan.L, Lo) for t in np.split(y, batch)])
+
+
+# ---- device arithmetic ------------------------------------------------------
+
+def operands(count, P, seed):
+    """count reals x, y with structured exponent gaps, cancellations, ties
+    and signed zeros (SoA at ceil(P/32) limbs)."""
+    L = O.limbs(P)
+    rng = np.random.default_rng(seed)
+    x = I.mp_random(count // 2, P, seed=seed)
+    y = I.mp_random(count // 2, P, seed=seed + 1)
+    hx, lx = I.split(x, count // 2, L)
+    hy, ly = I.split(y, count // 2, L)
+    ex = hx >> 1
+    gaps = rng.choice([0, 0, 1, 2, 31, 32, 33, 64, P - 1, P, P + 1, 32 * L + 31, 32 * L + 33,
+                       3 * P], size=count)
+    gaps = gaps * rng.choice([1, -1], size=count)
+    hy[:] = (((ex - gaps) << 1) | (hy & 1)).astype(np.int32)
+    # cancellation: y = x with low bits changed
+    c = rng.random(count) < 0.15
+    ly[:, c] = lx[:, c]
+    hy[c] = hx[c] ^ rng.integers(0, 2, size=int(c.sum()), dtype=np.int32)
+    ly[L - 1, c] ^= np.uint32(1 << (32 * L - P))
+    # signed zeros
+    for h, lim in ((hx, lx), (hy, ly)):
+        z = rng.random(count) < 0.03
+        h[z] = np.int32(-2**31) | rng.integers(0, 2, size=int(z.sum()), dtype=np.int32)
+        lim[:, z] = 0
+    return x, y
+
+
+@pytest.mark.parametrize("P", [53, 64, 100, 128, 160, 256, 333, 512, 700, 1024])
+@pytest.mark.parametrize("op", [0, 1, 2])
+def test_device_ops_match_mpfr(P, op):
+    count = 8192
+    x, y = operands(count, P, seed=P * 3 + op)
+    ref = O.mp_binop(op, count, P, x, y)
+    Ld, Lo = M.limbs_for_precision(P), O.limbs(P)
+    got = M.mp_binop(op, I.relimb(x, count // 2, Lo, Ld), I.relimb(y, count // 2, Lo, Ld),
+                     count, P)
+    got = I.relimb(got, count // 2, Ld, Lo)
+    bad = np.nonzero(got != ref)[0]
+    assert bad.size == 0, f"first mismatching word {bad[:5]}"
+
+
+# ---- FFT parity with the reference's outputs --------------------------------
+
+@pytest.mark.parametrize("n", MP_SIZES)
+@pytest.mark.parametrize("P", MP_PRECS)
+def test_fft_matches_reference_golden(n, P, mp_golden, digests):
+    x = I.mp_random(n, P, seed=digests["seed"])
+    print(f"case n={n} P={P}", flush=True)
+    y = run_fft(n, P, x)
+    assert np.array_equal(y, mp_golden[f"out_N{n}_P{P}"])
+
+
+@pytest.mark.parametrize("name", ["impulse", "constant", "zeros"])
+@pytest.mark.parametrize("n", [1, 2, 4, 8, 16, 64])
+@pytest.mark.parametrize("P", [128, 1024])
+def test_fft_special_inputs(name, n, P, mp_golden):
+    y = run_fft(n, P, mp_golden[f"{name}_in_N{n}_P{P}"])
+    assert np.array_equal(y, mp_golden[f"{name}_out_N{n}_P{P}"])
+
+
+@pytest.mark.parametrize("n,P", [(64, 64), (64, 256), (64, 1024)])
+def test_fft_reference_in2_input(n, P, mp_golden):
+    y = run_fft(n, P, mp_golden[f"in2_in_N{n}_P{P}"])
+    assert np.array_equal(y, mp_golden[f"in2_out_N{n}_P{P}"])
+
+
+@pytest.mark.parametrize("n,P,seed", [(4096, 256, 1), (2048, 1024, 2), (512, 100, 3),
+                                      (256, 333, 4), (128, 700, 5), (8192, 64, 6),
+                                      (32, 32 * 24, 7), (1 << 13, 128, 8)])
+def test_fft_matches_oracle(n, P, seed):
+    x = I.mp_random(n, P, seed=seed)
+    print(f"case n={n} P={P} seed={seed}", flush=True)
+    assert np.array_equal(run_fft(n, P, x), O.mp_fft(n, P, x))
+
+
+@pytest.mark.parametrize("key", ["N65536_P256", "N65536_P1024", "N1048576_P128"])
+def test_fft_large_matches_reference_digest(key, digests):
+    n = int(key.split("_")[0][1:]); P = int(key.split("_")[1][1:])
+    x = I.mp_random(n, P, seed=digests["seed"])
+    assert sha(x) == digests["mp_large"][key]["in_sha"]
+    print(f"case {key}", flush=True)
+    assert sha(run_fft(n, P, x)) == digests["mp_large"][key]["out_sha"]
+
+
+def test_batch_equals_single_transforms():
+    n, P, b = 256, 256, 3
+    xs = [I.mp_random(n, P, seed=40 + i) for i in range(b)]
+    yb = run_fft(n, P, np.concatenate(xs), batch=b)
+    for i in range(b):
+        assert np.array_equal(np.split(yb, b)[i], O.mp_fft(n, P, xs[i]))
+
+
+def test_device_pointer_execution():
+    torch = pytest.importorskip("torch")
+    n, P, b = 1024, 256, 2
+    plan = M.Plan(n, P, M.FORM_MP, b)
+    xs = np.concatenate([I.mp_random(n, P, seed=60 + i) for i in range(b)])
+    dx = torch.from_numpy(xs.view(np.int32)).cuda()
+    dy = torch.empty_like(dx)
+    plan.execute_device(dx.data_ptr(), dy.data_ptr())
+    plan.sync()
+    y = dy.cpu().numpy().view(np.uint32)
+    for i in range(b):
+        assert np.array_equal(np.split(y, b)[i], O.mp_fft(n, P, np.split(xs, b)[i]))
+
+
+# ---- the reference entry point on mpfr_t arrays -----------------------------
+
+class MpfrArray:
+    """n mpfr_t at precision P allocated through libmpfr."""
+    SIZE = 32   # sizeof(__mpfr_struct) on x86-64
+
+    _mpfr = None
+
+    @classmethod
+    def mpfr(cls):
+        if cls._mpfr is None:
+            cls._mpfr = ctypes.CDLL("libmpfr.so.6")
+        return cls._mpfr
+
+    def __init__(self, n, P):
+        self.n = n
+        self.buf = (ctypes.c_char * (self.SIZE * max(n, 1)))()
+        self.base = ctypes.addressof(self.buf)
+        for k in range(n):
+            self.mpfr().mpfr_init2(ctypes.c_void_p(self.base + self.SIZE * k), ctypes.c_long(P))
+
+    def ptr(self, k=0):
+        return ctypes.c_void_p(self.base + self.SIZE * k)
+
+    def __del__(self):
+        for k in range(self.n):
+            self.mpfr().mpfr_clear(self.ptr(k))
+
+
+def mpfr_pair_from_soa(n, P, x):
+    re, im = MpfrArray(n, P), MpfrArray(n, P)
+    L = O.limbs(P)
+    xx = np.ascontiguousarray(x, dtype=np.uint32)
+    M.check(M.lib().mpfft_limbs_to_mpfr(n, L, ctypes.c_void_p(xx.ctypes.data),
+                                         re.ptr(), im.ptr()), "limbs_to_mpfr")
+    return re, im
+
+
+def soa_from_mpfr_pair(n, P, re, im):
+    L = O.limbs(P)
+    y = np.zeros(2 * n * (L + 1), dtype=np.uint32)
+    M.check(M.lib().mpfft_mpfr_to_limbs(n, L, re.ptr(), im.ptr(),
+                                         ctypes.c_void_p(y.ctypes.data)), "mpfr_to_limbs")
+    return y
+
+
+@pytest.mark.parametrize("entry", ["v1", "v2"])
+@pytest.mark.parametrize("n,P", [(1024, 1024), (64, 256), (16, 53)])
+def test_mpfr_entry_point_matches_reference(entry, n, P, mp_golden, digests):
+    x = I.mp_random(n, P, seed=digests["seed"])
+    inr, ini = mpfr_pair_from_soa(n, P, x)
+    outr, outi = MpfrArray(n, P), MpfrArray(n, P)
+    lib = M.lib()
+    if entry == "v1":
+        lib.mp_sr24_cp_fft(n, 1, 0, inr.ptr(), ini.ptr(), outr.ptr(), outi.ptr())
+    else:
+        lib.mp_sr24_cp_fft_v2(n, 0, 1, 0, inr.ptr(), ini.ptr(), outr.ptr(), outi.ptr())
+    assert lib.mpfft_last_error() == 0
+    y = soa_from_mpfr_pair(n, P, outr, outi)
+    assert np.array_equal(y, mp_golden[f"out_N{n}_P{P}"])
+    # the input arrays are left untouched (out-of-place)
+    assert np.array_equal(soa_from_mpfr_pair(n, P, inr, ini), x)
+
+
+def test_mpfr_entry_point_errors():
+    lib = M.lib()
+    n, P = 16, 128
+    x = I.mp_random(n, P, seed=5)
+    inr, ini = mpfr_pair_from_soa(n, P, x)
+    outr, outi = MpfrArray(n, P), MpfrArray(n, P)
+    MpfrArray.mpfr().mpfr_set_nan(inr.ptr(3))
+    lib.mp_sr24_cp_fft(n, 1, 0, inr.ptr(), ini.ptr(), outr.ptr(), outi.ptr())
+    assert lib.mpfft_last_error() == -3
+    # arrays of different precision
+    inr2, ini2 = mpfr_pair_from_soa(n, P, x)
+    out_other = MpfrArray(n, P + 64)
+    lib.mp_sr24_cp_fft(n, 1, 0, inr2.ptr(), ini2.ptr(), outr.ptr(), out_other.ptr())
+    assert lib.mpfft_last_error() == -2
+    # and a good call right after still works
+    lib.mp_sr24_cp_fft(n, 1, 0, inr2.ptr(), ini2.ptr(), outr.ptr(), outi.ptr())
+    assert lib.mpfft_last_error() == 0
+    assert np.array_equal(soa_from_mpfr_pair(n, P, outr, outi), O.mp_fft(n, P, x))

@@ -1,0 +1,191 @@
+"""The oracle (oracle/liboracle.so, a from-scratch C + MPFR restatement) is
+pinned here before anything is checked against it:
+
+* bit-exact against the golden outputs of the reference's own code
+  (tests/golden, generated from the compiled reference);
+* against the live reference build (oracle/_ref) when it is present;
+* against closed forms and invariants that do not depend on the reference
+  (impulse, constant, textbook N <= 4, brute-force DFT at 2P bits,
+  linearity, the round trip of the reference's test(), Yavne op counts).
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from mpfft_amd import inputs as I
+from oracle import oracle as O
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+MP_SIZES = [1, 2, 4, 8, 16, 64, 1024]
+MP_PRECS = [53, 64, 128, 256, 1024]
+
+
+@pytest.mark.parametrize("n", MP_SIZES)
+@pytest.mark.parametrize("P", MP_PRECS)
+def test_oracle_matches_reference_golden(n, P, mp_golden, digests):
+    x = I.mp_random(n, P, seed=digests["seed"])
+    assert sha(x) == digests["mp"][f"N{n}_P{P}"]["in_sha"], "input generator drifted"
+    y = O.mp_fft(n, P, x)
+    assert np.array_equal(y, mp_golden[f"out_N{n}_P{P}"])
+
+
+@pytest.mark.parametrize("name", ["impulse", "constant", "zeros"])
+@pytest.mark.parametrize("n", [1, 2, 4, 8, 16, 64])
+@pytest.mark.parametrize("P", [128, 1024])
+def test_oracle_special_inputs_golden(name, n, P, mp_golden):
+    x = mp_golden[f"{name}_in_N{n}_P{P}"]
+    assert np.array_equal(O.mp_fft(n, P, x), mp_golden[f"{name}_out_N{n}_P{P}"])
+
+
+@pytest.mark.parametrize("n,P", [(64, 64), (64, 256), (64, 1024)])
+def test_oracle_in2_input_and_output(n, P, mp_golden):
+    # the reference's deterministic input (ic() IN=2) and its transform
+    x = O.mp_input_in2(n, P)
+    assert np.array_equal(x, mp_golden[f"in2_in_N{n}_P{P}"])
+    assert np.array_equal(O.mp_fft(n, P, x), mp_golden[f"in2_out_N{n}_P{P}"])
+
+
+def test_oracle_in2_1024(mp_golden, digests):
+    x = O.mp_input_in2(1024, 1024)
+    assert sha(x) == digests["in2_in_sha_N1024_P1024"]
+    assert np.array_equal(O.mp_fft(1024, 1024, x), mp_golden["in2_out_N1024_P1024"])
+
+
+def test_oracle_large_digest(digests):
+    n, P = 1 << 16, 256
+    x = I.mp_random(n, P, seed=digests["seed"])
+    assert sha(x) == digests["mp_large"][f"N{n}_P{P}"]["in_sha"]
+    assert sha(O.mp_fft(n, P, x)) == digests["mp_large"][f"N{n}_P{P}"]["out_sha"]
+
+
+@pytest.mark.parametrize("n", [1024, 1 << 16])
+def test_op_counts(n, digests):
+    # reference counters (serial_mp_fft3.c:123-250) and Yavne's total
+    c = O.counts(n)
+    assert c == digests["mp_counts"][str(n)]
+    lg = n.bit_length() - 1
+    assert c[0] == n * lg - 3 * n + 4
+    assert c[1] == 3 * n * lg - 3 * n + 4
+    assert c[0] + c[1] == 4 * n * lg - 6 * n + 8
+
+
+def _mp_const(n, P, re, im):
+    """SoA transform with every element equal to re + i*im (exact dyadics)."""
+    L = O.limbs(P)
+    hdr = np.zeros(2 * n, dtype=np.int32)
+    lim = np.zeros((L, 2 * n), dtype=np.uint32)
+    for part, v in ((0, re), (1, im)):
+        sl = slice(part * n, (part + 1) * n)
+        if v == 0:
+            hdr[sl] = np.int32(-2**31)
+            continue
+        m, e = np.frexp(abs(v))
+        hdr[sl] = (int(e) << 1) | (1 if v < 0 else 0)
+        lim[0, sl] = np.uint32(int(m * 2**32))
+    return I.join(hdr, lim)
+
+
+@pytest.mark.parametrize("n", [8, 64, 1024])
+def test_impulse_gives_all_ones(n):
+    P = 256
+    x = _mp_const(n, P, 0.0, 0.0)
+    hdr, lim = I.split(x, n, O.limbs(P))
+    hdr[0] = 1 << 1
+    lim[0, 0] = 0x80000000
+    y = I.to_float(O.mp_fft(n, P, x), n, O.limbs(P))
+    assert np.all(y == 1.0)
+
+
+@pytest.mark.parametrize("n", [4, 16, 256])
+def test_constant_input(n):
+    P = 128
+    y = O.mp_fft(n, P, _mp_const(n, P, 0.75, -0.5))
+    v = I.to_float(y, n, O.limbs(P))
+    assert v[0] == n * (0.75 - 0.5j)
+    # the other bins cancel exactly apart from twiddle rounding
+    assert np.max(np.abs(v[1:])) < n * 2.0 ** -(P - 8)
+
+
+def test_textbook_small_sizes():
+    P = 64
+    for n in (1, 2, 4):
+        x = I.mp_random(n, P, seed=3)
+        v = I.to_float(x, n, O.limbs(P))
+        y = I.to_float(O.mp_fft(n, P, x), n, O.limbs(P))
+        ref = np.fft.fft(v)
+        assert np.allclose(y, ref, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("n", [8, 32, 64])
+def test_bruteforce_dft_at_2p(n):
+    """direct DFT in exact rational-free form: compare the P-bit FFT with the
+    2P-bit FFT and both with numpy's DFT to double accuracy."""
+    P = 128
+    x = I.mp_random(n, P, seed=n)
+    y = O.mp_fft(n, P, x)
+    y2 = O.mp_fft(n, P, x, Pw=2 * P)
+    maxulp, norm, zb = O.err_stats(n, P, y, O.limbs(P), y2, O.limbs(2 * P))
+    assert zb == 0 and norm < 8.0
+    v = I.to_float(x, n, O.limbs(P))
+    k = np.arange(n)
+    dft = np.exp(-2j * np.pi * np.outer(k, k) / n) @ v
+    assert np.allclose(I.to_float(y2, n, O.limbs(2 * P)), dft, rtol=0, atol=1e-12)
+
+
+def test_linearity():
+    n, P = 256, 256
+    a = I.mp_random(n, P, seed=1)
+    fa = I.to_float(O.mp_fft(n, P, a), n, O.limbs(P))
+    # doubling every input doubles every output exactly (exponent shift)
+    hdr, lim = I.split(a.copy(), n, O.limbs(P))
+    b = a.copy()
+    hb, _ = I.split(b, n, O.limbs(P))
+    nz = (hb & ~1) != -2**31
+    hb[nz] += 2
+    fb = I.to_float(O.mp_fft(n, P, b), n, O.limbs(P))
+    assert np.array_equal(fb, 2 * fa)
+
+
+def test_roundtrip_rrmse():
+    # the reference's accuracy contract: rRMSE ~ 2^-P (serial/mp/OPTIONS.md:43)
+    n, P = 1024, 256
+    x = I.mp_random(n, P, seed=2)
+    r = O.roundtrip(n, P, x, O.mp_fft(n, P, x))
+    assert np.all(r[:2] < 2.0 ** -(P - 6)) and np.all(r[:2] > 0)
+
+
+def test_planted_mistake_is_caught(mp_golden, digests):
+    """a one-ulp change anywhere in a 1024-point output must fail the
+    bit-exact comparison (the comparison is not vacuous)."""
+    n, P = 1024, 1024
+    y = O.mp_fft(n, P, I.mp_random(n, P, seed=digests["seed"])).copy()
+    L = O.limbs(P)
+    y[2 * n + (L - 1) * 2 * n + 777] ^= 1
+    assert not np.array_equal(y, mp_golden[f"out_N{n}_P{P}"])
+
+
+@pytest.mark.parametrize("n", [1, 2, 4, 8, 16, 32, 64, 1024])
+@pytest.mark.parametrize("code", [5, 6, 7])
+def test_double_oracle_golden(n, code, d_golden, digests):
+    x = I.d_random(n, seed=digests["seed"])
+    y = O.d_fft(code, n, x)
+    assert np.array_equal(y.view(np.uint64), d_golden[f"out{code}_N{n}"].view(np.uint64))
+
+
+def test_double_lut_matches_reference(d_golden):
+    wr, wi, _ = O.d_lut(1024)
+    assert np.array_equal(wr, d_golden["lutr_N1024"])
+    assert np.array_equal(wi, d_golden["luti_N1024"])
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="reference build (oracle/_ref) not present")
+def test_oracle_vs_live_reference():
+    for n, P in [(512, 200), (128, 96), (2048, 333)]:
+        x = I.mp_random(n, P, seed=n + P)
+        assert np.array_equal(O.mp_fft(n, P, x), O.ref_mp("mp3", n, P, x)[1])
+    assert np.array_equal(O.mp_lut(256, 512), O.ref_mp_lut(256, 512))
