@@ -48,6 +48,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise OSError(f"{LIB_PATH} not built: run python -m mpfft_amd.build")
+    try:
+        # torch bundles its own libamdhip64 under the same soname; loading it
+        # first makes libmpfft bind to that runtime instead of pulling a
+        # second HIP runtime into the process
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     P, U, I, L_, VP = (ctypes.c_void_p, ctypes.c_ulong, ctypes.c_int, ctypes.c_long,
                        ctypes.c_void_p)

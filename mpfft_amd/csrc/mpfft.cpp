@@ -9,6 +9,7 @@
 #include "../../include/mpfft.h"
 #include "kernels.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -205,7 +206,15 @@ namespace {
 int build_mp_plan(mpfft_plan_s *p)
 {
     const unsigned long N = p->N;
-    const Tree t((uint32_t)N, 4);
+    Tree t((uint32_t)N, 4);
+    // Order leaves by the residue class of their inputs, (sn mod s): leaf j
+    // reads x[(sn + j*s) mod N], so neighbouring lanes then gather
+    // neighbouring elements (the tree order scatters them over the array).
+    std::stable_sort(t.leaves.begin(), t.leaves.end(), [N](const Tree::Leaf &a, const Tree::Leaf &b) {
+        if (a.n != b.n) return a.n > b.n;
+        const int64_t s = (int64_t)(N / a.n);
+        return ((a.sn % s) + s) % s < ((b.sn % s) + s) % s;
+    });
     std::vector<int32_t> sn4, sn2;
     std::vector<uint32_t> off4, off2;
     for (const auto &lf : t.leaves) {
@@ -265,12 +274,14 @@ int build_d_plan(mpfft_plan_s *p)
         all.insert(all.end(), t.level[l].begin(), t.level[l].end());
     }
     begin[t.lgN + 1] = (uint32_t)all.size();
-    // twiddles and constants exactly as serial_d_fft5.c:52-62 / serial_d_fft6.c:52-62
+    // twiddles and constants as serial_d_fft5.c:52-62 / serial_d_fft6.c:52-62.
+    // The reference's documented build (gcc -O3) fuses its cos(x) and sin(x)
+    // of one argument into a single glibc sincos(x) call, whose sine differs
+    // from glibc's sin(x) by one ulp for some angles; calling sincos here
+    // reproduces that build's table bit for bit whatever this compiler fuses.
     std::vector<double> tw(2 * (N / 4) + 1, 0.0);
-    for (unsigned long k = 0; k < N / 4; k++) {
-        tw[k] = cos(2. * M_PI * k / N);
-        tw[N / 4 + k] = sin(2. * M_PI * k / N);
-    }
+    for (unsigned long k = 0; k < N / 4; k++)
+        sincos(2. * M_PI * k / N, &tw[N / 4 + k], &tw[k]);
     int rc;
     if ((rc = p->dleaf_sn.upload(lsn)) || (rc = p->dleaf_off.upload(loff)) ||
         (rc = p->dleaf_n.upload(ln)) || (rc = p->dlvl_off.upload(all)) ||

@@ -27,6 +27,7 @@
  * limb[j*M+i], j=0 most significant.  hdr = (exp<<1)|neg for a nonzero value
  * 0.1b..b * 2^exp (MPFR convention), hdr = INT32_MIN|neg for a signed zero.
  */
+#define _GNU_SOURCE
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -545,13 +546,15 @@ void orc_mp_counts(unsigned long N, unsigned long *c)
 /* double path: CODE 5 (serial_d_fft5.c) and CODE 6/7 (serial_d_fft6.c) */
 /* ------------------------------------------------------------------ */
 
-/* LUT and fixed constants: serial_d_fft5.c:52-62 / serial_d_fft6.c:52-62 */
+/* LUT and fixed constants: serial_d_fft5.c:52-62 / serial_d_fft6.c:52-62.
+ * gcc -O3 (the reference's build) turns the cos(x), sin(x) pair of one
+ * argument into one glibc sincos(x) call, and glibc's sincos sine differs
+ * from its sin(x) by one ulp for some angles: call sincos explicitly so the
+ * table matches that build whatever the compiler fuses. */
 void orc_d_lut(unsigned long N, double *wr, double *wi, double *cst)
 {
-    for (unsigned long k = 0; k < N / 4; k++) {
-        wr[k] = cos(2. * M_PI * k / N);
-        wi[k] = sin(2. * M_PI * k / N);
-    }
+    for (unsigned long k = 0; k < N / 4; k++)
+        sincos(2. * M_PI * k / N, &wi[k], &wr[k]);
     cst[0] = cos(2. * M_PI * 1 / 8);
     cst[1] = cos(2. * M_PI * 1 / 16);
     cst[2] = cos(2. * M_PI * 1 / 16) - cos(2. * M_PI * 3 / 16);
