@@ -22,8 +22,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # MI355X published FP32 vector peak 157.3 TFLOP/s = CUs * SIMDs * lanes * clock
-# * 2 (FMA) * 2 (packed): the full-rate 32-bit VALU lane-instruction rate is
-# a quarter of it.
+# * 2 (FMA) * 2 (packed): the full-rate VALU lane-instruction rate is a
+# quarter of it.  v_mad_u64_u32 (one 32x32->64 product, plus a 64-bit
+# addend) issues at that full rate on gfx950 (mpfft_amd/valu_probe).
 VALU_LANE_OPS = 157.3e12 / 4
 
 
@@ -110,8 +111,8 @@ def main():
     prod_per_fft = cmul(n) * Lc * Lc
     lvl_s_per_step = r["lvl_ms"] * 1e-3 / a.steps
     achieved = prod_per_fft * B / lvl_s_per_step / 1e12          # T products / s
-    # peak: one v_mad_u64_u32 per product at the measured quarter rate
-    peak = VALU_LANE_OPS / 4 / 1e12
+    # peak: one v_mad_u64_u32 per 32x32->64 product at the full VALU rate
+    peak = VALU_LANE_OPS / 1e12
     bytes_fft = (4 * n + 3 * n // 4) * (4 * Lc + 4)
     line = {
         "metric": "mp-FFTs/s at N=2^16 @256/1024 bit, chosen vs plain GPU form at matched ulp error",

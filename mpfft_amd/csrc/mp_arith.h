@@ -59,18 +59,34 @@ MPFFT_FN uint32_t clz32(uint32_t v)   // v != 0
 #endif
 }
 
+// 32-bit add / subtract with carry (borrow) in and out: one
+// v_addc_co_u32 / v_subb_co_u32 each in a chain
 MPFFT_FN uint32_t add_cc(uint32_t a, uint32_t b, uint32_t &c)
 {
+#ifdef MPFFT_CPU_BUILD
     uint64_t s = (uint64_t)a + b + c;
     c = (uint32_t)(s >> 32);
     return (uint32_t)s;
+#else
+    unsigned int co;
+    const uint32_t r = __builtin_addc(a, b, c, &co);
+    c = co;
+    return r;
+#endif
 }
 
 MPFFT_FN uint32_t sub_bb(uint32_t a, uint32_t b, uint32_t &br)
 {
+#ifdef MPFFT_CPU_BUILD
     uint64_t s = (uint64_t)a - b - br;
     br = (uint32_t)(s >> 63);
     return (uint32_t)s;
+#else
+    unsigned int bo;
+    const uint32_t r = __builtin_subc(a, b, br, &bo);
+    br = bo;
+    return r;
+#endif
 }
 
 // Round the normalized window w[0..L] (w[0] has its top bit set) plus a
@@ -85,9 +101,12 @@ MPFFT_FN void round_window(uint32_t (&w)[L + 1], uint32_t sticky,
         uint32_t rest = (w[L] << 1) | sticky;
         uint32_t up = rbit & ((rest != 0) | (w[L - 1] & 1));
         uint32_t c = up;
+        w[L - 1] = add_cc(w[L - 1], 0, c);
+        if (c) {   // the increment carries past the last limb: rare
 #pragma unroll
-        for (int j = L - 1; j >= 0; j--) w[j] = add_cc(w[j], 0, c);
-        if (c) { w[0] = 0x80000000u; e += 1; }
+            for (int j = L - 2; j >= 0; j--) w[j] = add_cc(w[j], 0, c);
+            if (c) { w[0] = 0x80000000u; e += 1; }
+        }
     } else {
         // last kept bit: index P-1 from the top -> limb kl, bit kb (from LSB)
         const int kl = (P - 1) >> 5;
@@ -126,18 +145,16 @@ MPFFT_FN void set_zero(Mp<L> &z, uint32_t neg)
 }
 
 // z = (-1)^fx * x + (-1)^fy * y, correctly rounded to P bits.
+// Zeros go through the same code as data (a zero operand has zero limbs and
+// is never the larger-exponent operand unless both are zero); a zero result
+// takes the sign xs & ys, which gives x - x = +0 and (-0) + (-0) = -0.
 template <int L, bool FULL>
 MPFFT_FN void mp_add(Mp<L> &z, const Mp<L> &x, uint32_t fx,
                      const Mp<L> &y, uint32_t fy, int P)
 {
     const uint32_t xs = x.neg ^ fx, ys = y.neg ^ fy;
-    if (x.zero | y.zero) {
-        if (x.zero & y.zero) { set_zero(z, xs & ys); return; }
-        if (x.zero) { z = y; z.neg = ys; return; }
-        z = x; z.neg = xs; return;
-    }
-    // a = operand with the larger exponent, b = the other one
-    const bool sw = y.e > x.e;
+    // a = the operand with the larger exponent, b = the other one
+    const bool sw = x.zero || (!y.zero && y.e > x.e);
     uint32_t aw[L + 1], bw[L + 1];
 #pragma unroll
     for (int j = 0; j < L; j++) {
@@ -147,7 +164,8 @@ MPFFT_FN void mp_add(Mp<L> &z, const Mp<L> &x, uint32_t fx,
     aw[L] = 0; bw[L] = 0;
     int32_t e = sw ? y.e : x.e;
     const uint32_t sa = sw ? ys : xs, sb = sw ? xs : ys;
-    const uint32_t d = (uint32_t)(sw ? (y.e - x.e) : (x.e - y.e));
+    const bool bzero = sw ? x.zero : y.zero;
+    const uint32_t d = bzero ? 0u : (uint32_t)(sw ? (y.e - x.e) : (x.e - y.e));
 
     // align b: shift right by d bits into the (L+1)-limb window, OR-ing
     // everything that falls off the window into sticky
@@ -158,6 +176,7 @@ MPFFT_FN void mp_add(Mp<L> &z, const Mp<L> &x, uint32_t fx,
         sticky = 1;
     } else {
         const uint32_t q = d >> 5, r = d & 31u;
+#pragma unroll 1
         for (uint32_t t = 0; t < q; t++) {
             sticky |= bw[L];
 #pragma unroll
@@ -198,11 +217,13 @@ MPFFT_FN void mp_add(Mp<L> &z, const Mp<L> &x, uint32_t fx,
             for (int j = L; j >= 0; j--) aw[j] = add_cc(~aw[j], 0, c);
             sign ^= 1u;
         }
-        uint32_t any = sticky;
+    }
+    uint32_t any = sticky;
 #pragma unroll
-        for (int j = 0; j <= L; j++) any |= aw[j];
-        if (!any) { set_zero(z, 0); return; }    // exact cancellation: +0
-        // normalize (more than one bit only when d <= 1, i.e. exact)
+    for (int j = 0; j <= L; j++) any |= aw[j];
+    if (any) {
+        // normalize (more than one bit only after an exact subtraction)
+#pragma unroll 1
         while (aw[0] == 0) {
 #pragma unroll
             for (int j = 0; j < L; j++) aw[j] = aw[j + 1];
@@ -216,17 +237,103 @@ MPFFT_FN void mp_add(Mp<L> &z, const Mp<L> &x, uint32_t fx,
             aw[L] <<= lz;
             e -= (int32_t)lz;
         }
+        round_window<L, FULL>(aw, sticky, e, P);
     }
-    round_window<L, FULL>(aw, sticky, e, P);
 #pragma unroll
     for (int j = 0; j < L; j++) z.m[j] = aw[j];
+    z.e = e;
+    z.zero = any ? 0u : 1u;
+    z.neg = any ? sign : (xs & ys);
+}
+
+// ---- multiplication ----------------------------------------------------
+//
+// The significands are re-cut into 29-bit digits so that a whole product
+// column  sum_i A_i * B_(c-i)  accumulates in one 64-bit register with
+// v_mad_u64_u32 and no carry handling: a column has at most D <= 63 terms
+// below 2^58 each plus a carry-in below 2^35, which stays below 2^64.  One
+// mad per digit product; the carry moves on once per column.
+
+constexpr int kDigitBits = 29;
+constexpr uint32_t kDigitMask = (1u << kDigitBits) - 1u;
+
+template <int L>
+struct Digits {
+    static constexpr int D = (32 * L + kDigitBits - 1) / kDigitBits;
+    static_assert(D <= 63, "column sums would overflow 64 bits");
+};
+
+// MS-first 32-bit limbs -> LS-first 29-bit digits
+template <int L>
+MPFFT_FN void to_digits(const uint32_t (&m)[L], uint32_t (&d)[Digits<L>::D])
+{
+#pragma unroll
+    for (int k = 0; k < Digits<L>::D; k++) {
+        const int bit = kDigitBits * k, w = bit >> 5, s = bit & 31;
+        const uint32_t lo = w < L ? m[L - 1 - w] : 0u;
+        const uint32_t hi = w + 1 < L ? m[L - 2 - w] : 0u;
+        d[k] = (s == 0 ? lo : fshr(hi, lo, (uint32_t)s)) & kDigitMask;
+    }
+}
+
+// Deposit digit c (value v, bits [29c, 29c+29) of the 64L-bit product) into
+// the window w[0..L] (product bits [32(L-1), 64L), MS-first) or into sticky.
+template <int L>
+MPFFT_FN void put_digit(uint32_t (&w)[L + 1], uint32_t &sticky, int c, uint32_t v)
+{
+    const int bit = kDigitBits * c, j = bit >> 5, s = bit & 31;
+    // low part -> product word j, high part -> word j + 1
+    const uint32_t lo = v << s;
+    if (j < L - 1) sticky |= lo;
+    else if (j < 2 * L) w[2 * L - 1 - j] |= lo;
+    if (s + kDigitBits > 32) {
+        const uint32_t hi = v >> (32 - s);
+        if (j + 1 < L - 1) sticky |= hi;
+        else if (j + 1 < 2 * L) w[2 * L - 2 - j] |= hi;
+    }
+}
+
+// z = x * y, correctly rounded to P bits.
+template <int L, bool FULL>
+MPFFT_FN void mp_mul(Mp<L> &z, const Mp<L> &x, const Mp<L> &y, int P)
+{
+    constexpr int D = Digits<L>::D;
+    const uint32_t sign = x.neg ^ y.neg;
+    if (x.zero | y.zero) { set_zero(z, sign); return; }
+    uint32_t A[D], B[D];
+    to_digits<L>(x.m, A);
+    to_digits<L>(y.m, B);
+    uint32_t w[L + 1];
+#pragma unroll
+    for (int j = 0; j <= L; j++) w[j] = 0;
+    uint32_t sticky = 0;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int c = 0; c < 2 * D - 1; c++) {
+        const int i0 = c - (D - 1) > 0 ? c - (D - 1) : 0;
+        const int i1 = c < D - 1 ? c : D - 1;
+#pragma unroll
+        for (int i = i0; i <= i1; i++) acc = (uint64_t)A[i] * B[c - i] + acc;
+        put_digit<L>(w, sticky, c, (uint32_t)acc & kDigitMask);
+        acc >>= kDigitBits;
+    }
+    put_digit<L>(w, sticky, 2 * D - 1, (uint32_t)acc);
+    int32_t e = x.e + y.e;
+    if (!(w[0] >> 31)) {
+#pragma unroll
+        for (int j = 0; j < L; j++) w[j] = fshr(w[j], w[j + 1], 31);
+        w[L] <<= 1;
+        e -= 1;
+    }
+    round_window<L, FULL>(w, sticky != 0, e, P);
+#pragma unroll
+    for (int j = 0; j < L; j++) z.m[j] = w[j];
     z.e = e; z.neg = sign; z.zero = 0;
 }
 
-// z = x * y, correctly rounded to P bits (schoolbook product, column-wise
-// from the least significant column; the low L-1 columns only feed sticky).
+// Reference schoolbook product on 32-bit limbs (kept for cross-checks).
 template <int L, bool FULL>
-MPFFT_FN void mp_mul(Mp<L> &z, const Mp<L> &x, const Mp<L> &y, int P)
+MPFFT_FN void mp_mul_limb32(Mp<L> &z, const Mp<L> &x, const Mp<L> &y, int P)
 {
     const uint32_t sign = x.neg ^ y.neg;
     if (x.zero | y.zero) { set_zero(z, sign); return; }
@@ -284,9 +391,10 @@ MPFFT_FN void mp_load(Mp<L> &x, const uint32_t *__restrict__ buf,
     x.neg = (uint32_t)h & 1u;
     x.zero = (h & ~1) == kHdrZero;
     x.e = h >> 1;
-    const uint32_t *lb = buf + M;
+    // limb j of every element starts at the uniform address buf + (j+1)*M;
+    // the per-lane part is the 32-bit index i (saddr + voffset addressing)
 #pragma unroll
-    for (int j = 0; j < L; j++) x.m[j] = lb[(size_t)j * M + i];
+    for (int j = 0; j < L; j++) x.m[j] = (buf + (size_t)(j + 1) * M)[i];
 }
 
 template <int L>
@@ -294,9 +402,8 @@ MPFFT_FN void mp_store(const Mp<L> &x, uint32_t *__restrict__ buf,
                        uint32_t M, uint32_t i)
 {
     buf[i] = x.zero ? ((uint32_t)kHdrZero | x.neg) : (((uint32_t)x.e << 1) | x.neg);
-    uint32_t *lb = buf + M;
 #pragma unroll
-    for (int j = 0; j < L; j++) lb[(size_t)j * M + i] = x.zero ? 0u : x.m[j];
+    for (int j = 0; j < L; j++) (buf + (size_t)(j + 1) * M)[i] = x.zero ? 0u : x.m[j];
 }
 
 #ifndef MPFFT_CPU_BUILD

@@ -30,53 +30,63 @@ __device__ __forceinline__ uint32_t wrap_idx(int32_t sn, uint32_t s, uint32_t j,
     return (uint32_t)(v < 0 ? v + N : v);
 }
 
+// The kernels below run each of their additions (and the level kernel each
+// of its multiplications) through one instance of mp_add / mp_mul inside a
+// loop whose trip count is uniform across the wave; operands are selected
+// per iteration.  Inlining every operation separately would give ~200 KB of
+// code at 1024 bits, far more than the instruction cache holds.
+
+// Leaves run in two passes; each iteration is "load two operands, add,
+// store", so no multi-limb value lives across iterations.
+//   pass 0, leaf of 4 (serial_mp_fft3.c:154-164):  y0 = x0 + x2,
+//     y1 = x0 - x2, y2 = a1 = x1 + x3, and a2 = x1 - x3 stored in the slot
+//     y3 of the OTHER component (the real part of y3 needs a2.im, the
+//     imaginary part a2.re, :168-174);
+//   pass 0, leaf of 2 (:141-145): y0 = x0 + x1, y1 = x0 - x1;
+//   pass 1, leaf of 4 (:166-174): y2 = y0 - a1, y0 = y0 + a1,
+//     y3 = y1 -+ a2', y1 = y1 +- a2'.
+// Every lane reads and writes only its own component's slots within a pass.
 template <int L, bool FULL>
 __global__ void __launch_bounds__(256)
-mp_leaf_kernel(LeafArgs a)
+mp_leaf_kernel(LeafArgs a, int pass)
 {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t pair = tid >> 1, B = tid & 1u;   // B: 0 real part, 1 imag part
     const uint32_t *in = a.in + a.in_stride * blockIdx.y;
     uint32_t *out = a.data + a.stride * blockIdx.y;
-    const uint32_t N = a.N, M = 2 * N, base = B ? N : 0;
+    const uint32_t N = a.N, M = 2 * N, base = B ? N : 0, other = B ? 0 : N;
     const int P = a.P;
-    Mp<L> x0, x1, e0, e1;
-    if (pair < a.r4) {
-        // size-4 leaf (serial_mp_fft3.c:149-177), stride s = N/4
-        if (pair >= a.n4) return;
-        const int32_t sn = a.sn4[pair];
-        const uint32_t o = a.off4[pair], s = N >> 2;
-        mp_load(x0, in, M, base + wrap_idx(sn, s, 0, N));
-        mp_load(x1, in, M, base + wrap_idx(sn, s, 2, N));
-        mp_add<L, FULL>(e0, x0, 0, x1, 0, P);
-        mp_add<L, FULL>(e1, x0, 0, x1, 1, P);
-        mp_load(x0, in, M, base + wrap_idx(sn, s, 1, N));
-        mp_load(x1, in, M, base + wrap_idx(sn, s, 3, N));
-        Mp<L> a1, a2, pa2;
-        mp_add<L, FULL>(a1, x0, 0, x1, 0, P);
-        mp_add<L, FULL>(a2, x0, 0, x1, 1, P);
-        // the real lane needs a2.im, the imaginary lane a2.re
-        mp_swap_pair(pa2, a2);
-        mp_add<L, FULL>(x0, e0, 0, a1, 1, P);
-        mp_store(x0, out, M, base + o + 2);
-        mp_add<L, FULL>(x0, e0, 0, a1, 0, P);
-        mp_store(x0, out, M, base + o);
-        mp_add<L, FULL>(x0, e1, 0, pa2, B ^ 1u, P);
-        mp_store(x0, out, M, base + o + 3);
-        mp_add<L, FULL>(x0, e1, 0, pa2, B, P);
-        mp_store(x0, out, M, base + o + 1);
-    } else {
-        // size-2 leaf (serial_mp_fft3.c:136-148), stride s = N/2
-        const uint32_t q = pair - a.r4;
-        if (q >= a.n2) return;
-        const int32_t sn = a.sn2[q];
-        const uint32_t o = a.off2[q], s = N >> 1;
-        mp_load(x0, in, M, base + wrap_idx(sn, s, 0, N));
-        mp_load(x1, in, M, base + wrap_idx(sn, s, 1, N));
-        mp_add<L, FULL>(e0, x0, 0, x1, 0, P);
-        mp_store(e0, out, M, base + o);
-        mp_add<L, FULL>(e1, x0, 0, x1, 1, P);
-        mp_store(e1, out, M, base + o + 1);
+    // size-4 leaves (input stride N/4) in pairs [0, r4), size-2 leaves
+    // (stride N/2) after them; pass 1 covers the size-4 leaves only
+    const bool four = pair < a.r4;
+    const uint32_t q = four ? pair : pair - a.r4;
+    if (q >= (four ? a.n4 : a.n2) || (pass && !four)) return;
+    const int32_t sn = four ? a.sn4[q] : a.sn2[q];
+    const uint32_t o = four ? a.off4[q] : a.off2[q];
+    const uint32_t s = four ? N >> 2 : N >> 1;
+    Mp<L> u, v, r;
+    const int nit = four ? 4 : 2;
+#pragma unroll 1
+    for (int it = 0; it < nit; it++) {
+        uint32_t fy = (uint32_t)(it & 1), dst = base + o + (uint32_t)it;
+        if (pass == 0) {
+            if ((it & 1) == 0) {
+                const uint32_t j0 = it ? 1u : 0u, j1 = four ? (it ? 3u : 2u) : 1u;
+                mp_load(u, in, M, base + wrap_idx(sn, s, j0, N));
+                mp_load(v, in, M, base + wrap_idx(sn, s, j1, N));
+            }
+            if (it == 3) dst = other + o + 3;
+        } else {
+            if ((it & 1) == 0) {
+                mp_load(u, out, M, base + o + (it ? 1u : 0u));
+                mp_load(v, out, M, base + o + (it ? 3u : 2u));
+            }
+            // y2 = y0 - a1, y0 = y0 + a1, y3 = y1 -+ a2', y1 = y1 +- a2'
+            dst = base + o + (it == 0 ? 2u : it == 1 ? 0u : it == 2 ? 3u : 1u);
+            fy = it == 0 ? 1u : it == 1 ? 0u : it == 2 ? (B ^ 1u) : B;
+        }
+        mp_add<L, FULL>(r, u, 0, v, fy, P);
+        mp_store(r, out, M, dst);
     }
 }
 
@@ -114,66 +124,80 @@ mp_level_kernel(LevelArgs a)
     const uint32_t ui = o + k + (B ? q : 0);          // U0 or U1
     const uint32_t kn = k * (N / n);
 
+    // o1, o2: m1 = Z+ * w^k on the Z+ lane, m2 = Z- * conj(w^k) on the Z- lane
     Mp<L> o1, o2;
-    {
-        Mp<L> X, Y;
-        mp_load(X, out, M, zi);
-        mp_load(Y, out, M, N + zi);
-        if (kind == 0) {
-            // 3m3a (serial_mp_fft3.c:220-229).  Z+ lane: m1 = Z+ * (wr - i wi);
-            // Z- lane: m2 = Z- * (wr + i wi).
-            Mp<L> t, T, W, Pr;
-            mp_add<L, FULL>(t, X, 0, Y, B, P);          // Zr +- Zi
-            mp_load(W, a.lut, ML, kn);                  // LUTr
-            mp_mul<L, FULL>(T, t, W, P);
-            mp_load(W, a.lut, ML, 2 * Q + kn);          // LUTd
-            mp_mul<L, FULL>(Pr, W, Y, P);
-            mp_add<L, FULL>(o1, T, 0, Pr, B ^ 1u, P);   // m1r = T - d Zi | m2r = T + d Zi
-            mp_load(W, a.lut, ML, Q + kn);              // LUTs
-            mp_mul<L, FULL>(Pr, W, X, P);
-            mp_add<L, FULL>(o2, T, B, Pr, B ^ 1u, P);   // m1i = T - s Zr | m2i = s Zr - T
-        } else if (kind == 1) {
-            // k = n/8: w = (1 - i) c (serial_mp_fft3.c:209-217)
-            Mp<L> t, W;
-            mp_load(W, a.lut, ML, kn);
-            mp_add<L, FULL>(t, X, 0, Y, B, P);          // Zr + Zi | Zr - Zi
-            mp_mul<L, FULL>(o1, t, W, P);
-            mp_add<L, FULL>(t, X, B ^ 1u, Y, 0, P);     // Zi - Zr | Zr + Zi
-            mp_mul<L, FULL>(o2, t, W, P);
-        } else {
-            o1 = X;                                     // k = 0: m = Z
-            o2 = Y;
+    mp_load(o1, out, M, zi);
+    mp_load(o2, out, M, N + zi);
+    if (kind != 2) {
+        // pre-additions.  general k (3m3a, serial_mp_fft3.c:220-229):
+        //   t = Zr +- Zi.  k = n/8 (:209-217): Zr +- Zi and -+Zr + Zi.
+        Mp<L> r, T;
+        const int npre = kind == 0 ? 1 : 2;
+#pragma unroll 1
+        for (int it = 0; it < npre; it++) {
+            mp_add<L, FULL>(r, o1, it ? (B ^ 1u) : 0u, o2, it ? 0u : B, P);
+            if (it == 0) T = r;
+            else o2 = r;
+        }
+        if (kind == 1) o1 = T;
+        // products.  general k: T = t*LUTr; m.r = T -+ LUTd*Zi;
+        //   m.i = T - LUTs*Zr (Z+ lane) or LUTs*Zr - T (Z- lane).
+        //   k = n/8: both pre-sums times LUTr[N/8].
+        const int nmul = kind == 0 ? 3 : 2;
+#pragma unroll 1
+        for (int it = 0; it < nmul; it++) {
+            Mp<L> f, w;
+            uint32_t li = kn;
+            if (kind == 0) {
+                if (it == 0) f = T;
+                else mp_load(f, out, M, it == 1 ? N + zi : zi);
+                li = it == 0 ? kn : (it == 1 ? 2 * Q + kn : Q + kn);
+            } else {
+                f = it == 0 ? o1 : o2;
+            }
+            mp_load(w, a.lut, ML, li);
+            mp_mul<L, FULL>(r, f, w, P);
+            if (kind == 1) {
+                if (it == 0) o1 = r;
+                else o2 = r;
+            } else if (it == 0) {
+                T = r;
+            } else {
+                Mp<L> m;
+                mp_add<L, FULL>(m, T, it == 2 ? B : 0u, r, B ^ 1u, P);
+                if (it == 1) o1 = m;
+                else o2 = m;
+            }
         }
     }
-    // pair exchange: Z+ lane gets m2, Z- lane gets m1
-    Mp<L> p1, p2, v1, v2;
-    mp_swap_pair(p1, o1);
-    mp_swap_pair(p2, o2);
-    // Z+ lane: a1 = m1 + m2; Z- lane: a2 = m1 - m2 (serial_mp_fft3.c:234-238)
-    mp_add<L, FULL>(v1, o1, B, p1, 0, P);
-    mp_add<L, FULL>(v2, o2, B, p2, 0, P);
+    // pair exchange: the Z+ lane gets m2, the Z- lane m1; then
+    // Z+ lane: a1 = m1 + m2, Z- lane: a2 = m1 - m2 (serial_mp_fft3.c:234-238)
+    Mp<L> v1, v2;
+#pragma unroll 1
+    for (int it = 0; it < 2; it++) {
+        Mp<L> mine, other;
+        mp_select(mine, (uint32_t)it, o2, o1);
+        mp_swap_pair(other, mine);
+        Mp<L> r;
+        mp_add<L, FULL>(r, mine, B, other, 0, P);
+        if (it == 0) v1 = r;
+        else v2 = r;
+    }
     // outputs (serial_mp_fft3.c:240-248):
     //   Z+ lane: X[k] = U0 + a1, X[k+n/2] = U0 - a1
     //   Z- lane: X[k+n/4] = U1 - i a2, X[k+3n/4] = U1 + i a2
-    Mp<L> U, R;
-    const uint32_t f = o + k + (B ? q : 0), s = f + 2 * q;
-    mp_load(U, out, M, ui);
-    {
-        Mp<L> al;
-        mp_select(al, B, v2, v1);
-        mp_add<L, FULL>(R, U, 0, al, 0, P);
-        mp_store(R, out, M, f);
-        mp_add<L, FULL>(R, U, 0, al, 1, P);
-        mp_store(R, out, M, s);
-    }
-    mp_load(U, out, M, N + ui);
-    {
-        Mp<L> be;
-        mp_select(be, B, v1, v2);
-        mp_add<L, FULL>(R, U, 0, be, B, P);
-        mp_store(R, out, M, N + f);
-        mp_add<L, FULL>(R, U, 0, be, B ^ 1u, P);
-        mp_store(R, out, M, N + s);
+    const uint32_t f0 = o + k + (B ? q : 0), s0 = f0 + 2 * q;
+    Mp<L> U;
+#pragma unroll 1
+    for (int it = 0; it < 4; it++) {
+        const uint32_t im = it >> 1;                 // 0: real parts, 1: imaginary parts
+        if ((it & 1) == 0) mp_load(U, out, M, (im ? N : 0) + ui);
+        Mp<L> x, r;
+        // real: U.r +- (a1.r | a2.i); imaginary: U.i +- (a1.i | -a2.r)
+        mp_select(x, B ^ im, v2, v1);
+        const uint32_t fy = (uint32_t)(it & 1) ^ (im & B);
+        mp_add<L, FULL>(r, U, 0, x, fy, P);
+        mp_store(r, out, M, (im ? N : 0) + ((it & 1) ? s0 : f0));
     }
 }
 
@@ -205,7 +229,11 @@ static hipError_t launch_fft(const LeafArgs &la, const LevelArgs *lv, int nlev,
         uint32_t thr = 2 * (la.r4 + la.n2);
         if (thr) {
             dim3 grid((thr + 255) / 256, batch);
-            hipLaunchKernelGGL((mp_leaf_kernel<L, FULL>), grid, dim3(256), 0, st, la);
+            hipLaunchKernelGGL((mp_leaf_kernel<L, FULL>), grid, dim3(256), 0, st, la, 0);
+            if (la.n4) {
+                dim3 grid4((2 * la.r4 + 255) / 256, batch);
+                hipLaunchKernelGGL((mp_leaf_kernel<L, FULL>), grid4, dim3(256), 0, st, la, 1);
+            }
         }
     }
     for (int i = 0; i < nlev; i++) {

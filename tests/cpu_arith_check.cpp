@@ -69,11 +69,17 @@ static void rand_mant(Mp<L> &x, int P)
         default: x.m[j] = rnd32(); break;
         }
     }
-    if (kind == 0 && (rnd32() & 1)) x.m[L - 1] = 1u << (32 * L - P);   // 1000...01
     x.m[0] |= 0x80000000u;
-    // clear bits below position P
-    int W = 32 * L - P;
-    if (W) x.m[L - 1] &= ~((1u << W) - 1u);
+    // clear bits below position P (the padding may span several limbs)
+    for (int j = 0; j < L; j++) {
+        const int keep = P - 32 * j;                 // bits of limb j that are kept
+        if (keep <= 0) x.m[j] = 0;
+        else if (keep < 32) x.m[j] &= ~((1u << (32 - keep)) - 1u);
+    }
+    if (kind == 0 && (rnd32() & 1)) {                // 1000...01
+        const int b = P - 1;
+        x.m[b >> 5] |= 0x80000000u >> (b & 31);
+    }
 }
 
 template <int L>
@@ -172,8 +178,8 @@ template <int L>
 static int run_L(long trials)
 {
     int fail = 0;
-    int Ps[4] = {32 * L, 32 * (L - 1) + 1, 32 * L - 11, 32 * L - 1};
-    for (int pi = 0; pi < 4; pi++) {
+    int Ps[5] = {32 * L, 32 * (L - 1) + 1, 32 * L - 11, 32 * L - 1, 32 * (L - 2) + 7};
+    for (int pi = 0; pi < 5; pi++) {
         int P = Ps[pi];
         if (P < 2 || (pi > 0 && P == Ps[0])) continue;
         for (int op = 0; op < 2; op++) {
@@ -193,8 +199,11 @@ int main(int argc, char **argv)
     fail |= run_L<2>(trials);
     fail |= run_L<3>(trials);
     fail |= run_L<4>(trials);
+    fail |= run_L<6>(trials);
     fail |= run_L<8>(trials);
-    fail |= run_L<16>(trials);
+    fail |= run_L<12>(trials / 2 + 1);
+    fail |= run_L<16>(trials / 2 + 1);
+    fail |= run_L<24>(trials / 4 + 1);
     fail |= run_L<32>(trials / 4 + 1);
     printf(fail ? "FAIL\n" : "PASS\n");
     return fail;
