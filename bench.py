@@ -89,6 +89,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0x6D7046465420)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-err", action="store_true", help="skip the 2P-bit oracle error")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (reference kernel for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -100,6 +103,12 @@ def main():
     from mpfft_amd import inputs as I
 
     n, P, B = a.n, a.prec, a.batch
+    if a.calib_copy_mib:
+        src = torch.empty(a.calib_copy_mib << 18, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     r = measure(M, I, torch, n, P, B, a.steps, a.warmup, a.seed)
     ms_step = r["total_ms"] / a.steps
     fft_s = B * a.steps / (r["total_ms"] * 1e-3)

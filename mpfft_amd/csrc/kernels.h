@@ -21,14 +21,18 @@ struct LevelArgs {
 };
 
 struct LeafArgs {
-    const uint32_t *in;         // input buffer of transform 0
+    const uint32_t *in;         // input buffer of transform 0 (SoA limbs)
+    uint32_t *aos;              // element-major copy of the input: element i is
+                                // aos_words consecutive words hdr, limb 0..L-1
+    uint64_t aos_stride;        // words between transforms in aos
+    uint32_t aos_words;         // words per element (L+1 rounded up to 4)
     uint32_t *data;             // out buffer of transform 0
-    const int32_t *sn4;         // size-4 leaves: input offset sn
-    const uint32_t *off4;       //                output offset
-    const int32_t *sn2;         // size-2 leaves
-    const uint32_t *off2;
+    const int32_t *leaf_sn;     // per leaf, in output order: input offset sn
+    const uint32_t *leaf_off;   //                            output offset
+    const uint32_t *leaf_n;     //                            size (2 or 4)
+    uint32_t nleaf;
     uint64_t in_stride, stride;
-    uint32_t N, n4, r4, n2;
+    uint32_t N;
     int P;
 };
 

@@ -32,6 +32,12 @@ namespace mpfft {
 
 constexpr int32_t kHdrZero = INT32_MIN;   // hdr of a signed zero (| sign bit)
 
+#ifdef MPFFT_CPU_BUILD
+// host check build only: how often a truncated product fell back to the
+// exact one
+static unsigned long mp_mul_fallbacks = 0;
+#endif
+
 template <int L>
 struct Mp {
     uint32_t m[L];
@@ -293,16 +299,12 @@ MPFFT_FN void put_digit(uint32_t (&w)[L + 1], uint32_t &sticky, int c, uint32_t 
     }
 }
 
-// z = x * y, correctly rounded to P bits.
+// Exact product: every column, then one rounding step.
 template <int L, bool FULL>
-MPFFT_FN void mp_mul(Mp<L> &z, const Mp<L> &x, const Mp<L> &y, int P)
+MPFFT_FN void mp_mul_exact(Mp<L> &z, const uint32_t (&A)[Digits<L>::D],
+                           const uint32_t (&B)[Digits<L>::D], int32_t e, uint32_t sign, int P)
 {
     constexpr int D = Digits<L>::D;
-    const uint32_t sign = x.neg ^ y.neg;
-    if (x.zero | y.zero) { set_zero(z, sign); return; }
-    uint32_t A[D], B[D];
-    to_digits<L>(x.m, A);
-    to_digits<L>(y.m, B);
     uint32_t w[L + 1];
 #pragma unroll
     for (int j = 0; j <= L; j++) w[j] = 0;
@@ -318,7 +320,6 @@ MPFFT_FN void mp_mul(Mp<L> &z, const Mp<L> &x, const Mp<L> &y, int P)
         acc >>= kDigitBits;
     }
     put_digit<L>(w, sticky, 2 * D - 1, (uint32_t)acc);
-    int32_t e = x.e + y.e;
     if (!(w[0] >> 31)) {
 #pragma unroll
         for (int j = 0; j < L; j++) w[j] = fshr(w[j], w[j + 1], 31);
@@ -329,6 +330,151 @@ MPFFT_FN void mp_mul(Mp<L> &z, const Mp<L> &x, const Mp<L> &y, int P)
 #pragma unroll
     for (int j = 0; j < L; j++) z.m[j] = w[j];
     z.e = e; z.neg = sign; z.zero = 0;
+}
+
+// Truncated product: columns below K0 are skipped.  Their sum is below
+// 2^E, E = 29*K0 + 36 (a column holds at most K0 < 64 terms below 2^58),
+// so the exact product lies in [T, T + 2^E) for the computed partial sum T.
+// Rounding is monotone, hence RN(T) == RN(T + 2^E - 1) proves that RN(T) is
+// the correctly rounded product; when the two could differ the caller
+// computes the exact product instead.  K0 keeps E at least two bits below
+// the window, whose bottom is bit Wb = 32(L-1).  At 32 limbs the extra
+// state of the check pushes the level kernel past 256 registers per lane
+// (measured: the spills cost more than the skipped columns save), so the
+// product is computed in full there.
+template <int L>
+struct Trunc {
+    static constexpr int K0 = (L <= 24 && (32 * L - 70) >= 29) ? (32 * L - 70) / kDigitBits : 0;
+    static constexpr int Wb = 32 * (L - 1);
+    static constexpr int E = kDigitBits * K0 + 36;
+    static constexpr int J0 = (kDigitBits * K0) >> 5;      // first word below the window
+    static constexpr int NS = (L - 1) - J0;                 // words [J0, L-1) below the window
+};
+
+// (any bit set, all bits set) of the absolute bit range [lo, hi) of the
+// words [J0, L-1) held in sub[]
+template <int L>
+MPFFT_FN void sub_range(const uint32_t (&sub)[Trunc<L>::NS > 0 ? Trunc<L>::NS : 1],
+                        int lo, int hi, uint32_t &any, uint32_t &all)
+{
+    any = 0; all = 1;
+#pragma unroll
+    for (int k = 0; k < Trunc<L>::NS; k++) {
+        const int b0 = 32 * (Trunc<L>::J0 + k);
+        const int l = lo > b0 ? lo - b0 : 0, h = hi < b0 + 32 ? hi - b0 : 32;
+        if (l >= h) continue;
+        const uint32_t m = (h - l == 32) ? 0xffffffffu : (((1u << (h - l)) - 1u) << l);
+        any |= (sub[k] & m) != 0;
+        all &= (sub[k] & m) == m;
+    }
+}
+
+template <int L, bool FULL>
+MPFFT_FN bool mp_mul_trunc(Mp<L> &z, const uint32_t (&A)[Digits<L>::D],
+                           const uint32_t (&B)[Digits<L>::D], int32_t e, uint32_t sign, int P)
+{
+    constexpr int D = Digits<L>::D, K0 = Trunc<L>::K0, Wb = Trunc<L>::Wb, E = Trunc<L>::E;
+    constexpr int J0 = Trunc<L>::J0, NS = Trunc<L>::NS;
+    static_assert(E <= Wb - 2, "bound must stay below the window");
+    uint32_t w[L + 1], sub[NS > 0 ? NS : 1];
+#pragma unroll
+    for (int j = 0; j <= L; j++) w[j] = 0;
+#pragma unroll
+    for (int k = 0; k < (NS > 0 ? NS : 1); k++) sub[k] = 0;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int c = K0; c < 2 * D; c++) {
+        uint32_t v;
+        if (c < 2 * D - 1) {
+            const int i0 = c - (D - 1) > 0 ? c - (D - 1) : 0;
+            const int i1 = c < D - 1 ? c : D - 1;
+#pragma unroll
+            for (int i = i0; i <= i1; i++) acc = (uint64_t)A[i] * B[c - i] + acc;
+            v = (uint32_t)acc & kDigitMask;
+            acc >>= kDigitBits;
+        } else {
+            v = (uint32_t)acc;
+        }
+        // deposit digit c: word j of the product (window above L-2, sub below)
+        const int bit = kDigitBits * c, j = bit >> 5, sh = bit & 31;
+        const uint32_t lo = v << sh;
+        if (j >= L - 1) { if (j < 2 * L) w[2 * L - 1 - j] |= lo; }
+        else sub[j - J0] |= lo;
+        if (sh + kDigitBits > 32) {
+            const uint32_t hi = v >> (32 - sh);
+            if (j + 1 >= L - 1) { if (j + 1 < 2 * L) w[2 * L - 2 - j] |= hi; }
+            else sub[j + 1 - J0] |= hi;
+        }
+    }
+    // predicates on T's bits below the window, for a normalization shift of
+    // 0 (window bottom Wb) or 1 (window bottom Wb - 1)
+    uint32_t nz_low, dummy, any0, all0, any1, all1;
+    sub_range<L>(sub, kDigitBits * K0, E, nz_low, dummy);
+    sub_range<L>(sub, E, Wb, dummy, all0);
+    sub_range<L>(sub, E, Wb - 1, dummy, all1);
+    sub_range<L>(sub, kDigitBits * K0, Wb, any0, dummy);
+    sub_range<L>(sub, kDigitBits * K0, Wb - 1, any1, dummy);
+    const uint32_t s = (w[0] >> 31) ^ 1u;
+    if (s) {
+#pragma unroll
+        for (int j = 0; j < L; j++) w[j] = fshr(w[j], w[j + 1], 31);
+        w[L] = (w[L] << 1) | (sub[NS - 1] >> 31);
+        e -= 1;
+    }
+    const uint32_t sticky = s ? any1 : any0;
+    // carry into the window when 2^E - 1 is added to T
+    const uint32_t c = (s ? all1 : all0) & nz_low;
+    // round bit, the window bits below it ("rest"), and the last kept bit
+    uint32_t rbit, rest_zero, rest_ones, lsb;
+    if (FULL) {
+        rbit = w[L] >> 31;
+        rest_zero = (w[L] << 1) == 0;
+        rest_ones = (w[L] | 0x80000000u) == 0xffffffffu;
+        lsb = w[L - 1] & 1u;
+    } else {
+        const int kl = (P - 1) >> 5, rl = P >> 5;
+        const uint32_t kb = 31u - (uint32_t)((P - 1) & 31), rb = 31u - (uint32_t)(P & 31);
+        const uint32_t bm = rb == 0 ? 0u : (1u << rb) - 1u;
+        rbit = 0; rest_zero = 1; rest_ones = 1; lsb = 0;
+#pragma unroll
+        for (int j = 0; j <= L; j++) {
+            const uint32_t v = w[j];
+            if (j == rl) {
+                rbit = (v >> rb) & 1u;
+                rest_zero &= (v & bm) == 0;
+                rest_ones &= (v & bm) == bm;
+            }
+            if (j > rl) { rest_zero &= v == 0; rest_ones &= v == 0xffffffffu; }
+            if (j == kl) lsb = (v >> kb) & 1u;
+        }
+    }
+    const uint32_t amb = c ? rest_ones : (rbit & rest_zero & (sticky ^ 1u) & (lsb ^ 1u));
+    if (amb) return false;
+    round_window<L, FULL>(w, sticky, e, P);
+#pragma unroll
+    for (int j = 0; j < L; j++) z.m[j] = w[j];
+    z.e = e; z.neg = sign; z.zero = 0;
+    return true;
+}
+
+// z = x * y, correctly rounded to P bits.
+template <int L, bool FULL>
+MPFFT_FN void mp_mul(Mp<L> &z, const Mp<L> &x, const Mp<L> &y, int P)
+{
+    constexpr int D = Digits<L>::D;
+    const uint32_t sign = x.neg ^ y.neg;
+    if (x.zero | y.zero) { set_zero(z, sign); return; }
+    uint32_t A[D], B[D];
+    to_digits<L>(x.m, A);
+    to_digits<L>(y.m, B);
+    const int32_t e = x.e + y.e;
+    if constexpr (Trunc<L>::K0 > 0) {
+        if (mp_mul_trunc<L, FULL>(z, A, B, e, sign, P)) return;
+#ifdef MPFFT_CPU_BUILD
+        mp_mul_fallbacks++;
+#endif
+    }
+    mp_mul_exact<L, FULL>(z, A, B, e, sign, P);
 }
 
 // Reference schoolbook product on 32-bit limbs (kept for cross-checks).
@@ -407,6 +553,61 @@ MPFFT_FN void mp_store(const Mp<L> &x, uint32_t *__restrict__ buf,
 }
 
 #ifndef MPFFT_CPU_BUILD
+// Values parked in LDS between uses (keeps them out of the registers while a
+// multiplication runs).  Slot-major layout: word j of slot k of thread t is
+// at ((k * (L+1) + j) * nthreads + t), so a wave touches consecutive words.
+template <int L>
+MPFFT_FN void lds_put(uint32_t *lds, uint32_t nthr, uint32_t t, int k, const Mp<L> &x)
+{
+    uint32_t *p = lds + (uint32_t)k * (L + 1) * nthr + t;
+    p[0] = x.zero ? ((uint32_t)kHdrZero | x.neg) : (((uint32_t)x.e << 1) | x.neg);
+#pragma unroll
+    for (int j = 0; j < L; j++) p[(1 + j) * nthr] = x.m[j];
+}
+
+template <int L>
+MPFFT_FN void lds_get(Mp<L> &x, const uint32_t *lds, uint32_t nthr, uint32_t t, int k)
+{
+    const uint32_t *p = lds + (uint32_t)k * (L + 1) * nthr + t;
+    const int32_t h = (int32_t)p[0];
+    x.neg = (uint32_t)h & 1u;
+    x.zero = (h & ~1) == kHdrZero;
+    x.e = h >> 1;
+#pragma unroll
+    for (int j = 0; j < L; j++) x.m[j] = x.zero ? 0u : p[(1 + j) * nthr];
+}
+
+// element-major ("AoS") form: element i occupies W = round_up(L+1, 4) words
+// starting at buf + i*W (hdr, limb 0, ..., limb L-1, padding), so a gather
+// of one element reads whole 16-byte chunks
+template <int L>
+struct Aos {
+    static constexpr int W = (L + 1 + 3) & ~3;
+};
+
+template <int L>
+MPFFT_FN void mp_load_aos(Mp<L> &x, const uint32_t *__restrict__ buf, uint32_t i)
+{
+    constexpr int W = Aos<L>::W;
+    uint32_t t[W];
+#ifdef MPFFT_CPU_BUILD
+    for (int k = 0; k < W; k++) t[k] = buf[(size_t)i * W + k];
+#else
+    const uint4 *p = reinterpret_cast<const uint4 *>(buf + (size_t)i * W);
+#pragma unroll
+    for (int k = 0; k < W / 4; k++) {
+        const uint4 v = p[k];
+        t[4 * k] = v.x; t[4 * k + 1] = v.y; t[4 * k + 2] = v.z; t[4 * k + 3] = v.w;
+    }
+#endif
+    const int32_t h = (int32_t)t[0];
+    x.neg = (uint32_t)h & 1u;
+    x.zero = (h & ~1) == kHdrZero;
+    x.e = h >> 1;
+#pragma unroll
+    for (int j = 0; j < L; j++) x.m[j] = t[1 + j];
+}
+
 // exchange a number with the partner lane (lane ^ 1) of the same wave
 MPFFT_FN uint32_t swap_pair(uint32_t v)
 {

@@ -36,62 +36,101 @@ __device__ __forceinline__ uint32_t wrap_idx(int32_t sn, uint32_t s, uint32_t j,
 // per iteration.  Inlining every operation separately would give ~200 KB of
 // code at 1024 bits, far more than the instruction cache holds.
 
-// Leaves run in two passes; each iteration is "load two operands, add,
-// store", so no multi-limb value lives across iterations.
-//   pass 0, leaf of 4 (serial_mp_fft3.c:154-164):  y0 = x0 + x2,
-//     y1 = x0 - x2, y2 = a1 = x1 + x3, and a2 = x1 - x3 stored in the slot
-//     y3 of the OTHER component (the real part of y3 needs a2.im, the
-//     imaginary part a2.re, :168-174);
-//   pass 0, leaf of 2 (:141-145): y0 = x0 + x1, y1 = x0 - x1;
-//   pass 1, leaf of 4 (:166-174): y2 = y0 - a1, y0 = y0 + a1,
-//     y3 = y1 -+ a2', y1 = y1 +- a2'.
-// Every lane reads and writes only its own component's slots within a pass.
-template <int L, bool FULL>
+// Element-major copy of the input for the leaf gathers: element i becomes
+// the W = L+1 (rounded up to 4) consecutive words hdr, limb 0..L-1, so a
+// gather of one element is W/4 16-byte loads from one place instead of
+// L+1 loads from L+1 rows.  Reads are coalesced per limb row; each lane
+// writes its own contiguous W words.
+template <int L>
 __global__ void __launch_bounds__(256)
-mp_leaf_kernel(LeafArgs a, int pass)
+mp_to_aos_kernel(LeafArgs a)
 {
+    constexpr int W = Aos<L>::W;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t M = 2 * a.N;
+    if (i >= M) return;
+    const uint32_t *src = a.in + a.in_stride * blockIdx.y;
+    uint4 *dst = reinterpret_cast<uint4 *>(a.aos + a.aos_stride * blockIdx.y + (size_t)i * W);
+    uint32_t t[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) t[k] = k <= L ? src[(size_t)k * M + i] : 0u;
+#pragma unroll
+    for (int k = 0; k < W / 4; k++)
+        dst[k] = make_uint4(t[4 * k], t[4 * k + 1], t[4 * k + 2], t[4 * k + 3]);
+}
+
+// Leaves, one pass, in output order with both sizes interleaved, so the
+// stores of a wave cover a contiguous range of outputs.  Two lanes per
+// leaf (real and imaginary part); every step loads two operands, adds and
+// stores, with the values that outlive a step waiting in LDS slots:
+//   leaf of 4 (serial_mp_fft3.c:154-174): y0 = x0 + x2, y1 = x0 - x2,
+//     a1 = x1 + x3 (slot 0), a2 = x1 - x3; the real lane needs a2.im and
+//     the imaginary lane a2.re, so a2 is swapped with the partner lane
+//     (slot 1); then y2 = y0 - a1, y0 = y0 + a1, y3 = y1 -+ a2',
+//     y1 = y1 +- a2'.
+//   leaf of 2 (:141-145): the first two steps with x1 in place of x2.
+// Inputs are gathered as whole elements from the element-major copy.
+template <int L, bool FULL>
+__global__ void __launch_bounds__(256, 2)
+mp_leaf_kernel(LeafArgs a)
+{
+    extern __shared__ uint32_t lds[];
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t pair = tid >> 1, B = tid & 1u;   // B: 0 real part, 1 imag part
-    const uint32_t *in = a.in + a.in_stride * blockIdx.y;
+    const uint32_t *in = a.aos + a.aos_stride * blockIdx.y;
     uint32_t *out = a.data + a.stride * blockIdx.y;
-    const uint32_t N = a.N, M = 2 * N, base = B ? N : 0, other = B ? 0 : N;
+    const uint32_t N = a.N, M = 2 * N, base = B ? N : 0;
+    const uint32_t lt = threadIdx.x, nt = blockDim.x;
     const int P = a.P;
-    // size-4 leaves (input stride N/4) in pairs [0, r4), size-2 leaves
-    // (stride N/2) after them; pass 1 covers the size-4 leaves only
-    const bool four = pair < a.r4;
-    const uint32_t q = four ? pair : pair - a.r4;
-    if (q >= (four ? a.n4 : a.n2) || (pass && !four)) return;
-    const int32_t sn = four ? a.sn4[q] : a.sn2[q];
-    const uint32_t o = four ? a.off4[q] : a.off2[q];
+    if (pair >= a.nleaf) return;
+    const int32_t sn = a.leaf_sn[pair];
+    const uint32_t o = a.leaf_off[pair];
+    const bool four = a.leaf_n[pair] == 4u;
     const uint32_t s = four ? N >> 2 : N >> 1;
+    const int nit = four ? 8 : 2;
     Mp<L> u, v, r;
-    const int nit = four ? 4 : 2;
 #pragma unroll 1
     for (int it = 0; it < nit; it++) {
-        uint32_t fy = (uint32_t)(it & 1), dst = base + o + (uint32_t)it;
-        if (pass == 0) {
-            if ((it & 1) == 0) {
-                const uint32_t j0 = it ? 1u : 0u, j1 = four ? (it ? 3u : 2u) : 1u;
-                mp_load(u, in, M, base + wrap_idx(sn, s, j0, N));
-                mp_load(v, in, M, base + wrap_idx(sn, s, j1, N));
+        // even steps load a new operand pair, odd steps reuse it
+        if ((it & 1) == 0) {
+            if (it < 4) {
+                const uint32_t j = (uint32_t)it >> 1;          // x0,x2 (x0,x1) then x1,x3
+                mp_load_aos(u, in, base + wrap_idx(sn, s, j, N));
+                mp_load_aos(v, in, base + wrap_idx(sn, s, j ? 3 : (four ? 2 : 1), N));
+            } else {
+                const uint32_t k = (uint32_t)(it - 4) >> 1;    // y0 with a1, y1 with a2'
+                mp_load(u, out, M, base + o + k);
+                lds_get(v, lds, nt, lt, k);
             }
-            if (it == 3) dst = other + o + 3;
-        } else {
-            if ((it & 1) == 0) {
-                mp_load(u, out, M, base + o + (it ? 1u : 0u));
-                mp_load(v, out, M, base + o + (it ? 3u : 2u));
-            }
-            // y2 = y0 - a1, y0 = y0 + a1, y3 = y1 -+ a2', y1 = y1 +- a2'
-            dst = base + o + (it == 0 ? 2u : it == 1 ? 0u : it == 2 ? 3u : 1u);
-            fy = it == 0 ? 1u : it == 1 ? 0u : it == 2 ? (B ^ 1u) : B;
+        }
+        // sign of v and destination per step
+        uint32_t fy, dst;
+        switch (it) {
+        case 0: fy = 0; dst = o; break;
+        case 1: fy = 1; dst = o + 1; break;
+        case 2: fy = 0; dst = 0; break;
+        case 3: fy = 1; dst = 0; break;
+        case 4: fy = 1; dst = o + 2; break;
+        case 5: fy = 0; dst = o; break;
+        case 6: fy = B ^ 1u; dst = o + 3; break;
+        default: fy = B; dst = o + 1; break;
         }
         mp_add<L, FULL>(r, u, 0, v, fy, P);
-        mp_store(r, out, M, dst);
+        if (it == 2) {
+            lds_put(lds, nt, lt, 0, r);
+        } else if (it == 3) {
+            mp_swap_pair(r, r);                     // word-wise, so in place is fine
+            lds_put(lds, nt, lt, 1, r);
+        } else {
+            mp_store(r, out, M, base + dst);
+        }
     }
 }
 
+// Two waves per SIMD (256 registers per lane): one wave per SIMD leaves the
+// dependent multiply chains nothing to overlap with.
 template <int L, bool FULL>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
 mp_level_kernel(LevelArgs a)
 {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -124,51 +163,49 @@ mp_level_kernel(LevelArgs a)
     const uint32_t ui = o + k + (B ? q : 0);          // U0 or U1
     const uint32_t kn = k * (N / n);
 
-    // o1, o2: m1 = Z+ * w^k on the Z+ lane, m2 = Z- * conj(w^k) on the Z- lane
+    // o1, o2: m1 = Z+ * w^k on the Z+ lane, m2 = Z- * conj(w^k) on the Z- lane.
+    // Intermediates that must survive a multiplication wait in LDS slots
+    // 0 and 1, so only the multiplication's own state is live across it.
+    extern __shared__ uint32_t lds[];
+    const uint32_t lt = threadIdx.x, nt = blockDim.x;
     Mp<L> o1, o2;
     mp_load(o1, out, M, zi);
     mp_load(o2, out, M, N + zi);
     if (kind != 2) {
         // pre-additions.  general k (3m3a, serial_mp_fft3.c:220-229):
         //   t = Zr +- Zi.  k = n/8 (:209-217): Zr +- Zi and -+Zr + Zi.
-        Mp<L> r, T;
+        Mp<L> r;
         const int npre = kind == 0 ? 1 : 2;
 #pragma unroll 1
-        for (int it = 0; it < npre; it++) {
+        for (int it = npre - 1; it >= 0; it--) {
             mp_add<L, FULL>(r, o1, it ? (B ^ 1u) : 0u, o2, it ? 0u : B, P);
-            if (it == 0) T = r;
-            else o2 = r;
+            if (it) lds_put(lds, nt, lt, 1, r);        // second pre-sum -> slot 1
         }
-        if (kind == 1) o1 = T;
-        // products.  general k: T = t*LUTr; m.r = T -+ LUTd*Zi;
-        //   m.i = T - LUTs*Zr (Z+ lane) or LUTs*Zr - T (Z- lane).
-        //   k = n/8: both pre-sums times LUTr[N/8].
+        // products.  general k: T = t*LUTr -> slot 0; m.r = T -+ LUTd*Zi ->
+        //   slot 1; m.i = T - LUTs*Zr (Z+ lane) or LUTs*Zr - T (Z- lane).
+        //   k = n/8: slot j = pre-sum j times LUTr[N/8].
         const int nmul = kind == 0 ? 3 : 2;
 #pragma unroll 1
         for (int it = 0; it < nmul; it++) {
             Mp<L> f, w;
             uint32_t li = kn;
-            if (kind == 0) {
-                if (it == 0) f = T;
-                else mp_load(f, out, M, it == 1 ? N + zi : zi);
-                li = it == 0 ? kn : (it == 1 ? 2 * Q + kn : Q + kn);
-            } else {
-                f = it == 0 ? o1 : o2;
-            }
+            if (it == 0) f = r;
+            else if (kind == 0) mp_load(f, out, M, it == 1 ? N + zi : zi);
+            else lds_get(f, lds, nt, lt, 1);
+            if (kind == 0) li = it == 0 ? kn : (it == 1 ? 2 * Q + kn : Q + kn);
             mp_load(w, a.lut, ML, li);
             mp_mul<L, FULL>(r, f, w, P);
-            if (kind == 1) {
-                if (it == 0) o1 = r;
-                else o2 = r;
-            } else if (it == 0) {
-                T = r;
+            if (kind == 0 && it > 0) {
+                Mp<L> T;
+                lds_get(T, lds, nt, lt, 0);
+                mp_add<L, FULL>(o2, T, it == 2 ? B : 0u, r, B ^ 1u, P);
+                if (it == 1) lds_put(lds, nt, lt, 1, o2);
             } else {
-                Mp<L> m;
-                mp_add<L, FULL>(m, T, it == 2 ? B : 0u, r, B ^ 1u, P);
-                if (it == 1) o1 = m;
-                else o2 = m;
+                lds_put(lds, nt, lt, it, r);
             }
         }
+        lds_get(o1, lds, nt, lt, kind == 0 ? 1 : 0);
+        if (kind == 1) lds_get(o2, lds, nt, lt, 1);
     }
     // pair exchange: the Z+ lane gets m2, the Z- lane m1; then
     // Z+ lane: a1 = m1 + m2, Z- lane: a2 = m1 - m2 (serial_mp_fft3.c:234-238)
@@ -226,20 +263,21 @@ static hipError_t launch_fft(const LeafArgs &la, const LevelArgs *lv, int nlev,
                              uint32_t batch, hipStream_t st)
 {
     {
-        uint32_t thr = 2 * (la.r4 + la.n2);
+        uint32_t thr = 2 * la.nleaf;
         if (thr) {
+            dim3 gridc((2 * la.N + 255) / 256, batch);
+            hipLaunchKernelGGL((mp_to_aos_kernel<L>), gridc, dim3(256), 0, st, la);
             dim3 grid((thr + 255) / 256, batch);
-            hipLaunchKernelGGL((mp_leaf_kernel<L, FULL>), grid, dim3(256), 0, st, la, 0);
-            if (la.n4) {
-                dim3 grid4((2 * la.r4 + 255) / 256, batch);
-                hipLaunchKernelGGL((mp_leaf_kernel<L, FULL>), grid4, dim3(256), 0, st, la, 1);
-            }
+            const size_t sh = sizeof(uint32_t) * 2 * (L + 1) * 256;
+            hipLaunchKernelGGL((mp_leaf_kernel<L, FULL>), grid, dim3(256), sh, st, la);
         }
     }
     for (int i = 0; i < nlev; i++) {
         uint32_t thr = 2 * (lv[i].rG + lv[i].rS + lv[i].nZ);
         dim3 grid((thr + 255) / 256, batch);
-        hipLaunchKernelGGL((mp_level_kernel<L, FULL>), grid, dim3(256), 0, st, lv[i]);
+        // two parking slots of L+1 words per thread
+        const size_t sh = sizeof(uint32_t) * 2 * (L + 1) * 256;
+        hipLaunchKernelGGL((mp_level_kernel<L, FULL>), grid, dim3(256), sh, st, lv[i]);
     }
     return hipGetLastError();
 }

@@ -188,7 +188,7 @@ struct mpfft_plan_s {
     std::mutex mu;
 
     // mp form
-    DevBuf lut, sn4, off4, sn2, off2, nodes;
+    DevBuf lut, leaf_sn, leaf_off, leaf_n, nodes, aos;
     LeafArgs leaf{};
     std::vector<LevelArgs> levels;
     // double forms
@@ -207,23 +207,15 @@ int build_mp_plan(mpfft_plan_s *p)
 {
     const unsigned long N = p->N;
     Tree t((uint32_t)N, 4);
-    // Order leaves by the residue class of their inputs, (sn mod s): leaf j
-    // reads x[(sn + j*s) mod N], so neighbouring lanes then gather
-    // neighbouring elements (the tree order scatters them over the array).
-    std::stable_sort(t.leaves.begin(), t.leaves.end(), [N](const Tree::Leaf &a, const Tree::Leaf &b) {
-        if (a.n != b.n) return a.n > b.n;
-        const int64_t s = (int64_t)(N / a.n);
-        return ((a.sn % s) + s) % s < ((b.sn % s) + s) % s;
-    });
-    std::vector<int32_t> sn4, sn2;
-    std::vector<uint32_t> off4, off2;
-    for (const auto &lf : t.leaves) {
-        if (lf.n == 4) { sn4.push_back(lf.sn); off4.push_back(lf.off); }
-        else if (lf.n == 2) { sn2.push_back(lf.sn); off2.push_back(lf.off); }
-    }
+    // leaves stay in output order (the tree walk emits them that way): the
+    // leaf kernels' stores then cover contiguous outputs, and their input
+    // gathers read whole elements from the element-major copy
+    std::vector<int32_t> lsn;
+    std::vector<uint32_t> loff, ln;
+    for (const auto &lf : t.leaves) { lsn.push_back(lf.sn); loff.push_back(lf.off); ln.push_back(lf.n); }
     int rc;
-    if ((rc = p->sn4.upload(sn4)) || (rc = p->off4.upload(off4)) ||
-        (rc = p->sn2.upload(sn2)) || (rc = p->off2.upload(off2)))
+    if ((rc = p->leaf_sn.upload(lsn)) || (rc = p->leaf_off.upload(loff)) ||
+        (rc = p->leaf_n.upload(ln)))
         return rc;
     std::vector<uint32_t> all;
     std::vector<size_t> first(t.level.size(), 0);
@@ -237,11 +229,16 @@ int build_mp_plan(mpfft_plan_s *p)
         build_mp_lut(N, p->P, p->L, lutbuf);
         if ((rc = p->lut.upload(lutbuf))) return rc;
     }
+    // element-major staging copy of the input for the leaf gathers
+    const uint32_t aw = (uint32_t)((p->L + 1 + 3) & ~3L);
+    if ((rc = p->aos.alloc(sizeof(uint32_t) * (size_t)aw * 2 * N * p->batch))) return rc;
     LeafArgs &la = p->leaf;
-    la.sn4 = p->sn4.as<int32_t>(); la.off4 = p->off4.as<uint32_t>();
-    la.sn2 = p->sn2.as<int32_t>(); la.off2 = p->off2.as<uint32_t>();
-    la.N = (uint32_t)N; la.n4 = (uint32_t)sn4.size(); la.r4 = pad32(la.n4);
-    la.n2 = (uint32_t)sn2.size(); la.P = (int)p->P;
+    la.aos = p->aos.as<uint32_t>();
+    la.aos_words = aw;
+    la.aos_stride = (uint64_t)aw * 2 * N;
+    la.leaf_sn = p->leaf_sn.as<int32_t>(); la.leaf_off = p->leaf_off.as<uint32_t>();
+    la.leaf_n = p->leaf_n.as<uint32_t>(); la.nleaf = N >= 2 ? (uint32_t)lsn.size() : 0u;
+    la.N = (uint32_t)N; la.P = (int)p->P;
     la.in_stride = la.stride = p->words;
     p->levels.clear();
     for (uint32_t l = 3; l <= t.lgN; l++) {
@@ -316,7 +313,7 @@ int run_mp(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t s
     }
     LeafArgs la = p->leaf;
     la.in = d_in; la.data = d_out;
-    if (s0 > 0) la.n4 = la.r4 = la.n2 = 0;        // no leaf launch
+    if (s0 > 0) la.nleaf = 0;                     // no leaf launch
     const int l0 = s0 > 0 ? s0 - 1 : 0, l1 = s1 - 1;
     std::vector<LevelArgs> lv(p->levels.begin() + l0, p->levels.begin() + (l1 > l0 ? l1 : l0));
     for (auto &a : lv) a.data = d_out;

@@ -170,7 +170,9 @@ static int run_case(int P, int op, long trials)
         }
     }
     mpfr_clears(a, b, c, (mpfr_ptr)0);
-    printf("L=%2d P=%4d %s trials=%ld mismatches=%ld\n", L, P, op ? "mul" : "add", trials, bad);
+    printf("L=%2d P=%4d %s trials=%ld mismatches=%ld fallbacks=%lu\n", L, P, op ? "mul" : "add",
+           trials, bad, op ? mp_mul_fallbacks : 0ul);
+    mp_mul_fallbacks = 0;
     return bad != 0;
 }
 
