@@ -27,10 +27,9 @@ struct LeafArgs {
     uint64_t aos_stride;        // words between transforms in aos
     uint32_t aos_words;         // words per element (L+1 rounded up to 4)
     uint32_t *data;             // out buffer of transform 0
-    const int32_t *leaf_sn;     // per leaf, in output order: input offset sn
-    const uint32_t *leaf_off;   //                            output offset
-    const uint32_t *leaf_n;     //                            size (2 or 4)
-    uint32_t nleaf;
+    const int32_t *pos_sn;      // per output position p < N: sn of its leaf
+    const uint32_t *pos_nr;     //   (leaf size n) | (p - leaf offset) << 8
+    uint32_t npos;              // N, or 0 for no leaf launch
     uint64_t in_stride, stride;
     uint32_t N;
     int P;

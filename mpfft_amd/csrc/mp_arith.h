@@ -338,13 +338,11 @@ MPFFT_FN void mp_mul_exact(Mp<L> &z, const uint32_t (&A)[Digits<L>::D],
 // Rounding is monotone, hence RN(T) == RN(T + 2^E - 1) proves that RN(T) is
 // the correctly rounded product; when the two could differ the caller
 // computes the exact product instead.  K0 keeps E at least two bits below
-// the window, whose bottom is bit Wb = 32(L-1).  At 32 limbs the extra
-// state of the check pushes the level kernel past 256 registers per lane
-// (measured: the spills cost more than the skipped columns save), so the
-// product is computed in full there.
+// the window, whose bottom is bit Wb = 32(L-1).  At 32 limbs (36 digits)
+// this skips columns 0..31, 528 of the 1296 digit products.
 template <int L>
 struct Trunc {
-    static constexpr int K0 = (L <= 24 && (32 * L - 70) >= 29) ? (32 * L - 70) / kDigitBits : 0;
+    static constexpr int K0 = (32 * L - 70) >= kDigitBits ? (32 * L - 70) / kDigitBits : 0;
     static constexpr int Wb = 32 * (L - 1);
     static constexpr int E = kDigitBits * K0 + 36;
     static constexpr int J0 = (kDigitBits * K0) >> 5;      // first word below the window

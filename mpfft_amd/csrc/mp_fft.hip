@@ -59,72 +59,56 @@ mp_to_aos_kernel(LeafArgs a)
         dst[k] = make_uint4(t[4 * k], t[4 * k + 1], t[4 * k + 2], t[4 * k + 3]);
 }
 
-// Leaves, one pass, in output order with both sizes interleaved, so the
-// stores of a wave cover a contiguous range of outputs.  Two lanes per
-// leaf (real and imaginary part); every step loads two operands, adds and
-// stores, with the values that outlive a step waiting in LDS slots:
-//   leaf of 4 (serial_mp_fft3.c:154-174): y0 = x0 + x2, y1 = x0 - x2,
-//     a1 = x1 + x3 (slot 0), a2 = x1 - x3; the real lane needs a2.im and
-//     the imaginary lane a2.re, so a2 is swapped with the partner lane
-//     (slot 1); then y2 = y0 - a1, y0 = y0 + a1, y3 = y1 -+ a2',
-//     y1 = y1 +- a2'.
-//   leaf of 2 (:141-145): the first two steps with x1 in place of x2.
+// Leaves (serial_mp_fft3.c:136-177), one lane per output real: lane
+// (h, p) computes part h (0 real, 1 imaginary) of out[p], so the lanes of
+// a wave store 64 consecutive words of every limb row.  With r = p - o the
+// position inside its leaf (offset o, size n, input stride s = N/n):
+//   n = 2: out[o+r] = x0 +- x1                                  (1 add)
+//   n = 4: y  = x0 +- x2        (y0 for even r, y1 for odd r)
+//          a  = x1 + x3 (a1, same part h) for even r,
+//               x1 - x3 (a2, the other part) for odd r
+//          out = y +- a, the signs of :166-176               (3 adds)
+// Each add is the reference's own correctly rounded operation on the same
+// operands, so recomputing y and a per lane changes no result bit.
 // Inputs are gathered as whole elements from the element-major copy.
 template <int L, bool FULL>
 __global__ void __launch_bounds__(256, 2)
 mp_leaf_kernel(LeafArgs a)
 {
-    extern __shared__ uint32_t lds[];
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t pair = tid >> 1, B = tid & 1u;   // B: 0 real part, 1 imag part
+    const uint32_t N = a.N, M = 2 * N;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M) return;
+    const uint32_t h = t >= N ? 1u : 0u, p = t - h * N;
     const uint32_t *in = a.aos + a.aos_stride * blockIdx.y;
     uint32_t *out = a.data + a.stride * blockIdx.y;
-    const uint32_t N = a.N, M = 2 * N, base = B ? N : 0;
-    const uint32_t lt = threadIdx.x, nt = blockDim.x;
     const int P = a.P;
-    if (pair >= a.nleaf) return;
-    const int32_t sn = a.leaf_sn[pair];
-    const uint32_t o = a.leaf_off[pair];
-    const bool four = a.leaf_n[pair] == 4u;
-    const uint32_t s = four ? N >> 2 : N >> 1;
-    const int nit = four ? 8 : 2;
-    Mp<L> u, v, r;
+    const int32_t sn = a.pos_sn[p];
+    const uint32_t nr = a.pos_nr[p], n = nr & 0xffu, r = nr >> 8;
+    const bool four = n == 4u;
+    const uint32_t s = N / n;
+    const uint32_t odd = r & 1u;
+    const uint32_t hb = h * N, ha = (four && odd) ? (N - hb) : hb;   // part of a
+    const int nit = four ? 3 : 1;
+    Mp<L> u, v, y, res;
 #pragma unroll 1
     for (int it = 0; it < nit; it++) {
-        // even steps load a new operand pair, odd steps reuse it
-        if ((it & 1) == 0) {
-            if (it < 4) {
-                const uint32_t j = (uint32_t)it >> 1;          // x0,x2 (x0,x1) then x1,x3
-                mp_load_aos(u, in, base + wrap_idx(sn, s, j, N));
-                mp_load_aos(v, in, base + wrap_idx(sn, s, j ? 3 : (four ? 2 : 1), N));
-            } else {
-                const uint32_t k = (uint32_t)(it - 4) >> 1;    // y0 with a1, y1 with a2'
-                mp_load(u, out, M, base + o + k);
-                lds_get(v, lds, nt, lt, k);
-            }
-        }
-        // sign of v and destination per step
-        uint32_t fy, dst;
-        switch (it) {
-        case 0: fy = 0; dst = o; break;
-        case 1: fy = 1; dst = o + 1; break;
-        case 2: fy = 0; dst = 0; break;
-        case 3: fy = 1; dst = 0; break;
-        case 4: fy = 1; dst = o + 2; break;
-        case 5: fy = 0; dst = o; break;
-        case 6: fy = B ^ 1u; dst = o + 3; break;
-        default: fy = B; dst = o + 1; break;
-        }
-        mp_add<L, FULL>(r, u, 0, v, fy, P);
-        if (it == 2) {
-            lds_put(lds, nt, lt, 0, r);
-        } else if (it == 3) {
-            mp_swap_pair(r, r);                     // word-wise, so in place is fine
-            lds_put(lds, nt, lt, 1, r);
+        uint32_t fy;
+        if (it == 0) {
+            mp_load_aos(u, in, hb + wrap_idx(sn, s, 0, N));
+            mp_load_aos(v, in, hb + wrap_idx(sn, s, four ? 2 : 1, N));
+            fy = four ? odd : r;
+        } else if (it == 1) {
+            mp_load_aos(u, in, ha + wrap_idx(sn, s, 1, N));
+            mp_load_aos(v, in, ha + wrap_idx(sn, s, 3, N));
+            fy = odd;
         } else {
-            mp_store(r, out, M, base + dst);
+            u = y; v = res;
+            fy = (r >> 1) ^ (odd & h);
         }
+        mp_add<L, FULL>(res, u, 0, v, fy, P);
+        if (it == 0) y = res;
     }
+    mp_store(res, out, M, t);
 }
 
 // Two waves per SIMD (256 registers per lane): one wave per SIMD leaves the
@@ -182,8 +166,10 @@ mp_level_kernel(LevelArgs a)
             if (it) lds_put(lds, nt, lt, 1, r);        // second pre-sum -> slot 1
         }
         // products.  general k: T = t*LUTr -> slot 0; m.r = T -+ LUTd*Zi ->
-        //   slot 1; m.i = T - LUTs*Zr (Z+ lane) or LUTs*Zr - T (Z- lane).
-        //   k = n/8: slot j = pre-sum j times LUTr[N/8].
+        //   slot 1; m.i = T - LUTs*Zr (Z+ lane) or LUTs*Zr - T (Z- lane) ->
+        //   slot 0 (T is dead by then).  k = n/8: slot j = pre-sum j times
+        //   LUTr[N/8].  Nothing but the product's own state stays in
+        //   registers across a multiplication.
         const int nmul = kind == 0 ? 3 : 2;
 #pragma unroll 1
         for (int it = 0; it < nmul; it++) {
@@ -198,14 +184,12 @@ mp_level_kernel(LevelArgs a)
             if (kind == 0 && it > 0) {
                 Mp<L> T;
                 lds_get(T, lds, nt, lt, 0);
-                mp_add<L, FULL>(o2, T, it == 2 ? B : 0u, r, B ^ 1u, P);
-                if (it == 1) lds_put(lds, nt, lt, 1, o2);
-            } else {
-                lds_put(lds, nt, lt, it, r);
+                mp_add<L, FULL>(r, T, it == 2 ? B : 0u, r, B ^ 1u, P);
             }
+            lds_put(lds, nt, lt, (kind == 0 && it == 2) ? 0 : (it ? 1 : 0), r);
         }
         lds_get(o1, lds, nt, lt, kind == 0 ? 1 : 0);
-        if (kind == 1) lds_get(o2, lds, nt, lt, 1);
+        lds_get(o2, lds, nt, lt, kind == 0 ? 0 : 1);
     }
     // pair exchange: the Z+ lane gets m2, the Z- lane m1; then
     // Z+ lane: a1 = m1 + m2, Z- lane: a2 = m1 - m2 (serial_mp_fft3.c:234-238)
@@ -263,13 +247,10 @@ static hipError_t launch_fft(const LeafArgs &la, const LevelArgs *lv, int nlev,
                              uint32_t batch, hipStream_t st)
 {
     {
-        uint32_t thr = 2 * la.nleaf;
-        if (thr) {
-            dim3 gridc((2 * la.N + 255) / 256, batch);
-            hipLaunchKernelGGL((mp_to_aos_kernel<L>), gridc, dim3(256), 0, st, la);
-            dim3 grid((thr + 255) / 256, batch);
-            const size_t sh = sizeof(uint32_t) * 2 * (L + 1) * 256;
-            hipLaunchKernelGGL((mp_leaf_kernel<L, FULL>), grid, dim3(256), sh, st, la);
+        if (la.npos) {
+            dim3 grid((2 * la.N + 255) / 256, batch);
+            hipLaunchKernelGGL((mp_to_aos_kernel<L>), grid, dim3(256), 0, st, la);
+            hipLaunchKernelGGL((mp_leaf_kernel<L, FULL>), grid, dim3(256), 0, st, la);
         }
     }
     for (int i = 0; i < nlev; i++) {

@@ -188,7 +188,7 @@ struct mpfft_plan_s {
     std::mutex mu;
 
     // mp form
-    DevBuf lut, leaf_sn, leaf_off, leaf_n, nodes, aos;
+    DevBuf lut, pos_sn, pos_nr, nodes, aos;
     LeafArgs leaf{};
     std::vector<LevelArgs> levels;
     // double forms
@@ -207,16 +207,16 @@ int build_mp_plan(mpfft_plan_s *p)
 {
     const unsigned long N = p->N;
     Tree t((uint32_t)N, 4);
-    // leaves stay in output order (the tree walk emits them that way): the
-    // leaf kernels' stores then cover contiguous outputs, and their input
-    // gathers read whole elements from the element-major copy
-    std::vector<int32_t> lsn;
-    std::vector<uint32_t> loff, ln;
-    for (const auto &lf : t.leaves) { lsn.push_back(lf.sn); loff.push_back(lf.off); ln.push_back(lf.n); }
+    // per output position: its leaf's sn, size and the position inside it
+    std::vector<int32_t> psn(N);
+    std::vector<uint32_t> pnr(N);
+    for (const auto &lf : t.leaves)
+        for (uint32_t r = 0; r < lf.n; r++) {
+            psn[lf.off + r] = lf.sn;
+            pnr[lf.off + r] = lf.n | (r << 8);
+        }
     int rc;
-    if ((rc = p->leaf_sn.upload(lsn)) || (rc = p->leaf_off.upload(loff)) ||
-        (rc = p->leaf_n.upload(ln)))
-        return rc;
+    if ((rc = p->pos_sn.upload(psn)) || (rc = p->pos_nr.upload(pnr))) return rc;
     std::vector<uint32_t> all;
     std::vector<size_t> first(t.level.size(), 0);
     for (size_t l = 0; l < t.level.size(); l++) {
@@ -236,8 +236,8 @@ int build_mp_plan(mpfft_plan_s *p)
     la.aos = p->aos.as<uint32_t>();
     la.aos_words = aw;
     la.aos_stride = (uint64_t)aw * 2 * N;
-    la.leaf_sn = p->leaf_sn.as<int32_t>(); la.leaf_off = p->leaf_off.as<uint32_t>();
-    la.leaf_n = p->leaf_n.as<uint32_t>(); la.nleaf = N >= 2 ? (uint32_t)lsn.size() : 0u;
+    la.pos_sn = p->pos_sn.as<int32_t>(); la.pos_nr = p->pos_nr.as<uint32_t>();
+    la.npos = N >= 2 ? (uint32_t)N : 0u;
     la.N = (uint32_t)N; la.P = (int)p->P;
     la.in_stride = la.stride = p->words;
     p->levels.clear();
@@ -313,7 +313,7 @@ int run_mp(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t s
     }
     LeafArgs la = p->leaf;
     la.in = d_in; la.data = d_out;
-    if (s0 > 0) la.nleaf = 0;                     // no leaf launch
+    if (s0 > 0) la.npos = 0;                      // no leaf launch
     const int l0 = s0 > 0 ? s0 - 1 : 0, l1 = s1 - 1;
     std::vector<LevelArgs> lv(p->levels.begin() + l0, p->levels.begin() + (l1 > l0 ? l1 : l0));
     for (auto &a : lv) a.data = d_out;
