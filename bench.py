@@ -38,6 +38,24 @@ def cadd(n):
     return 3 * n * lg - 3 * n + 4 if n >= 8 else {1: 0, 2: 4, 4: 16}[n]
 
 
+PROFILE = os.path.join(ROOT, "profiles", "mp_level_kernel_N65536_P1024.txt")
+
+
+def profiled_traffic(n, P, batch):
+    """HBM bytes per level-kernel launch (FETCH_SIZE + WRITE_SIZE, scaled by
+    the copy-kernel calibration) from the committed rocprofv3 summary, when
+    it covers this workload; None otherwise."""
+    try:
+        for line in open(PROFILE):
+            f = line.split()
+            if f and f[0] == "level_kernel_traffic_bytes_per_launch":
+                if f[2:] == [f"N={n}", f"P={P}", f"batch={batch}"]:
+                    return int(f[1])
+    except OSError:
+        pass
+    return None
+
+
 def measure(M, I, torch, n, P, batch, steps, warmup, seed):
     plan = M.Plan(n, P, M.FORM_MP, batch)
     Lc = I.limbs_for(P)
@@ -84,7 +102,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1 << 16)
     ap.add_argument("--prec", type=int, default=1024)
-    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--prec2", type=int, default=256, help="second precision (0 = skip)")
     ap.add_argument("--seed", type=int, default=0x6D7046465420)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -148,7 +166,7 @@ def main():
                       "mp_level_kernel_avg_us": round(r["lvl_ms"] / a.steps / nlev * 1e3, 2)},
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": round(peak, 4),
                      "unit": "T(32x32->64 limb products)/s", "frac": round(achieved / peak, 4),
-                     "traffic": None,
+                     "traffic": profiled_traffic(n, P, B),
                      "kernel": "mpfft::mp_level_kernel",
                      "algorithmic_per_fft": prod_per_fft},
         "roofline_hbm": {"achieved": round(bytes_fft * fft_s / 1e9, 2), "peak": HBM_PEAK_GBS,

@@ -14,9 +14,6 @@ struct LevelArgs {
     const uint32_t *node_off;   // output offsets of the size-n nodes
     uint64_t stride;            // words between consecutive transforms
     uint32_t N, n, cnt;         // size, node size, number of size-n nodes
-    uint32_t nG, rG;            // general butterflies; padded range (pairs)
-    uint32_t nS, rS;            // k = n/8 butterflies; padded range
-    uint32_t nZ;                // k = 0 butterflies
     int P;
 };
 

@@ -15,9 +15,10 @@
 // Work mapping: two lanes (a DPP quad pair) per butterfly.  The even lane
 // computes m1 = Z+ * w^k, the odd lane m2 = Z- * conj(w^k), with the same
 // instruction stream (operand signs are data); the pair then swaps m1/m2
-// and each lane produces two of the four outputs.  Butterflies of the three
-// kinds (k = 0, k = n/8, general k) are laid out in separate 32-pair ranges
-// so every wave runs one kind only.
+// and each lane produces two of the four outputs.  Butterflies are laid
+// out node by node, k fastest: a wave's accesses to U0, U1, Z+ and Z- are
+// dense runs, at the price of some divergence where k = 0 (no product) and
+// k = n/8 (two products) sit beside general k (three products).
 #include "mp_arith.h"
 #include "kernels.h"
 
@@ -123,25 +124,13 @@ mp_level_kernel(LevelArgs a)
     const uint32_t N = a.N, n = a.n, q = n >> 2, M = 2 * N, Q = N >> 2, ML = 3 * Q;
     const int P = a.P;
 
-    uint32_t kind, node, k;
-    if (pair < a.rG) {
-        if (pair >= a.nG) return;
-        const uint32_t per = q - 2;                  // k in [1, n/4) \ {n/8}
-        node = pair / per;
-        k = pair - node * per + 1;
-        k += (k >= (n >> 3));
-        kind = 0;
-    } else if (pair < a.rG + a.rS) {
-        node = pair - a.rG;
-        if (node >= a.nS) return;
-        k = n >> 3;
-        kind = 1;
-    } else {
-        node = pair - a.rG - a.rS;
-        if (node >= a.nZ) return;
-        k = 0;
-        kind = 2;
-    }
+    // node-major: all k of a node on neighbouring lanes, so each load or
+    // store instruction covers dense runs of positions; the three kinds
+    // (0: general k, 1: k = n/8, 2: k = 0) diverge only around the products
+    const uint32_t node = pair / q;
+    if (node >= a.cnt) return;
+    const uint32_t k = pair - node * q;
+    const uint32_t kind = k == 0 ? 2u : (k == (n >> 3) ? 1u : 0u);
     const uint32_t o = a.node_off[node];
     const uint32_t zi = o + k + (B ? 3 * q : 2 * q);  // Z+ or Z-
     const uint32_t ui = o + k + (B ? q : 0);          // U0 or U1
@@ -254,7 +243,7 @@ static hipError_t launch_fft(const LeafArgs &la, const LevelArgs *lv, int nlev,
         }
     }
     for (int i = 0; i < nlev; i++) {
-        uint32_t thr = 2 * (lv[i].rG + lv[i].rS + lv[i].nZ);
+        uint32_t thr = 2 * lv[i].cnt * (lv[i].n / 4);
         dim3 grid((thr + 255) / 256, batch);
         // two parking slots of L+1 words per thread
         const size_t sh = sizeof(uint32_t) * 2 * (L + 1) * 256;

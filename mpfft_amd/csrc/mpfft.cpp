@@ -38,8 +38,6 @@ constexpr unsigned long kMaxN = 1UL << 24;
 
 bool is_pow2(unsigned long n) { return n && !(n & (n - 1)); }
 
-uint32_t pad32(uint32_t v) { return (v + 31u) & ~31u; }
-
 // ---- recursion tree (serial_mp_fft3.c:180-182) ----------------------------
 
 struct Tree {
@@ -248,9 +246,6 @@ int build_mp_plan(mpfft_plan_s *p)
         a.node_off = p->nodes.as<uint32_t>() + first[l];
         a.stride = p->words;
         a.N = (uint32_t)N; a.n = n; a.cnt = cnt;
-        a.nG = n >= 16 ? cnt * (n / 4 - 2) : 0; a.rG = pad32(a.nG);
-        a.nS = cnt; a.rS = pad32(cnt);
-        a.nZ = cnt;
         a.P = (int)p->P;
         p->levels.push_back(a);
     }
