@@ -114,6 +114,26 @@ int mpfft_execute_limbs_range(mpfft_plan plan, const uint32_t *d_in, uint32_t *d
                               void *stream, int stage_begin, int stage_end);
 int mpfft_plan_stages(mpfft_plan plan);
 
+/* Inverse transform, the way the reference's test() computes it
+ * (serial_mp_fft3.c:367-385): out = swap(FFT(swap(in))) / N, where swap
+ * exchanges real and imaginary parts; division by N is exact.  Same buffers
+ * and stream contract as mpfft_execute_limbs. */
+int mpfft_execute_inverse_limbs(mpfft_plan plan, const uint32_t *d_in, uint32_t *d_out,
+                                void *stream);
+
+/* Error of y against x per transform, the reference's accuracy measures
+ * (serial_mp_fft3.c:387-404): aE = x - y at P bits, then
+ * out[8b + 0/1] = rRMSE sqrt(sum aE^2 / sum x^2) of the real / imaginary
+ * parts, out[8b + 2/3] = max |aE|, out[8b + 4..7] = log2 of those four (the
+ * doubles underflow for P beyond ~1000 bits; the log2 values do not).  The
+ * sums are accumulated in double, not at P bits.  Waits for `stream`. */
+int mpfft_roundtrip_stats(mpfft_plan plan, const uint32_t *d_x, const uint32_t *d_y,
+                          void *stream, double *out);
+
+/* The reference's test() on host data: forward transform of x, inverse of
+ * the result, then mpfft_roundtrip_stats(x, round trip) into out[8 * batch]. */
+int mpfft_roundtrip_host(mpfft_plan plan, const uint32_t *h_x, double *out);
+
 /* The same from host SoA-limb buffers (copies in, runs, copies out). */
 int mpfft_execute_limbs_host(mpfft_plan plan, const uint32_t *h_in, uint32_t *h_out);
 

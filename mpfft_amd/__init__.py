@@ -30,6 +30,7 @@ EXPORTS = [
     "mpfft_execute_f64", "mpfft_execute_f64_host",
     "mpfft_plan_sync", "mpfft_mpfr_to_limbs", "mpfft_limbs_to_mpfr",
     "mpfft_mp_binop_host", "mpfft_limbs_for_precision",
+    "mpfft_execute_inverse_limbs", "mpfft_roundtrip_stats", "mpfft_roundtrip_host",
 ]
 
 _lib = None
@@ -68,6 +69,9 @@ def lib():
     L.mpfft_execute_f64.argtypes = [VP, P, P, P]
     L.mpfft_execute_f64_host.argtypes = [VP, P, P]
     L.mpfft_plan_sync.argtypes = [VP]
+    L.mpfft_execute_inverse_limbs.argtypes = [VP, P, P, P]
+    L.mpfft_roundtrip_stats.argtypes = [VP, P, P, P, P]
+    L.mpfft_roundtrip_host.argtypes = [VP, P, P]
     L.mpfft_mp_binop_host.argtypes = [I, U, L_, P, P, P]
     L.mpfft_limbs_for_precision.argtypes = [L_]
     L.mpfft_limbs_for_precision.restype = L_
@@ -151,6 +155,29 @@ class Plan:
 
     def sync(self):
         check(lib().mpfft_plan_sync(self.h), "plan_sync")
+
+    def execute_inverse(self, d_in, d_out, stream=None):
+        """inverse transform by the reference's swap trick (device pointers)"""
+        s = ctypes.c_void_p(stream) if stream else None
+        check(lib().mpfft_execute_inverse_limbs(self.h, ctypes.c_void_p(d_in),
+                                                ctypes.c_void_p(d_out), s), "execute_inverse")
+
+    def roundtrip_stats(self, d_x, d_y, stream=None):
+        """error statistics of y against x (device pointers): array (batch, 8)
+        of rRMSE re/im, max|err| re/im and the log2 of the four"""
+        out = np.zeros(8 * self.batch)
+        s = ctypes.c_void_p(stream) if stream else None
+        check(lib().mpfft_roundtrip_stats(self.h, ctypes.c_void_p(d_x), ctypes.c_void_p(d_y),
+                                          s, _ptr(out)), "roundtrip_stats")
+        return out.reshape(self.batch, 8)
+
+    def roundtrip_host(self, x):
+        """the reference's test(): forward, inverse, statistics; (batch, 8)"""
+        x = np.ascontiguousarray(x, dtype=np.uint32)
+        assert x.size == self.words * self.batch
+        out = np.zeros(8 * self.batch)
+        check(lib().mpfft_roundtrip_host(self.h, _ptr(x), _ptr(out)), "roundtrip_host")
+        return out.reshape(self.batch, 8)
 
 
 def mp_binop(op, x, y, count, P):

@@ -29,6 +29,7 @@ struct LeafArgs {
     uint32_t npos;              // N, or 0 for no leaf launch
     uint64_t in_stride, stride;
     uint32_t N;
+    uint32_t in_swap;           // 1: read element i from (i + N) mod 2N (re <-> im)
     int P;
 };
 
@@ -38,6 +39,18 @@ int limbs_for_precision(long P);
 
 hipError_t mp_fft_launch(int L, int P, const LeafArgs &la, const LevelArgs *lv,
                          int nlev, uint32_t batch, hipStream_t st);
+// in place: swap the real and imaginary halves and divide by 2^lgN (exact:
+// an exponent shift), the closing step of the inverse by the swap trick
+hipError_t mp_unswap_scale_launch(int L, uint32_t *data, uint64_t stride, uint32_t N,
+                                  int32_t lgN, uint32_t batch, hipStream_t st);
+
+// error statistics of y against x (aE = x - y at P bits): per transform and
+// per block of 256 reals, kErrPartial doubles
+constexpr int kErrPartial = 10;
+constexpr uint32_t kErrBlock = 256;
+hipError_t mp_err_launch(int L, int P, const uint32_t *x, const uint32_t *y, uint64_t stride,
+                         uint32_t N, double *partial, uint32_t batch, hipStream_t st);
+
 hipError_t mp_binop_launch(int L, int P, int op, uint32_t count, const uint32_t *x,
                            const uint32_t *y, uint32_t *z, hipStream_t st);
 

@@ -52,9 +52,10 @@ mp_to_aos_kernel(LeafArgs a)
     if (i >= M) return;
     const uint32_t *src = a.in + a.in_stride * blockIdx.y;
     uint4 *dst = reinterpret_cast<uint4 *>(a.aos + a.aos_stride * blockIdx.y + (size_t)i * W);
+    const uint32_t j = a.in_swap ? (i < a.N ? i + a.N : i - a.N) : i;
     uint32_t t[W];
 #pragma unroll
-    for (int k = 0; k < W; k++) t[k] = k <= L ? src[(size_t)k * M + i] : 0u;
+    for (int k = 0; k < W; k++) t[k] = k <= L ? src[(size_t)k * M + j] : 0u;
 #pragma unroll
     for (int k = 0; k < W / 4; k++)
         dst[k] = make_uint4(t[4 * k], t[4 * k + 1], t[4 * k + 2], t[4 * k + 3]);
@@ -228,6 +229,118 @@ mp_binop_kernel(int op, uint32_t count, const uint32_t *x, const uint32_t *y,
     mp_store(c, z, count, i);
 }
 
+// Closing step of the inverse transform by the swap trick
+// (serial_mp_fft3.c:367-378 and :385): out.re = X.im / N, out.im = X.re / N
+// for X the forward transform of the swapped input.  Division by N = 2^lgN
+// only moves the exponent, so it is exact.
+template <int L>
+__global__ void __launch_bounds__(256)
+mp_unswap_scale_kernel(uint32_t *data, uint64_t stride, uint32_t N, int32_t lgN)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    uint32_t *buf = data + stride * blockIdx.y;
+    const uint32_t M = 2 * N;
+    uint32_t ha = buf[i], hb = buf[N + i];
+    auto scale = [lgN](uint32_t h) -> uint32_t {
+        if (((int32_t)h & ~1) == kHdrZero) return h;
+        return (uint32_t)((((int32_t)h >> 1) - lgN) * 2) | (h & 1u);
+    };
+    buf[i] = scale(hb);
+    buf[N + i] = scale(ha);
+#pragma unroll
+    for (int j = 1; j <= L; j++) {
+        uint32_t *row = buf + (size_t)j * M;
+        const uint32_t a = row[i], b = row[N + i];
+        row[i] = b;
+        row[N + i] = a;
+    }
+}
+
+// Error statistics of y against x, as the reference's test()
+// (serial_mp_fft3.c:387-398): aE = x - y rounded at P bits, then per part
+// (real, imaginary) sum x^2, sum aE^2 and max |aE|.  The sums are formed in
+// double from the top 64 bits of each significand with a per-block common
+// exponent (values near 2^-P*|x| are far below the double range), so they
+// carry double rounding rather than the reference's P-bit rounding.  Per
+// block and part: {sum x^2 mantissa, its exponent, sum aE^2 mantissa, its
+// exponent, max log2|aE|}.
+template <int L, bool FULL>
+__global__ void __launch_bounds__(256)
+mp_err_kernel(const uint32_t *x, const uint32_t *y, uint64_t stride, uint32_t N, int P,
+              double *partial)
+{
+    __shared__ int32_t se[2][256];
+    __shared__ double sv[2][256];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x, M = 2 * N;
+    const uint32_t lt = threadIdx.x;
+    const uint32_t h = t >= N ? 1u : 0u;
+    const bool live = t < M;
+    int32_t ex = INT32_MIN, ed = INT32_MIN;
+    double mx = 0.0, md = 0.0, lg = -INFINITY;
+    if (live) {
+        Mp<L> a, b, d;
+        mp_load(a, x + stride * blockIdx.y, M, t);
+        mp_load(b, y + stride * blockIdx.y, M, t);
+        mp_add<L, FULL>(d, a, 0, b, 1, P);
+        auto mant = [](const Mp<L> &v) {
+            const double hi = (double)v.m[0], lo = (double)v.m[L > 1 ? 1 : 0] * (L > 1);
+            return (hi * 4294967296.0 + lo) * 0x1p-64;         // in [0.5, 1)
+        };
+        if (!a.zero) { const double m = mant(a); mx = m * m; ex = 2 * a.e; }
+        if (!d.zero) {
+            const double m = mant(d);
+            md = m * m; ed = 2 * d.e;
+            lg = (double)d.e + log2(m);
+        }
+    }
+    // per part: common exponent (max), then scaled sums; max of log2|aE|
+    for (int q = 0; q < 2; q++) {
+        const int32_t e = q ? ed : ex;
+        const double v = q ? md : mx;
+        for (uint32_t p = 0; p < 2; p++) se[p][lt] = (p == h) ? e : INT32_MIN;
+        __syncthreads();
+        for (uint32_t w = 128; w > 0; w >>= 1) {
+            if (lt < w)
+                for (int p = 0; p < 2; p++)
+                    se[p][lt] = max(se[p][lt], se[p][lt + w]);
+            __syncthreads();
+        }
+        const int32_t E0 = se[0][0], E1 = se[1][0];
+        __syncthreads();
+        for (uint32_t p = 0; p < 2; p++) {
+            const int32_t E = p ? E1 : E0;
+            sv[p][lt] = (p == h && e != INT32_MIN) ? ldexp(v, e - E) : 0.0;
+        }
+        __syncthreads();
+        for (uint32_t w = 128; w > 0; w >>= 1) {
+            if (lt < w)
+                for (int p = 0; p < 2; p++) sv[p][lt] += sv[p][lt + w];
+            __syncthreads();
+        }
+        if (lt == 0) {
+            double *o = partial + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kErrPartial;
+            for (int p = 0; p < 2; p++) {
+                o[5 * p + 2 * q] = sv[p][0];
+                o[5 * p + 2 * q + 1] = (double)(p ? E1 : E0);
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t p = 0; p < 2; p++) sv[p][lt] = (p == h) ? lg : -INFINITY;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (lt < w)
+            for (int p = 0; p < 2; p++) sv[p][lt] = fmax(sv[p][lt], sv[p][lt + w]);
+        __syncthreads();
+    }
+    if (lt == 0) {
+        double *o = partial + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kErrPartial;
+        o[4] = sv[0][0];
+        o[9] = sv[1][0];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host-side launchers (one instantiation per supported limb count)
 
@@ -292,6 +405,66 @@ hipError_t mp_fft_launch(int L, int P, const LeafArgs &la, const LevelArgs *lv,
     default: return hipErrorInvalidValue;
     }
 #undef CALLF
+}
+
+template <int L>
+static hipError_t launch_unswap(uint32_t *data, uint64_t stride, uint32_t N, int32_t lgN,
+                                uint32_t batch, hipStream_t st)
+{
+    dim3 grid((N + 255) / 256, batch);
+    hipLaunchKernelGGL((mp_unswap_scale_kernel<L>), grid, dim3(256), 0, st, data, stride, N, lgN);
+    return hipGetLastError();
+}
+
+hipError_t mp_unswap_scale_launch(int L, uint32_t *data, uint64_t stride, uint32_t N,
+                                  int32_t lgN, uint32_t batch, hipStream_t st)
+{
+#define CALLU(Lv) launch_unswap<Lv>(data, stride, N, lgN, batch, st)
+    switch (L) {
+    case 1: return CALLU(1);
+    case 2: return CALLU(2);
+    case 3: return CALLU(3);
+    case 4: return CALLU(4);
+    case 6: return CALLU(6);
+    case 8: return CALLU(8);
+    case 12: return CALLU(12);
+    case 16: return CALLU(16);
+    case 24: return CALLU(24);
+    case 32: return CALLU(32);
+    default: return hipErrorInvalidValue;
+    }
+#undef CALLU
+}
+
+template <int L, bool FULL>
+static hipError_t launch_err(const uint32_t *x, const uint32_t *y, uint64_t stride, uint32_t N,
+                             int P, double *partial, uint32_t batch, hipStream_t st)
+{
+    dim3 grid((2 * N + kErrBlock - 1) / kErrBlock, batch);
+    hipLaunchKernelGGL((mp_err_kernel<L, FULL>), grid, dim3(kErrBlock), 0, st,
+                       x, y, stride, N, P, partial);
+    return hipGetLastError();
+}
+
+hipError_t mp_err_launch(int L, int P, const uint32_t *x, const uint32_t *y, uint64_t stride,
+                         uint32_t N, double *partial, uint32_t batch, hipStream_t st)
+{
+    const bool full = (P == 32 * L);
+#define CALLE(Lv, F) launch_err<Lv, F>(x, y, stride, N, P, partial, batch, st)
+    switch (L) {
+    case 1: return full ? CALLE(1, true) : CALLE(1, false);
+    case 2: return full ? CALLE(2, true) : CALLE(2, false);
+    case 3: return full ? CALLE(3, true) : CALLE(3, false);
+    case 4: return full ? CALLE(4, true) : CALLE(4, false);
+    case 6: return full ? CALLE(6, true) : CALLE(6, false);
+    case 8: return full ? CALLE(8, true) : CALLE(8, false);
+    case 12: return full ? CALLE(12, true) : CALLE(12, false);
+    case 16: return full ? CALLE(16, true) : CALLE(16, false);
+    case 24: return full ? CALLE(24, true) : CALLE(24, false);
+    case 32: return full ? CALLE(32, true) : CALLE(32, false);
+    default: return hipErrorInvalidValue;
+    }
+#undef CALLE
 }
 
 hipError_t mp_binop_launch(int L, int P, int op, uint32_t count, const uint32_t *x,

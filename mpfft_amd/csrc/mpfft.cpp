@@ -193,7 +193,8 @@ struct mpfft_plan_s {
     DevBuf tw, dleaf_sn, dleaf_off, dleaf_n, dlvl_off, dlvl_begin;
     DTables dt{};
     // staging for host / mpfr execution
-    DevBuf d_in, d_out;
+    DevBuf d_in, d_out, d_aux;
+    DevBuf errbuf;                       // per-block partial sums of the error statistics
     std::vector<uint32_t> h_stage_in, h_stage_out;
 
     ~mpfft_plan_s() { if (stream) (void)hipStreamDestroy(stream); }
@@ -236,7 +237,7 @@ int build_mp_plan(mpfft_plan_s *p)
     la.aos_stride = (uint64_t)aw * 2 * N;
     la.pos_sn = p->pos_sn.as<int32_t>(); la.pos_nr = p->pos_nr.as<uint32_t>();
     la.npos = N >= 2 ? (uint32_t)N : 0u;
-    la.N = (uint32_t)N; la.P = (int)p->P;
+    la.N = (uint32_t)N; la.P = (int)p->P; la.in_swap = 0;
     la.in_stride = la.stride = p->words;
     p->levels.clear();
     for (uint32_t l = 3; l <= t.lgN; l++) {
@@ -294,13 +295,17 @@ int build_d_plan(mpfft_plan_s *p)
 }
 
 // stages [s0, s1): 0 = leaves, i >= 1 = combine level of node size 2^(i+2)
+// Stages [s0, s1) of the forward transform; with `inverse` (whole range
+// only) the inverse by the reference's swap trick (serial_mp_fft3.c:367-385):
+// out = swap(FFT(swap(in))) / N, swap exchanging real and imaginary parts.
 int run_mp(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t st,
-           int s0 = 0, int s1 = 1 << 30)
+           int s0 = 0, int s1 = 1 << 30, bool inverse = false)
 {
     const int nst = 1 + (int)p->levels.size();
     if (s1 > nst) s1 = nst;
     if (s0 < 0 || s0 > s1) return MPFFT_ERR_ARG;
-    if (p->N == 1) {
+    if (inverse && (s0 != 0 || s1 != nst)) return MPFFT_ERR_ARG;
+    if (p->N == 1) {                              // both directions: the identity
         if (s0 == 0 && s1 >= 1)
             HIP_TRY(hipMemcpyAsync(d_out, d_in, sizeof(uint32_t) * p->words * p->batch,
                                    hipMemcpyDeviceToDevice, st));
@@ -308,13 +313,62 @@ int run_mp(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t s
     }
     LeafArgs la = p->leaf;
     la.in = d_in; la.data = d_out;
+    la.in_swap = inverse ? 1u : 0u;
     if (s0 > 0) la.npos = 0;                      // no leaf launch
     const int l0 = s0 > 0 ? s0 - 1 : 0, l1 = s1 - 1;
     std::vector<LevelArgs> lv(p->levels.begin() + l0, p->levels.begin() + (l1 > l0 ? l1 : l0));
     for (auto &a : lv) a.data = d_out;
     hipError_t e = mp_fft_launch((int)p->L, (int)p->P, la, lv.data(), (int)lv.size(),
                                  p->batch, st);
+    if (e == hipSuccess && inverse) {
+        int32_t lg = 0;
+        while ((1UL << lg) < p->N) lg++;
+        e = mp_unswap_scale_launch((int)p->L, d_out, p->words, (uint32_t)p->N, lg, p->batch, st);
+    }
     return e == hipSuccess ? MPFFT_OK : MPFFT_ERR_HIP;
+}
+
+// Combine the per-block partials of mp_err_kernel into, per transform,
+// {rRMSE re, rRMSE im, max|aE| re, max|aE| im} and their log2.
+void combine_err(const double *part, uint32_t nblk, uint32_t batch, double *out)
+{
+    for (uint32_t b = 0; b < batch; b++) {
+        for (int h = 0; h < 2; h++) {
+            double S[2] = {0, 0}, E[2] = {0, 0}, lg = -INFINITY;
+            for (int q = 0; q < 2; q++) {
+                double emax = -INFINITY;
+                for (uint32_t k = 0; k < nblk; k++) {
+                    const double *o = part + ((size_t)b * nblk + k) * kErrPartial + 5 * h;
+                    if (o[2 * q] > 0) emax = fmax(emax, o[2 * q + 1]);
+                }
+                double sum = 0;
+                if (emax > -INFINITY)
+                    for (uint32_t k = 0; k < nblk; k++) {
+                        const double *o = part + ((size_t)b * nblk + k) * kErrPartial + 5 * h;
+                        if (o[2 * q] > 0) sum += ldexp(o[2 * q], (int)(o[2 * q + 1] - emax));
+                    }
+                S[q] = sum; E[q] = emax > -INFINITY ? emax : 0;
+            }
+            for (uint32_t k = 0; k < nblk; k++)
+                lg = fmax(lg, part[((size_t)b * nblk + k) * kErrPartial + 5 * h + 4]);
+            double rr, lrr;
+            if (S[0] == 0) { rr = NAN; lrr = NAN; }
+            else if (S[1] == 0) { rr = 0; lrr = -INFINITY; }
+            else {
+                // sqrt(S1 2^E1 / (S0 2^E0)), with the exponent halved exactly
+                const long de = (long)(E[1] - E[0]);
+                const long half = de >= 0 ? de / 2 : -((-de + 1) / 2);
+                const double m = sqrt(S[1] / S[0] * ldexp(1.0, (int)(de - 2 * half)));
+                rr = ldexp(m, (int)half);
+                lrr = log2(m) + (double)half;
+            }
+            double *o = out + 8 * (size_t)b;
+            o[h] = rr;
+            o[2 + h] = lg > -INFINITY ? exp2(lg) : 0.0;
+            o[4 + h] = lrr;
+            o[6 + h] = lg;
+        }
+    }
 }
 
 int ensure_stage(mpfft_plan_s *p)
@@ -464,6 +518,60 @@ int mpfft_execute_limbs(mpfft_plan p, const uint32_t *d_in, uint32_t *d_out, voi
     if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
     HIP_TRY(hipSetDevice(p->device));
     return run_mp(p, d_in, d_out, stream ? (hipStream_t)stream : p->stream);
+}
+
+int mpfft_execute_inverse_limbs(mpfft_plan p, const uint32_t *d_in, uint32_t *d_out,
+                                void *stream)
+{
+    if (!p || !d_in || !d_out) return MPFFT_ERR_ARG;
+    if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    HIP_TRY(hipSetDevice(p->device));
+    return run_mp(p, d_in, d_out, stream ? (hipStream_t)stream : p->stream, 0, 1 << 30, true);
+}
+
+int mpfft_roundtrip_stats(mpfft_plan p, const uint32_t *d_x, const uint32_t *d_y,
+                          void *stream, double *out)
+{
+    if (!p || !d_x || !d_y || !out) return MPFFT_ERR_ARG;
+    if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    std::lock_guard<std::mutex> g(p->mu);
+    HIP_TRY(hipSetDevice(p->device));
+    hipStream_t st = stream ? (hipStream_t)stream : p->stream;
+    const uint32_t nblk = (uint32_t)((2 * p->N + kErrBlock - 1) / kErrBlock);
+    const size_t nd = (size_t)nblk * p->batch * kErrPartial;
+    if (p->errbuf.bytes != nd * sizeof(double)) {
+        int rc = p->errbuf.alloc(nd * sizeof(double));
+        if (rc) return rc;
+    }
+    HIP_TRY(mp_err_launch((int)p->L, (int)p->P, d_x, d_y, p->words, (uint32_t)p->N,
+                          p->errbuf.as<double>(), p->batch, st));
+    std::vector<double> part(nd);
+    HIP_TRY(hipMemcpyAsync(part.data(), p->errbuf.p, nd * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    combine_err(part.data(), nblk, p->batch, out);
+    return MPFFT_OK;
+}
+
+int mpfft_roundtrip_host(mpfft_plan p, const uint32_t *h_x, double *out)
+{
+    if (!p || !h_x || !out) return MPFFT_ERR_ARG;
+    if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    int rc;
+    {
+        std::lock_guard<std::mutex> g(p->mu);
+        HIP_TRY(hipSetDevice(p->device));
+        if ((rc = ensure_stage(p))) return rc;
+        if (p->d_aux.bytes != p->d_in.bytes && (rc = p->d_aux.alloc(p->d_in.bytes))) return rc;
+        HIP_TRY(hipMemcpyAsync(p->d_in.p, h_x, p->d_in.bytes, hipMemcpyHostToDevice, p->stream));
+        if ((rc = run_mp(p, p->d_in.as<uint32_t>(), p->d_out.as<uint32_t>(), p->stream)))
+            return rc;
+        if ((rc = run_mp(p, p->d_out.as<uint32_t>(), p->d_aux.as<uint32_t>(), p->stream,
+                         0, 1 << 30, true)))
+            return rc;
+    }
+    return mpfft_roundtrip_stats(p, p->d_in.as<uint32_t>(), p->d_aux.as<uint32_t>(),
+                                 p->stream, out);
 }
 
 int mpfft_execute_limbs_range(mpfft_plan p, const uint32_t *d_in, uint32_t *d_out,
