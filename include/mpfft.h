@@ -82,7 +82,13 @@ const char *mpfft_strerror(int code);
 
 typedef struct mpfft_plan_s *mpfft_plan;
 
-#define MPFFT_FORM_MP  0   /* multiprecision, CODE 3 op order (3m3a)     */
+#define MPFFT_FORM_MP  0   /* multiprecision, CODE 3 op order (3m3a), the
+                              reference's default (serial_mp_fft3.c, and its
+                              OpenMP twin parallel_mp_fft2.c bound by mpFFT.c) */
+#define MPFFT_FORM_MP_C1 1 /* multiprecision, CODE 1 op order: 4m2a, one
+                              butterfly for every k (serial_mp_fft1.c)      */
+#define MPFFT_FORM_MP_C2 2 /* multiprecision, CODE 2 op order: 4m2a, special
+                              butterflies k = 0 and n/8 (serial_mp_fft2.c)  */
 #define MPFFT_FORM_D5  5   /* double, CODE 5 op order (4m2a, 3 butterflies) */
 #define MPFFT_FORM_D6  6   /* double, CODE 6/7 op order (1 butterfly)      */
 

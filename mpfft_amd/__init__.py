@@ -14,6 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmpfft.so")
 
 FORM_MP, FORM_D5, FORM_D6 = 0, 5, 6
+FORM_MP_C1, FORM_MP_C2 = 1, 2          # the reference's CODE 1 / CODE 2 op orders
+MP_FORMS = (FORM_MP, FORM_MP_C1, FORM_MP_C2)
 OK = 0
 ERRORS = {
     -1: "MPFFT_ERR_SIZE", -2: "MPFFT_ERR_PREC", -3: "MPFFT_ERR_VALUE",
@@ -123,7 +125,7 @@ class Plan:
 
     def execute_host(self, x):
         """x: host array of batch*words (uint32 for mp, float64 for double)."""
-        if self.form == FORM_MP:
+        if self.form in MP_FORMS:
             x = np.ascontiguousarray(x, dtype=np.uint32)
             y = np.empty_like(x)
             assert x.size == self.words * self.batch
@@ -138,7 +140,7 @@ class Plan:
     def execute_device(self, d_in, d_out, stream=None):
         """d_in, d_out: device pointers (ints), e.g. torch tensor data_ptr()."""
         s = ctypes.c_void_p(stream) if stream else None
-        if self.form == FORM_MP:
+        if self.form in MP_FORMS:
             rc = lib().mpfft_execute_limbs(self.h, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out), s)
         else:
             rc = lib().mpfft_execute_f64(self.h, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out), s)

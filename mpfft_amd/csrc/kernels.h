@@ -10,7 +10,11 @@ namespace mpfft {
 
 struct LevelArgs {
     uint32_t *data;             // out buffer of transform 0
-    const uint32_t *lut;        // twiddle buffer
+    const uint32_t *lut;        // twiddle buffer: SoA of lut_m reals, rows of Q = N/4:
+                                // r, s, d (code 3) or r, i (codes 1, 2)
+    uint32_t lut_m;
+    int code;                   // operation order: 3 (3m3a), 2 (4m2a), 1 (4m2a, no
+                                // special butterflies)
     const uint32_t *node_off;   // output offsets of the size-n nodes
     uint64_t stride;            // words between consecutive transforms
     uint32_t N, n, cnt;         // size, node size, number of size-n nodes

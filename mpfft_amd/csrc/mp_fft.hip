@@ -8,9 +8,10 @@
 // Data layout in HBM (per transform): buffer = hdr int32[2N] then limb
 // uint32[L][2N]; element i < N is the real part of X[i], N+i its imaginary
 // part.  Limb-major SoA: a wave reading limb j of 32 consecutive elements
-// touches one 128-byte segment.  Twiddles: hdr[3Q] + limb[L][3Q] with
-// Q = N/4, LUTr at [0,Q), LUTs at [Q,2Q), LUTd at [2Q,3Q) (LUTi is not used
-// by the kernel, serial_mp_fft3.c:79-80).
+// touches one 128-byte segment.  Twiddles (Q = N/4 per row, SoA): for the
+// 3m3a order of CODE 3, LUTr at [0,Q), LUTs at [Q,2Q), LUTd at [2Q,3Q)
+// (LUTi is not used, serial_mp_fft3.c:79-80); for the 4m2a orders of CODE 1
+// and CODE 2 (serial_mp_fft1.c, serial_mp_fft2.c), LUTr then LUTi.
 //
 // Work mapping: two lanes (a DPP quad pair) per butterfly.  The even lane
 // computes m1 = Z+ * w^k, the odd lane m2 = Z- * conj(w^k), with the same
@@ -115,14 +116,14 @@ mp_leaf_kernel(LeafArgs a)
 
 // Two waves per SIMD (256 registers per lane): one wave per SIMD leaves the
 // dependent multiply chains nothing to overlap with.
-template <int L, bool FULL>
+template <int L, bool FULL, bool FOUR_M>
 __global__ void __launch_bounds__(256, 2)
 mp_level_kernel(LevelArgs a)
 {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t pair = tid >> 1, B = tid & 1u;   // B: 0 -> Z+ lane, 1 -> Z- lane
     uint32_t *out = a.data + a.stride * blockIdx.y;
-    const uint32_t N = a.N, n = a.n, q = n >> 2, M = 2 * N, Q = N >> 2, ML = 3 * Q;
+    const uint32_t N = a.N, n = a.n, q = n >> 2, M = 2 * N, Q = N >> 2;
     const int P = a.P;
 
     // node-major: all k of a node on neighbouring lanes, so each load or
@@ -131,7 +132,8 @@ mp_level_kernel(LevelArgs a)
     const uint32_t node = pair / q;
     if (node >= a.cnt) return;
     const uint32_t k = pair - node * q;
-    const uint32_t kind = k == 0 ? 2u : (k == (n >> 3) ? 1u : 0u);
+    // CODE 1 (serial_mp_fft1.c:181-218) has no special butterflies
+    const uint32_t kind = a.code == 1 ? 0u : (k == 0 ? 2u : (k == (n >> 3) ? 1u : 0u));
     const uint32_t o = a.node_off[node];
     const uint32_t zi = o + k + (B ? 3 * q : 2 * q);  // Z+ or Z-
     const uint32_t ui = o + k + (B ? q : 0);          // U0 or U1
@@ -146,40 +148,50 @@ mp_level_kernel(LevelArgs a)
     mp_load(o1, out, M, zi);
     mp_load(o2, out, M, N + zi);
     if (kind != 2) {
-        // pre-additions.  general k (3m3a, serial_mp_fft3.c:220-229):
+        // m4: the general butterfly in the 4m2a order of CODE 1 and CODE 2
+        // (serial_mp_fft2.c:219-230, serial_mp_fft1.c:187-199); otherwise
+        // the 3m3a order of CODE 3.
+        const bool m4 = FOUR_M && kind == 0;
+        // pre-additions.  general k, 3m3a (serial_mp_fft3.c:220-229):
         //   t = Zr +- Zi.  k = n/8 (:209-217): Zr +- Zi and -+Zr + Zi.
         Mp<L> r;
-        const int npre = kind == 0 ? 1 : 2;
+        const int npre = m4 ? 0 : (kind == 0 ? 1 : 2);
 #pragma unroll 1
         for (int it = npre - 1; it >= 0; it--) {
             mp_add<L, FULL>(r, o1, it ? (B ^ 1u) : 0u, o2, it ? 0u : B, P);
             if (it) lds_put(lds, nt, lt, 1, r);        // second pre-sum -> slot 1
         }
-        // products.  general k: T = t*LUTr -> slot 0; m.r = T -+ LUTd*Zi ->
-        //   slot 1; m.i = T - LUTs*Zr (Z+ lane) or LUTs*Zr - T (Z- lane) ->
-        //   slot 0 (T is dead by then).  k = n/8: slot j = pre-sum j times
-        //   LUTr[N/8].  Nothing but the product's own state stays in
-        //   registers across a multiplication.
-        const int nmul = kind == 0 ? 3 : 2;
+        // products; every result goes to an LDS slot, so nothing but the
+        // product's own state stays in registers across a multiplication.
+        //   3m3a: T = t*LUTr -> 0; m.r = T -+ LUTd*Zi -> 1;
+        //         m.i = T - LUTs*Zr (Z+ lane) or LUTs*Zr - T (Z- lane) -> 0.
+        //   k = n/8: pre-sum j times LUTr[N/8] -> j.
+        //   4m2a: A = LUTr*Zr -> 0; m.r = A +- LUTi*Zi -> 1;
+        //         C = LUTr*Zi -> 0; m.i = C -+ LUTi*Zr -> 0.
+        const int nmul = m4 ? 4 : (kind == 0 ? 3 : 2);
 #pragma unroll 1
         for (int it = 0; it < nmul; it++) {
             Mp<L> f, w;
-            uint32_t li = kn;
-            if (it == 0) f = r;
-            else if (kind == 0) mp_load(f, out, M, it == 1 ? N + zi : zi);
-            else lds_get(f, lds, nt, lt, 1);
-            if (kind == 0) li = it == 0 ? kn : (it == 1 ? 2 * Q + kn : Q + kn);
-            mp_load(w, a.lut, ML, li);
+            uint32_t row;                              // twiddle row: r, s|i, d
+            if (it == 0 && !m4) f = r;
+            else if (kind == 1) lds_get(f, lds, nt, lt, 1);
+            else mp_load(f, out, M, (m4 ? (it == 0 || it == 3) : it == 2) ? zi : N + zi);
+            if (kind == 1) row = 0;
+            else if (m4) row = (it & 1) ? Q : 0;
+            else row = it == 0 ? 0 : (it == 1 ? 2 * Q : Q);
+            mp_load(w, a.lut, a.lut_m, row + kn);
             mp_mul<L, FULL>(r, f, w, P);
-            if (kind == 0 && it > 0) {
+            if (m4 ? (it & 1) : (kind == 0 && it > 0)) {
                 Mp<L> T;
                 lds_get(T, lds, nt, lt, 0);
-                mp_add<L, FULL>(r, T, it == 2 ? B : 0u, r, B ^ 1u, P);
+                const uint32_t fx = (!m4 && it == 2) ? B : 0u;
+                const uint32_t fy = (m4 && it == 1) ? B : (B ^ 1u);
+                mp_add<L, FULL>(r, T, fx, r, fy, P);
             }
-            lds_put(lds, nt, lt, (kind == 0 && it == 2) ? 0 : (it ? 1 : 0), r);
+            lds_put(lds, nt, lt, it == 1 ? 1 : 0, r);
         }
-        lds_get(o1, lds, nt, lt, kind == 0 ? 1 : 0);
-        lds_get(o2, lds, nt, lt, kind == 0 ? 0 : 1);
+        lds_get(o1, lds, nt, lt, kind == 1 ? 0 : 1);
+        lds_get(o2, lds, nt, lt, kind == 1 ? 1 : 0);
     }
     // pair exchange: the Z+ lane gets m2, the Z- lane m1; then
     // Z+ lane: a1 = m1 + m2, Z- lane: a2 = m1 - m2 (serial_mp_fft3.c:234-238)
@@ -360,7 +372,10 @@ static hipError_t launch_fft(const LeafArgs &la, const LevelArgs *lv, int nlev,
         dim3 grid((thr + 255) / 256, batch);
         // two parking slots of L+1 words per thread
         const size_t sh = sizeof(uint32_t) * 2 * (L + 1) * 256;
-        hipLaunchKernelGGL((mp_level_kernel<L, FULL>), grid, dim3(256), sh, st, lv[i]);
+        if (lv[i].code == 3)
+            hipLaunchKernelGGL((mp_level_kernel<L, FULL, false>), grid, dim3(256), sh, st, lv[i]);
+        else
+            hipLaunchKernelGGL((mp_level_kernel<L, FULL, true>), grid, dim3(256), sh, st, lv[i]);
     }
     return hipGetLastError();
 }

@@ -38,6 +38,18 @@ constexpr unsigned long kMaxN = 1UL << 24;
 
 bool is_pow2(unsigned long n) { return n && !(n & (n - 1)); }
 
+bool is_mp(int form)
+{
+    return form == MPFFT_FORM_MP || form == MPFFT_FORM_MP_C1 || form == MPFFT_FORM_MP_C2;
+}
+
+// the reference's operation order of a multiprecision form: CODE 3
+// (serial_mp_fft3.c), CODE 1 (serial_mp_fft1.c) or CODE 2 (serial_mp_fft2.c)
+int mp_code(int form)
+{
+    return form == MPFFT_FORM_MP_C1 ? 1 : form == MPFFT_FORM_MP_C2 ? 2 : 3;
+}
+
 // ---- recursion tree (serial_mp_fft3.c:180-182) ----------------------------
 
 struct Tree {
@@ -134,10 +146,11 @@ void mpfr_take(mpfr_ptr x, const uint32_t *hdr, const uint32_t *limb, size_t M, 
 
 // ---- twiddles: the reference's lut() (serial_mp_fft3.c:71-82) ---------------
 
-// r, s, d of k = 0..N/4-1 at precision P, as SoA (M = 3N/4)
-void build_mp_lut(unsigned long N, long P, long L, std::vector<uint32_t> &buf)
+// r, s, d (code 3) or r, i (codes 1, 2) of k = 0..N/4-1 at precision P, as
+// SoA (M = 3N/4 or N/2)
+void build_mp_lut(unsigned long N, long P, long L, int code, std::vector<uint32_t> &buf)
 {
-    const size_t Q = N / 4, M = 3 * Q;
+    const size_t Q = N / 4, M = (code == 3 ? 3 : 2) * Q;
     buf.assign(M * (L + 1), 0);
     uint32_t *hdr = buf.data(), *limb = hdr + M;
     unsigned nt = std::thread::hardware_concurrency();
@@ -156,6 +169,10 @@ void build_mp_lut(unsigned long N, long P, long L, std::vector<uint32_t> &buf)
             mpfr_cos(c, ang, MPFR_RNDN);
             mpfr_sin(s, ang, MPFR_RNDN);
             mpfr_put(c, hdr, limb, M, k, L);
+            if (code != 3) {
+                mpfr_put(s, hdr, limb, M, Q + k, L);
+                continue;
+            }
             mpfr_add(t, c, s, MPFR_RNDN);
             mpfr_put(t, hdr, limb, M, Q + k, L);
             mpfr_sub(t, c, s, MPFR_RNDN);
@@ -225,7 +242,7 @@ int build_mp_plan(mpfft_plan_s *p)
     if ((rc = p->nodes.upload(all))) return rc;
     if (N >= 8) {
         std::vector<uint32_t> lutbuf;
-        build_mp_lut(N, p->P, p->L, lutbuf);
+        build_mp_lut(N, p->P, p->L, mp_code(p->form), lutbuf);
         if ((rc = p->lut.upload(lutbuf))) return rc;
     }
     // element-major staging copy of the input for the leaf gathers
@@ -244,6 +261,8 @@ int build_mp_plan(mpfft_plan_s *p)
         LevelArgs a{};
         const uint32_t n = 1u << l, cnt = (uint32_t)t.level[l].size();
         a.lut = p->lut.as<uint32_t>();
+        a.code = mp_code(p->form);
+        a.lut_m = (uint32_t)((a.code == 3 ? 3 : 2) * (N / 4));
         a.node_off = p->nodes.as<uint32_t>() + first[l];
         a.stride = p->words;
         a.N = (uint32_t)N; a.n = n; a.cnt = cnt;
@@ -373,7 +392,7 @@ void combine_err(const double *part, uint32_t nblk, uint32_t batch, double *out)
 
 int ensure_stage(mpfft_plan_s *p)
 {
-    const size_t bytes = (p->form == MPFFT_FORM_MP ? sizeof(uint32_t) : sizeof(double)) *
+    const size_t bytes = (is_mp(p->form) ? sizeof(uint32_t) : sizeof(double)) *
                          p->words * p->batch;
     if (p->d_in.bytes != bytes) {
         int rc;
@@ -465,7 +484,7 @@ int mpfft_plan_create(mpfft_plan *plan, unsigned long n, long prec, int form, un
     if (!plan || batch == 0) return MPFFT_ERR_ARG;
     *plan = nullptr;
     if (!is_pow2(n) || n > kMaxN) return MPFFT_ERR_SIZE;
-    if (form != MPFFT_FORM_MP && form != MPFFT_FORM_D5 && form != MPFFT_FORM_D6)
+    if (!is_mp(form) && form != MPFFT_FORM_D5 && form != MPFFT_FORM_D6)
         return MPFFT_ERR_FORM;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MPFFT_ERR_NODEV;
@@ -474,7 +493,7 @@ int mpfft_plan_create(mpfft_plan *plan, unsigned long n, long prec, int form, un
     HIP_TRY(hipGetDevice(&p->device));
     HIP_TRY(hipStreamCreate(&p->stream));
     int rc;
-    if (form == MPFFT_FORM_MP) {
+    if (is_mp(form)) {
         if (prec < 2 || prec > kMaxPrec) return MPFFT_ERR_PREC;
         p->P = prec;
         p->L = limbs_for_precision(prec);
@@ -515,7 +534,7 @@ int mpfft_plan_sync(mpfft_plan p)
 int mpfft_execute_limbs(mpfft_plan p, const uint32_t *d_in, uint32_t *d_out, void *stream)
 {
     if (!p || !d_in || !d_out) return MPFFT_ERR_ARG;
-    if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    if (!is_mp(p->form)) return MPFFT_ERR_FORM;
     HIP_TRY(hipSetDevice(p->device));
     return run_mp(p, d_in, d_out, stream ? (hipStream_t)stream : p->stream);
 }
@@ -524,7 +543,7 @@ int mpfft_execute_inverse_limbs(mpfft_plan p, const uint32_t *d_in, uint32_t *d_
                                 void *stream)
 {
     if (!p || !d_in || !d_out) return MPFFT_ERR_ARG;
-    if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    if (!is_mp(p->form)) return MPFFT_ERR_FORM;
     HIP_TRY(hipSetDevice(p->device));
     return run_mp(p, d_in, d_out, stream ? (hipStream_t)stream : p->stream, 0, 1 << 30, true);
 }
@@ -533,7 +552,7 @@ int mpfft_roundtrip_stats(mpfft_plan p, const uint32_t *d_x, const uint32_t *d_y
                           void *stream, double *out)
 {
     if (!p || !d_x || !d_y || !out) return MPFFT_ERR_ARG;
-    if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    if (!is_mp(p->form)) return MPFFT_ERR_FORM;
     std::lock_guard<std::mutex> g(p->mu);
     HIP_TRY(hipSetDevice(p->device));
     hipStream_t st = stream ? (hipStream_t)stream : p->stream;
@@ -556,7 +575,7 @@ int mpfft_roundtrip_stats(mpfft_plan p, const uint32_t *d_x, const uint32_t *d_y
 int mpfft_roundtrip_host(mpfft_plan p, const uint32_t *h_x, double *out)
 {
     if (!p || !h_x || !out) return MPFFT_ERR_ARG;
-    if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    if (!is_mp(p->form)) return MPFFT_ERR_FORM;
     int rc;
     {
         std::lock_guard<std::mutex> g(p->mu);
@@ -578,7 +597,7 @@ int mpfft_execute_limbs_range(mpfft_plan p, const uint32_t *d_in, uint32_t *d_ou
                               void *stream, int stage_begin, int stage_end)
 {
     if (!p || !d_in || !d_out) return MPFFT_ERR_ARG;
-    if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    if (!is_mp(p->form)) return MPFFT_ERR_FORM;
     HIP_TRY(hipSetDevice(p->device));
     return run_mp(p, d_in, d_out, stream ? (hipStream_t)stream : p->stream,
                   stage_begin, stage_end);
@@ -587,13 +606,13 @@ int mpfft_execute_limbs_range(mpfft_plan p, const uint32_t *d_in, uint32_t *d_ou
 int mpfft_plan_stages(mpfft_plan p)
 {
     if (!p) return MPFFT_ERR_ARG;
-    return p->form == MPFFT_FORM_MP ? 1 + (int)p->levels.size() : MPFFT_ERR_FORM;
+    return is_mp(p->form) ? 1 + (int)p->levels.size() : MPFFT_ERR_FORM;
 }
 
 int mpfft_execute_limbs_host(mpfft_plan p, const uint32_t *h_in, uint32_t *h_out)
 {
     if (!p || !h_in || !h_out) return MPFFT_ERR_ARG;
-    if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    if (!is_mp(p->form)) return MPFFT_ERR_FORM;
     std::lock_guard<std::mutex> g(p->mu);
     HIP_TRY(hipSetDevice(p->device));
     int rc = ensure_stage(p);
@@ -634,7 +653,7 @@ int mpfft_limbs_to_mpfr(unsigned long n, long limbs, const uint32_t *h_in, mpfr_
 int mpfft_execute_mpfr(mpfft_plan p, mpfr_t *inr, mpfr_t *ini, mpfr_t *outr, mpfr_t *outi)
 {
     if (!p || !inr || !ini || !outr || !outi) return MPFFT_ERR_ARG;
-    if (p->form != MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    if (!is_mp(p->form)) return MPFFT_ERR_FORM;
     if (p->batch != 1) return MPFFT_ERR_ARG;
     const unsigned long n = p->N;
     for (unsigned long k = 0; k < n; k++) {
@@ -660,7 +679,7 @@ int mpfft_execute_mpfr(mpfft_plan p, mpfr_t *inr, mpfr_t *ini, mpfr_t *outr, mpf
 int mpfft_execute_f64(mpfft_plan p, const double *d_in, double *d_out, void *stream)
 {
     if (!p || !d_in || !d_out) return MPFFT_ERR_ARG;
-    if (p->form == MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    if (is_mp(p->form)) return MPFFT_ERR_FORM;
     HIP_TRY(hipSetDevice(p->device));
     hipError_t e = d_fft_launch(p->dt, d_in, d_out, p->words, p->batch,
                                 stream ? (hipStream_t)stream : p->stream);
@@ -670,7 +689,7 @@ int mpfft_execute_f64(mpfft_plan p, const double *d_in, double *d_out, void *str
 int mpfft_execute_f64_host(mpfft_plan p, const double *h_in, double *h_out)
 {
     if (!p || !h_in || !h_out) return MPFFT_ERR_ARG;
-    if (p->form == MPFFT_FORM_MP) return MPFFT_ERR_FORM;
+    if (is_mp(p->form)) return MPFFT_ERR_FORM;
     std::lock_guard<std::mutex> g(p->mu);
     HIP_TRY(hipSetDevice(p->device));
     int rc = ensure_stage(p);

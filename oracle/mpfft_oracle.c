@@ -8,7 +8,11 @@
  * What it restates (reference = mfrewer/mpFFT):
  *   - lut():            lab/serial/mp/serial_mp_fft3.c:71-82
  *   - mp_sr24_cp_fft:   lab/serial/mp/serial_mp_fft3.c:121-253 (CODE 3: 3m3a,
- *                       three butterflies, recursion break n<=4)
+ *                       three butterflies, recursion break n<=4); the
+ *                       alternative operation orders CODE 2
+ *                       (lab/serial/mp/serial_mp_fft2.c:201-232: 4m2a, three
+ *                       butterflies) and CODE 1 (serial_mp_fft1.c:181-200:
+ *                       4m2a, one butterfly for every k)
  *   - test() round trip lab/serial/mp/serial_mp_fft3.c:346-427 (swap trick)
  *   - ic() IN=2:        lab/serial/mp/serial_mp_fft3.c:104-115
  *   - d_sr24_cp_fft:    lab/serial/double/serial_d_fft5.c:52-62, 93-229 (CODE 5)
@@ -228,14 +232,33 @@ static void mp_tail(mpfr_t *yr, mpfr_t *yi, long o0, long q, orc_tmp *w,
     mpfr_sub(yi[o1], yi[o1], w->a2r, RN);
 }
 
-/* One butterfly k of a size-n combine at output offset off:
- * serial_mp_fft3.c:184-251 */
-static void mp_butterfly(long N, long n, long off, long k, const orc_lut *T,
+/* 4m2a twiddle products (serial_mp_fft2.c:219-230, the same sequence as
+ * serial_mp_fft1.c:187-199): m1 = Z+ w, m2 = Z- conj(w), w = LUTr - i LUTi */
+static void mp_4m2a(long p, long m, long kn, const orc_lut *T,
+                    mpfr_t *yr, mpfr_t *yi, orc_tmp *w)
+{
+    mpfr_mul(w->m1r, T->r[kn], yr[p], RN);
+    mpfr_mul(w->m2r, T->r[kn], yr[m], RN);
+    mpfr_mul(w->m1i, T->r[kn], yi[p], RN);
+    mpfr_mul(w->m2i, T->r[kn], yi[m], RN);
+    mpfr_mul(w->t, T->i[kn], yi[p], RN); mpfr_add(w->m1r, w->m1r, w->t, RN);
+    mpfr_mul(w->t, T->i[kn], yi[m], RN); mpfr_sub(w->m2r, w->m2r, w->t, RN);
+    mpfr_mul(w->t, T->i[kn], yr[p], RN); mpfr_sub(w->m1i, w->m1i, w->t, RN);
+    mpfr_mul(w->t, T->i[kn], yr[m], RN); mpfr_add(w->m2i, w->m2i, w->t, RN);
+}
+
+/* One butterfly k of a size-n combine at output offset off, in the
+ * operation order of CODE 3 (serial_mp_fft3.c:184-251), CODE 2
+ * (serial_mp_fft2.c:184-251) or CODE 1 (serial_mp_fft1.c:181-218). */
+static void mp_butterfly(int code, long N, long n, long off, long k, const orc_lut *T,
                          mpfr_t *yr, mpfr_t *yi, orc_tmp *w)
 {
     long q = n / 4;
     long p = off + k + 2 * q, m = off + k + 3 * q;   /* Z+ and Z- */
-    if (k == 0) {
+    if (code == 1) {
+        /* CODE 1: the general 4m2a butterfly for every k, k = 0 included */
+        mp_4m2a(p, m, k * (N / n), T, yr, yi, w);
+    } else if (k == 0) {
         /* k = 0: no twiddle (:185-199); m1 = Z+, m2 = Z- */
         mpfr_set(w->m1r, yr[p], RN); mpfr_set(w->m1i, yi[p], RN);
         mpfr_set(w->m2r, yr[m], RN); mpfr_set(w->m2i, yi[m], RN);
@@ -247,6 +270,8 @@ static void mp_butterfly(long N, long n, long off, long k, const orc_lut *T,
             mpfr_sub(w->m1i, yi[p], yr[p], RN); mpfr_mul(w->m1i, w->m1i, T->r[kn], RN);
             mpfr_sub(w->m2r, yr[m], yi[m], RN); mpfr_mul(w->m2r, w->m2r, T->r[kn], RN);
             mpfr_add(w->m2i, yr[m], yi[m], RN); mpfr_mul(w->m2i, w->m2i, T->r[kn], RN);
+        } else if (code == 2) {
+            mp_4m2a(p, m, kn, T, yr, yi, w);
         } else {
             /* 3m3a (:220-229): Z+ * conj-free twiddle, Z- * its conjugate */
             mpfr_mul(w->m1r, T->d[kn], yi[p], RN);
@@ -262,8 +287,8 @@ static void mp_butterfly(long N, long n, long off, long k, const orc_lut *T,
     mp_tail(yr, yi, off + k, q, w, w->m1r, w->m1i, w->m2r, w->m2i);
 }
 
-static void mp_fft_run(const orc_tree *t, const orc_lut *T, mpfr_prec_t P,
-                       mpfr_t *xr, mpfr_t *xi, mpfr_t *yr, mpfr_t *yi, int nthr)
+static void mp_fft_run_code(int code, const orc_tree *t, const orc_lut *T, mpfr_prec_t P,
+                            mpfr_t *xr, mpfr_t *xi, mpfr_t *yr, mpfr_t *yi, int nthr)
 {
     long N = t->N;
 #ifdef _OPENMP
@@ -284,12 +309,18 @@ static void mp_fft_run(const orc_tree *t, const orc_lut *T, mpfr_prec_t P,
             #pragma omp for schedule(static)
             for (long b = 0; b < tot; b++) {
                 long node = b / q, k = b % q;
-                mp_butterfly(N, n, t->lvl_off[l][node], k, T, yr, yi, &w);
+                mp_butterfly(code, N, n, t->lvl_off[l][node], k, T, yr, yi, &w);
             }
         }
         tmp_clear(&w);
         mpfr_free_cache();
     }
+}
+
+static void mp_fft_run(const orc_tree *t, const orc_lut *T, mpfr_prec_t P,
+                       mpfr_t *xr, mpfr_t *xi, mpfr_t *yr, mpfr_t *yi, int nthr)
+{
+    mp_fft_run_code(3, t, T, P, xr, xi, yr, yi, nthr);
 }
 
 static mpfr_t *mp_array(long M, mpfr_prec_t P)
@@ -310,11 +341,11 @@ static int is_pow2(unsigned long N) { return N && !(N & (N - 1)); }
  * Input has Lin limbs (values exact at Pw bits, e.g. Pin <= Pw),
  * output has Lout = ceil(Pw/32) limbs.  nthr <= 0 means all threads.
  * Returns 0 or a negative error code. */
-int orc_mp_fft(unsigned long N, long Pw, long Lin,
-               const int32_t *hdr_in, const uint32_t *limb_in,
-               int32_t *hdr_out, uint32_t *limb_out, int nthr)
+int orc_mp_fft_code(int code, unsigned long N, long Pw, long Lin,
+                    const int32_t *hdr_in, const uint32_t *limb_in,
+                    int32_t *hdr_out, uint32_t *limb_out, int nthr)
 {
-    if (!is_pow2(N) || Pw < 2) return -1;
+    if (!is_pow2(N) || Pw < 2 || code < 1 || code > 3) return -1;
     long Lout = (Pw + 31) / 32, M = 2 * (long)N;
     orc_tree t; tree_build(&t, (long)N, 4);
     orc_lut T; lut_build(&T, (long)N, Pw);
@@ -326,7 +357,7 @@ int orc_mp_fft(unsigned long N, long Pw, long Lin,
         rc |= soa_get(xi[i], hdr_in, limb_in, Lin, M, (long)N + i);
     }
     if (!rc) {
-        mp_fft_run(&t, &T, Pw, xr, xi, yr, yi, nthr);
+        mp_fft_run_code(code, &t, &T, Pw, xr, xi, yr, yi, nthr);
         for (long i = 0; i < (long)N; i++) {
             soa_put(yr[i], hdr_out, limb_out, Lout, M, i);
             soa_put(yi[i], hdr_out, limb_out, Lout, M, (long)N + i);
@@ -336,6 +367,13 @@ int orc_mp_fft(unsigned long N, long Pw, long Lin,
     mp_array_free(yr, (long)N); mp_array_free(yi, (long)N);
     lut_free(&T); tree_free(&t);
     return rc ? -2 : 0;
+}
+
+int orc_mp_fft(unsigned long N, long Pw, long Lin,
+               const int32_t *hdr_in, const uint32_t *limb_in,
+               int32_t *hdr_out, uint32_t *limb_out, int nthr)
+{
+    return orc_mp_fft_code(3, N, Pw, Lin, hdr_in, limb_in, hdr_out, limb_out, nthr);
 }
 
 /* Timing entry for the CPU baseline: runs `reps` forward FFTs on data

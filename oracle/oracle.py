@@ -31,6 +31,7 @@ def lib():
         L = ctypes.CDLL(_ORACLE)
         U, Lg, I, P = ctypes.c_ulong, ctypes.c_long, ctypes.c_int, ctypes.c_void_p
         L.orc_mp_fft.argtypes = [U, Lg, Lg, P, P, P, P, I]
+        L.orc_mp_fft_code.argtypes = [I, U, Lg, Lg, P, P, P, P, I]
         L.orc_mp_fft_time.argtypes = [U, Lg, Lg, P, P, I, I]
         L.orc_mp_fft_time.restype = ctypes.c_double
         L.orc_mp_lut.argtypes = [U, Lg, P, P]
@@ -56,15 +57,18 @@ def _split(buf, n, L):
     return buf, hdr, lim
 
 
-def mp_fft(n, P, x, L_in=None, Pw=None, nthr=0):
+def mp_fft(n, P, x, L_in=None, Pw=None, nthr=0, code=3):
     """Forward FFT of SoA transform x (L_in limbs, values exact at Pw bits)
-    computed at Pw bits (default P); output at ceil(Pw/32) limbs."""
+    computed at Pw bits (default P); output at ceil(Pw/32) limbs.  code:
+    the reference's operation order, 3 (serial_mp_fft3.c, 3m3a), 2
+    (serial_mp_fft2.c, 4m2a) or 1 (serial_mp_fft1.c, 4m2a, one butterfly)."""
     L_in = limbs(P) if L_in is None else L_in
     Pw = P if Pw is None else Pw
     Lo = limbs(Pw)
     x, hx, lx = _split(x, n, L_in)
     y = np.zeros(2 * n * (Lo + 1), dtype=np.uint32)
-    rc = lib().orc_mp_fft(n, Pw, L_in, _p(hx), _p(lx), _p(y[:2 * n]), _p(y[2 * n:]), nthr)
+    rc = lib().orc_mp_fft_code(code, n, Pw, L_in, _p(hx), _p(lx), _p(y[:2 * n]),
+                               _p(y[2 * n:]), nthr)
     if rc:
         raise RuntimeError(f"orc_mp_fft rc={rc}")
     return y
@@ -155,7 +159,8 @@ def d_fft(code, n, x):
 
 # ---- reference builds (oracle/_ref, compiled from the reference sources) ----
 
-REF_MP = {"mp3": "libref_mp3.so", "mp4": "libref_mp4.so", "mpp2": "libref_mpp2.so"}
+REF_MP = {"mp1": "libref_mp1.so", "mp2": "libref_mp2.so", "mp3": "libref_mp3.so",
+          "mp4": "libref_mp4.so", "mpp2": "libref_mpp2.so"}
 REF_D = {5: "libref_d5.so", 6: "libref_d6.so", 7: "libref_d7.so"}
 
 

@@ -3,8 +3,9 @@
  * TEST INFRASTRUCTURE ONLY.
  *
  * oracle/Makefile compiles this file with -DREF_KERNEL naming one of the
- * reference's own kernel files where it lies (lab/serial/mp/serial_mp_fft3.c,
- * serial_mp_fft4.c or lab/parallel/mp/parallel_mp_fft2.c) and -I pointing at
+ * reference's own kernel files where it lies (lab/serial/mp/serial_mp_fft1.c
+ * to serial_mp_fft4.c or lab/parallel/mp/parallel_mp_fft2.c; REF_4M2A marks
+ * the 4m2a codes 1 and 2, whose lut() keeps only LUTr and LUTi) and -I pointing at
  * the reference root for mpFFT.h.  The reference is configured through
  * compile-time macros (mpFFT.c:12-41); here N and P are bound to variables so
  * one library serves every size.  Output goes to oracle/_ref/ only.
@@ -42,6 +43,17 @@ static int g_in = 2;
 #endif
 
 #include REF_KERNEL
+
+static void lut_clear(void)
+{
+    for (unsigned long k = 0; k < N / 4; k++) {
+#ifdef REF_4M2A
+        mpfr_clears(LUTr[k], LUTi[k], (mpfr_ptr)0);
+#else
+        mpfr_clears(LUTr[k], LUTi[k], LUTs[k], LUTd[k], (mpfr_ptr)0);
+#endif
+    }
+}
 
 /* Run the reference forward FFT once.
  * in_mode 0: input taken from (hdr_in, limb_in) SoA at P bits;
@@ -88,8 +100,7 @@ int ref_mp_run(unsigned long n, long p, int in_mode,
     cnt[0] = cmul; cnt[1] = cadd; cnt[2] = cfc; cnt[3] = crb;
     for (unsigned long k = 0; k < n; k++)
         mpfr_clears(inr[k], ini[k], outr[k], outi[k], (mpfr_ptr)0);
-    for (unsigned long k = 0; k < n / 4; k++)
-        mpfr_clears(LUTr[k], LUTi[k], LUTs[k], LUTd[k], (mpfr_ptr)0);
+    lut_clear();
     clear_memory();
     mpfr_free_cache();
     return rc ? -2 : 0;
@@ -105,13 +116,14 @@ int ref_mp_lut(unsigned long n, long p, int32_t *hdr, uint32_t *limb)
     for (long k = 0; k < q; k++) {
         soa_put(LUTr[k], hdr, limb, L, (long)n, k);
         soa_put(LUTi[k], hdr, limb, L, (long)n, q + k);
+#ifndef REF_4M2A
         soa_put(LUTs[k], hdr, limb, L, (long)n, 2 * q + k);
         soa_put(LUTd[k], hdr, limb, L, (long)n, 3 * q + k);
+#endif
     }
     for (unsigned long k = 0; k < n; k++)
         mpfr_clears(inr[k], ini[k], outr[k], outi[k], (mpfr_ptr)0);
-    for (unsigned long k = 0; k < n / 4; k++)
-        mpfr_clears(LUTr[k], LUTi[k], LUTs[k], LUTd[k], (mpfr_ptr)0);
+    lut_clear();
     clear_memory();
     mpfr_free_cache();
     return 0;
@@ -136,8 +148,7 @@ double ref_mp_time(unsigned long n, long p, const int32_t *hdr_in, const uint32_
     double dt = 1e-9 * (double)(get_time_int() - t0) / reps;
     for (unsigned long k = 0; k < n; k++)
         mpfr_clears(inr[k], ini[k], outr[k], outi[k], (mpfr_ptr)0);
-    for (unsigned long k = 0; k < n / 4; k++)
-        mpfr_clears(LUTr[k], LUTi[k], LUTs[k], LUTd[k], (mpfr_ptr)0);
+    lut_clear();
     clear_memory();
     mpfr_free_cache();
     return dt;

@@ -6,7 +6,8 @@ inputs of mpfft_amd.inputs and on the reference's own IN=2 input, and
 writes:
 
   mp_golden.npz      multiprecision outputs (SoA limbs, include/mpfft.h
-                     layout) of serial CODE 3 (serial_mp_fft3.c:121-253)
+                     layout) of serial CODE 3 (serial_mp_fft3.c:121-253),
+                     and of CODE 1 / CODE 2 (serial_mp_fft1.c, serial_mp_fft2.c)
   d_golden.npz       double outputs of CODE 5 / 6 / 7 (serial_d_fft5/6/7.c)
   digests.json       SHA-256 of large outputs, input digests, op counters
 
@@ -93,6 +94,19 @@ def main():
     x, y, _ = O.ref_mp("mp3", 1024, 1024)
     mp["in2_out_N1024_P1024"] = y
     dig["in2_in_sha_N1024_P1024"] = sha(x)
+    # the alternative operation orders CODE 1 and CODE 2 (serial_mp_fft1.c,
+    # serial_mp_fft2.c) on the same seeded inputs as the CODE 3 fixtures
+    dig["mp_codes_large"] = {}
+    for code in (1, 2):
+        for n in MP_SIZES:
+            for P in (64, 256) + ((1024,) if n <= 64 else ()):
+                x = I.mp_random(n, P, seed=SEED)
+                _, y, _ = O.ref_mp(f"mp{code}", n, P, x)
+                mp[f"c{code}_out_N{n}_P{P}"] = y
+        n, P = 1 << 16, 256
+        x = I.mp_random(n, P, seed=SEED)
+        _, y, _ = O.ref_mp(f"mp{code}", n, P, x)
+        dig["mp_codes_large"][f"c{code}_N{n}_P{P}"] = {"in_sha": sha(x), "out_sha": sha(y)}
     for n, P in MP_LARGE:
         x = I.mp_random(n, P, seed=SEED)
         _, y, cnt = O.ref_mp("mp3", n, P, x)

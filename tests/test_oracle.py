@@ -34,6 +34,61 @@ def test_oracle_matches_reference_golden(n, P, mp_golden, digests):
     assert np.array_equal(y, mp_golden[f"out_N{n}_P{P}"])
 
 
+CODE_CASES = [(c, n, P) for c in (1, 2) for n in MP_SIZES
+              for P in (64, 256) + ((1024,) if n <= 64 else ())]
+
+
+@pytest.mark.parametrize("code,n,P", CODE_CASES)
+def test_oracle_codes_match_reference_golden(code, n, P, mp_golden):
+    # the alternative operation orders: serial_mp_fft1.c (4m2a, one
+    # butterfly) and serial_mp_fft2.c (4m2a, three butterflies)
+    x = I.mp_random(n, P, seed=I.DEFAULT_SEED)
+    assert np.array_equal(O.mp_fft(n, P, x, code=code), mp_golden[f"c{code}_out_N{n}_P{P}"])
+
+
+@pytest.mark.parametrize("code", [1, 2])
+def test_oracle_codes_large_digest(code, digests):
+    n, P = 1 << 16, 256
+    x = I.mp_random(n, P, seed=digests["seed"])
+    d = digests["mp_codes_large"][f"c{code}_N{n}_P{P}"]
+    assert sha(x) == d["in_sha"]
+    assert sha(O.mp_fft(n, P, x, code=code)) == d["out_sha"]
+
+
+def test_codes_agree_where_no_general_butterfly_runs():
+    # up to n = 8 every combine butterfly is k = 0 or k = n/8, which CODE 2
+    # and CODE 3 treat alike; CODE 1 runs the general formula there too
+    P = 256
+    for n in (1, 2, 4, 8):
+        x = I.mp_random(n, P, seed=30 + n)
+        assert np.array_equal(O.mp_fft(n, P, x, code=2), O.mp_fft(n, P, x, code=3))
+    x = I.mp_random(16, P, seed=31)
+    assert not np.array_equal(O.mp_fft(16, P, x, code=2), O.mp_fft(16, P, x, code=3))
+    x = I.mp_random(64, P, seed=32)
+    assert not np.array_equal(O.mp_fft(64, P, x, code=1), O.mp_fft(64, P, x, code=2))
+
+
+@pytest.mark.parametrize("code", [1, 2])
+@pytest.mark.parametrize("n", [16, 256])
+def test_codes_within_bound_of_2p(code, n):
+    P = 128
+    x = I.mp_random(n, P, seed=n + code)
+    y = O.mp_fft(n, P, x, code=code)
+    y2 = O.mp_fft(n, P, x, Pw=2 * P)
+    maxulp, norm, zb = O.err_stats(n, P, y, O.limbs(P), y2, O.limbs(2 * P))
+    assert zb == 0 and norm < 8.0
+
+
+@pytest.mark.parametrize("code", [1, 2])
+def test_codes_impulse_gives_all_ones(code):
+    n, P = 64, 256
+    x = _mp_const(n, P, 0.0, 0.0)
+    hdr, lim = I.split(x, n, O.limbs(P))
+    hdr[0] = 1 << 1
+    lim[0, 0] = 0x80000000
+    assert np.all(I.to_float(O.mp_fft(n, P, x, code=code), n, O.limbs(P)) == 1.0)
+
+
 @pytest.mark.parametrize("name", ["impulse", "constant", "zeros"])
 @pytest.mark.parametrize("n", [1, 2, 4, 8, 16, 64])
 @pytest.mark.parametrize("P", [128, 1024])
@@ -188,4 +243,7 @@ def test_oracle_vs_live_reference():
     for n, P in [(512, 200), (128, 96), (2048, 333)]:
         x = I.mp_random(n, P, seed=n + P)
         assert np.array_equal(O.mp_fft(n, P, x), O.ref_mp("mp3", n, P, x)[1])
+        for code in (1, 2):
+            assert np.array_equal(O.mp_fft(n, P, x, code=code),
+                                  O.ref_mp(f"mp{code}", n, P, x)[1])
     assert np.array_equal(O.mp_lut(256, 512), O.ref_mp_lut(256, 512))
