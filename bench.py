@@ -95,6 +95,35 @@ def measure(M, I, torch, n, P, batch, steps, warmup, seed):
                 out0=I.relimb(out0, n, plan_L, Lc), x0=I.relimb(x0, n, plan_L, Lc), L=plan_L)
 
 
+def measure_mpfr(M, I, n, P, seed, reps=3):
+    """the drop-in path: one transform from and to mpfr_t arrays
+    (marshalling, PCIe transfers and the GPU transform), per call"""
+    import ctypes
+    from mpfft_amd import mpfr_arrays as MA
+    Lc = I.limbs_for(P)
+    x = I.mp_random(n, P, seed=seed)
+    inr, ini = MA.pair_from_limbs(n, P, x, Lc)
+    outr, outi = MA.MpfrArray(n, P), MA.MpfrArray(n, P)
+    plan = M.Plan(n, P, M.FORM_MP, 1)
+    call = lambda: M.check(M.lib().mpfft_execute_mpfr(plan.h, inr.ptr(), ini.ptr(),
+                                                       outr.ptr(), outi.ptr()), "execute_mpfr")
+    call()
+    t = np.zeros(4)
+    w0 = time.perf_counter()
+    acc = np.zeros(4)
+    for _ in range(reps):
+        call()
+        M.check(M.lib().mpfft_plan_last_timing(plan.h, ctypes.c_void_p(t.ctypes.data)), "timing")
+        acc += t
+    wall = (time.perf_counter() - w0) / reps * 1e3
+    plan.close()
+    acc /= reps
+    return {"N": n, "precision_bits": P, "call_ms": round(wall, 3),
+            "marshal_in_ms": round(acc[0], 3), "gpu_transform_ms": round(acc[1], 3),
+            "marshal_out_ms": round(acc[2], 3),
+            "note": "one transform per call from mpfr_t arrays, PCIe transfers included"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +136,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0x6D7046465420)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-err", action="store_true", help="skip the 2P-bit oracle error")
+    ap.add_argument("--no-mpfr", action="store_true", help="skip the mpfr_t-array call timing")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -177,6 +207,9 @@ def main():
         r2 = measure(M, I, torch, n, a.prec2, B, max(2, a.steps // 2), 1, a.seed)
         line["forms"][f"plain_P{a.prec2}"] = {
             "fft_per_s": round(B * max(2, a.steps // 2) / (r2["total_ms"] * 1e-3), 3)}
+
+    if not a.no_mpfr:
+        line["mpfr_path"] = measure_mpfr(M, I, n, P, a.seed)
 
     from oracle import oracle as O
     if not a.no_err:

@@ -102,9 +102,16 @@ int mpfft_plan_destroy(mpfft_plan plan);
 /* limbs per real used by the device format, and words per transform */
 int mpfft_plan_info(mpfft_plan plan, long *limbs, size_t *words_per_transform);
 
-/* One transform through mpfr_t arrays (marshalling included). */
+/* One transform through mpfr_t arrays (marshalling included).  The
+ * significands are marshalled on up to 16 host threads into page-locked
+ * staging, in chunks whose transfers overlap the conversion of the next. */
 int mpfft_execute_mpfr(mpfft_plan plan, mpfr_t *inr, mpfr_t *ini,
                        mpfr_t *outr, mpfr_t *outi);
+
+/* Wall times of the plan's last mpfft_execute_mpfr call, in ms:
+ * ms[0] marshalling in (conversion and upload), ms[1] the transform on the
+ * GPU, ms[2] download and conversion out, ms[3] total. */
+int mpfft_plan_last_timing(mpfft_plan plan, double *ms);
 
 /* `batch` transforms on device-resident SoA-limb buffers (d_in and d_out
  * hold batch * words_per_transform words each; they must not overlap).

@@ -33,6 +33,7 @@ EXPORTS = [
     "mpfft_plan_sync", "mpfft_mpfr_to_limbs", "mpfft_limbs_to_mpfr",
     "mpfft_mp_binop_host", "mpfft_limbs_for_precision",
     "mpfft_execute_inverse_limbs", "mpfft_roundtrip_stats", "mpfft_roundtrip_host",
+    "mpfft_plan_last_timing",
 ]
 
 _lib = None
@@ -74,6 +75,7 @@ def lib():
     L.mpfft_execute_inverse_limbs.argtypes = [VP, P, P, P]
     L.mpfft_roundtrip_stats.argtypes = [VP, P, P, P, P]
     L.mpfft_roundtrip_host.argtypes = [VP, P, P]
+    L.mpfft_plan_last_timing.argtypes = [VP, P]
     L.mpfft_mp_binop_host.argtypes = [I, U, L_, P, P, P]
     L.mpfft_limbs_for_precision.argtypes = [L_]
     L.mpfft_limbs_for_precision.restype = L_

@@ -10,6 +10,8 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -95,6 +97,23 @@ struct DevBuf {
         if (!v.empty())
             HIP_TRY(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
         return MPFFT_OK;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+// page-locked host memory for the staging of mpfr_t data (transfers at full
+// PCIe rate, asynchronous with respect to the host)
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~PinnedBuf() { if (p) (void)hipHostFree(p); }
+    int alloc(size_t b)
+    {
+        if (b == bytes && p) return MPFFT_OK;
+        if (p) { (void)hipHostFree(p); p = nullptr; }
+        bytes = b;
+        return hipHostMalloc(&p, b, hipHostMallocDefault) == hipSuccess ? MPFFT_OK
+                                                                        : MPFFT_ERR_HIP;
     }
     template <class T> T *as() const { return (T *)p; }
 };
@@ -212,9 +231,15 @@ struct mpfft_plan_s {
     // staging for host / mpfr execution
     DevBuf d_in, d_out, d_aux;
     DevBuf errbuf;                       // per-block partial sums of the error statistics
-    std::vector<uint32_t> h_stage_in, h_stage_out;
+    PinnedBuf h_stage_in, h_stage_out;
+    std::vector<hipEvent_t> chunk_ev;    // one per marshalling chunk (mpfr_t path)
+    double timing[4] = {0, 0, 0, 0};     // last mpfr_t call: marshal in, compute, marshal out, total (ms)
 
-    ~mpfft_plan_s() { if (stream) (void)hipStreamDestroy(stream); }
+    ~mpfft_plan_s()
+    {
+        for (auto e : chunk_ev) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
 };
 
 namespace {
@@ -662,18 +687,111 @@ int mpfft_execute_mpfr(mpfft_plan p, mpfr_t *inr, mpfr_t *ini, mpfr_t *outr, mpf
             return MPFFT_ERR_PREC;
     }
     std::lock_guard<std::mutex> g(p->mu);
-    p->h_stage_in.resize(p->words);
-    p->h_stage_out.resize(p->words);
-    int rc = mpfft_mpfr_to_limbs(n, p->L, inr, ini, p->h_stage_in.data());
-    if (rc) return rc;
     HIP_TRY(hipSetDevice(p->device));
+    int rc;
     if ((rc = ensure_stage(p))) return rc;
     const size_t bytes = sizeof(uint32_t) * p->words;
-    HIP_TRY(hipMemcpyAsync(p->d_in.p, p->h_stage_in.data(), bytes, hipMemcpyHostToDevice, p->stream));
-    if ((rc = run_mp(p, p->d_in.as<uint32_t>(), p->d_out.as<uint32_t>(), p->stream))) return rc;
-    HIP_TRY(hipMemcpyAsync(p->h_stage_out.data(), p->d_out.p, bytes, hipMemcpyDeviceToHost, p->stream));
-    HIP_TRY(hipStreamSynchronize(p->stream));
-    return mpfft_limbs_to_mpfr(n, p->L, p->h_stage_out.data(), outr, outi);
+    if ((rc = p->h_stage_in.alloc(bytes)) || (rc = p->h_stage_out.alloc(bytes))) return rc;
+
+    // Marshalling in chunks of points, on worker threads.  Input: as each
+    // chunk is converted, its rows go to the device (two strided copies,
+    // real and imaginary part) while the next chunks convert.  Output: the
+    // chunks come back one by one and convert as they land.
+    const long L = p->L;
+    const size_t M = 2 * n, pitch = sizeof(uint32_t) * M;
+    unsigned nthr = std::thread::hardware_concurrency();
+    nthr = std::max(1u, std::min({nthr, 16u, (unsigned)(n / 512 ? n / 512 : 1)}));
+    const unsigned C = n >= 4096 ? 4 * nthr : 1;              // chunks
+    const unsigned long cl = (n + C - 1) / C;
+    while (p->chunk_ev.size() < C) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        p->chunk_ev.push_back(e);
+    }
+    uint32_t *hi = p->h_stage_in.as<uint32_t>(), *ho = p->h_stage_out.as<uint32_t>();
+    uint32_t *di = p->d_in.as<uint32_t>(), *dout = p->d_out.as<uint32_t>();
+    auto t0 = std::chrono::steady_clock::now();
+
+    std::vector<std::atomic<int>> state(C);                    // 0 pending, 1 done, <0 error
+    for (auto &x : state) x.store(0);
+    std::vector<std::thread> th;
+    for (unsigned w = 0; w < nthr; w++)
+        th.emplace_back([&, w] {
+            for (unsigned c = w; c < C; c += nthr) {
+                const unsigned long k0 = c * cl, k1 = std::min(n, k0 + cl);
+                int r = MPFFT_OK;
+                for (unsigned long k = k0; k < k1 && !r; k++) {
+                    r = mpfr_put(inr[k], hi, hi + M, M, k, L);
+                    if (!r) r = mpfr_put(ini[k], hi, hi + M, M, n + k, L);
+                }
+                state[c].store(r ? r : 1, std::memory_order_release);
+            }
+        });
+    for (unsigned c = 0; c < C && !rc; c++) {
+        int v;
+        while ((v = state[c].load(std::memory_order_acquire)) == 0) std::this_thread::yield();
+        if (v < 0) { rc = v; break; }
+        const unsigned long k0 = c * cl, k1 = std::min(n, k0 + cl);
+        if (k1 <= k0) continue;
+        for (unsigned long base : {0UL, n})
+            if (hipMemcpy2DAsync(di + base + k0, pitch, hi + base + k0, pitch,
+                                 sizeof(uint32_t) * (k1 - k0), (size_t)L + 1,
+                                 hipMemcpyHostToDevice, p->stream) != hipSuccess)
+                rc = MPFFT_ERR_HIP;
+    }
+    for (auto &x : th) x.join();
+    th.clear();
+    if (rc) { (void)hipStreamSynchronize(p->stream); return rc; }
+    auto t1 = std::chrono::steady_clock::now();
+
+    hipEvent_t c0, c1;
+    HIP_TRY(hipEventCreate(&c0));
+    HIP_TRY(hipEventCreate(&c1));
+    HIP_TRY(hipEventRecord(c0, p->stream));
+    rc = run_mp(p, di, dout, p->stream);
+    HIP_TRY(hipEventRecord(c1, p->stream));
+    if (rc) { (void)hipEventDestroy(c0); (void)hipEventDestroy(c1); return rc; }
+    for (unsigned c = 0; c < C; c++) {
+        const unsigned long k0 = c * cl, k1 = std::min(n, k0 + cl);
+        if (k1 > k0)
+            for (unsigned long base : {0UL, n})
+                HIP_TRY(hipMemcpy2DAsync(ho + base + k0, pitch, dout + base + k0, pitch,
+                                         sizeof(uint32_t) * (k1 - k0), (size_t)L + 1,
+                                         hipMemcpyDeviceToHost, p->stream));
+        HIP_TRY(hipEventRecord(p->chunk_ev[c], p->stream));
+    }
+    std::atomic<int> err{MPFFT_OK};
+    for (unsigned w = 0; w < nthr; w++)
+        th.emplace_back([&, w] {
+            for (unsigned c = w; c < C; c += nthr) {
+                if (hipEventSynchronize(p->chunk_ev[c]) != hipSuccess) { err = MPFFT_ERR_HIP; return; }
+                const unsigned long k0 = c * cl, k1 = std::min(n, k0 + cl);
+                for (unsigned long k = k0; k < k1; k++) {
+                    mpfr_take(outr[k], ho, ho + M, M, k, L);
+                    mpfr_take(outi[k], ho, ho + M, M, n + k, L);
+                }
+            }
+        });
+    for (auto &x : th) x.join();
+    auto t2 = std::chrono::steady_clock::now();
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c0, c1);
+    (void)hipEventDestroy(c0);
+    (void)hipEventDestroy(c1);
+    const double tin = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    const double ttot = std::chrono::duration<double, std::milli>(t2 - t0).count();
+    p->timing[0] = tin;
+    p->timing[1] = ms;
+    p->timing[2] = ttot - tin - ms;
+    p->timing[3] = ttot;
+    return err.load();
+}
+
+int mpfft_plan_last_timing(mpfft_plan p, double *ms)
+{
+    if (!p || !ms) return MPFFT_ERR_ARG;
+    for (int i = 0; i < 4; i++) ms[i] = p->timing[i];
+    return MPFFT_OK;
 }
 
 int mpfft_execute_f64(mpfft_plan p, const double *d_in, double *d_out, void *stream)

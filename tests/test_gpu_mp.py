@@ -12,6 +12,7 @@ import pytest
 
 import mpfft_amd as M
 from mpfft_amd import inputs as I
+from mpfft_amd import mpfr_arrays as MA
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -145,48 +146,12 @@ def test_device_pointer_execution():
 
 # ---- the reference entry point on mpfr_t arrays -----------------------------
 
-class MpfrArray:
-    """n mpfr_t at precision P allocated through libmpfr."""
-    SIZE = 32   # sizeof(__mpfr_struct) on x86-64
-
-    _mpfr = None
-
-    @classmethod
-    def mpfr(cls):
-        if cls._mpfr is None:
-            cls._mpfr = ctypes.CDLL("libmpfr.so.6")
-        return cls._mpfr
-
-    def __init__(self, n, P):
-        self.n = n
-        self.buf = (ctypes.c_char * (self.SIZE * max(n, 1)))()
-        self.base = ctypes.addressof(self.buf)
-        for k in range(n):
-            self.mpfr().mpfr_init2(ctypes.c_void_p(self.base + self.SIZE * k), ctypes.c_long(P))
-
-    def ptr(self, k=0):
-        return ctypes.c_void_p(self.base + self.SIZE * k)
-
-    def __del__(self):
-        for k in range(self.n):
-            self.mpfr().mpfr_clear(self.ptr(k))
-
-
 def mpfr_pair_from_soa(n, P, x):
-    re, im = MpfrArray(n, P), MpfrArray(n, P)
-    L = O.limbs(P)
-    xx = np.ascontiguousarray(x, dtype=np.uint32)
-    M.check(M.lib().mpfft_limbs_to_mpfr(n, L, ctypes.c_void_p(xx.ctypes.data),
-                                         re.ptr(), im.ptr()), "limbs_to_mpfr")
-    return re, im
+    return MA.pair_from_limbs(n, P, x, O.limbs(P))
 
 
 def soa_from_mpfr_pair(n, P, re, im):
-    L = O.limbs(P)
-    y = np.zeros(2 * n * (L + 1), dtype=np.uint32)
-    M.check(M.lib().mpfft_mpfr_to_limbs(n, L, re.ptr(), im.ptr(),
-                                         ctypes.c_void_p(y.ctypes.data)), "mpfr_to_limbs")
-    return y
+    return MA.limbs_from_pair(n, O.limbs(P), re, im)
 
 
 @pytest.mark.parametrize("entry", ["v1", "v2"])
@@ -194,7 +159,7 @@ def soa_from_mpfr_pair(n, P, re, im):
 def test_mpfr_entry_point_matches_reference(entry, n, P, mp_golden, digests):
     x = I.mp_random(n, P, seed=digests["seed"])
     inr, ini = mpfr_pair_from_soa(n, P, x)
-    outr, outi = MpfrArray(n, P), MpfrArray(n, P)
+    outr, outi = MA.MpfrArray(n, P), MA.MpfrArray(n, P)
     lib = M.lib()
     if entry == "v1":
         lib.mp_sr24_cp_fft(n, 1, 0, inr.ptr(), ini.ptr(), outr.ptr(), outi.ptr())
@@ -207,18 +172,48 @@ def test_mpfr_entry_point_matches_reference(entry, n, P, mp_golden, digests):
     assert np.array_equal(soa_from_mpfr_pair(n, P, inr, ini), x)
 
 
+def test_mpfr_entry_point_large_chunked(digests):
+    # N = 2^16 at 1024 bits: the marshalling runs in chunks on several host
+    # threads, overlapped with the transfers; the result is the reference's
+    n, P = 1 << 16, 1024
+    x = I.mp_random(n, P, seed=digests["seed"])
+    assert sha(x) == digests["mp_large"][f"N{n}_P{P}"]["in_sha"]
+    inr, ini = mpfr_pair_from_soa(n, P, x)
+    outr, outi = MA.MpfrArray(n, P), MA.MpfrArray(n, P)
+    lib = M.lib()
+    lib.mp_sr24_cp_fft(n, 1, 0, inr.ptr(), ini.ptr(), outr.ptr(), outi.ptr())
+    assert lib.mpfft_last_error() == 0
+    y = soa_from_mpfr_pair(n, P, outr, outi)
+    assert sha(y) == digests["mp_large"][f"N{n}_P{P}"]["out_sha"]
+
+
+def test_mpfr_plan_timing():
+    n, P = 8192, 256
+    plan = M.Plan(n, P, M.FORM_MP, 1)
+    x = I.mp_random(n, P, seed=5)
+    inr, ini = mpfr_pair_from_soa(n, P, x)
+    outr, outi = MA.MpfrArray(n, P), MA.MpfrArray(n, P)
+    M.check(M.lib().mpfft_execute_mpfr(plan.h, inr.ptr(), ini.ptr(), outr.ptr(), outi.ptr()),
+            "execute_mpfr")
+    t = np.zeros(4)
+    M.check(M.lib().mpfft_plan_last_timing(plan.h, ctypes.c_void_p(t.ctypes.data)), "timing")
+    plan.close()
+    assert np.all(t > 0) and t[3] >= t[0] + t[1]
+    assert np.array_equal(soa_from_mpfr_pair(n, P, outr, outi), O.mp_fft(n, P, x))
+
+
 def test_mpfr_entry_point_errors():
     lib = M.lib()
     n, P = 16, 128
     x = I.mp_random(n, P, seed=5)
     inr, ini = mpfr_pair_from_soa(n, P, x)
-    outr, outi = MpfrArray(n, P), MpfrArray(n, P)
-    MpfrArray.mpfr().mpfr_set_nan(inr.ptr(3))
+    outr, outi = MA.MpfrArray(n, P), MA.MpfrArray(n, P)
+    MA.MpfrArray.mpfr().mpfr_set_nan(inr.ptr(3))
     lib.mp_sr24_cp_fft(n, 1, 0, inr.ptr(), ini.ptr(), outr.ptr(), outi.ptr())
     assert lib.mpfft_last_error() == -3
     # arrays of different precision
     inr2, ini2 = mpfr_pair_from_soa(n, P, x)
-    out_other = MpfrArray(n, P + 64)
+    out_other = MA.MpfrArray(n, P + 64)
     lib.mp_sr24_cp_fft(n, 1, 0, inr2.ptr(), ini2.ptr(), outr.ptr(), out_other.ptr())
     assert lib.mpfft_last_error() == -2
     # and a good call right after still works
