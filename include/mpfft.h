@@ -166,6 +166,21 @@ int mpfft_mpfr_to_limbs(unsigned long n, long limbs, mpfr_t *re, mpfr_t *im,
 int mpfft_limbs_to_mpfr(unsigned long n, long limbs, const uint32_t *h_in,
                         mpfr_t *re, mpfr_t *im);
 
+/* Exact text form: `count` reals of a SoA-limb array (hdr[count], then
+ * limbs[limbs][count]) as hexadecimal floats in the format MPFR prints for
+ * "%Ra" (e.g. "0x1.8p-4", "-0x0p+0"), one per line.  Writes at most
+ * `bufsize` bytes including the terminating NUL; *needed receives the size
+ * required (MPFFT_ERR_ARG when buf is too small). */
+int mpfft_limbs_to_hex(unsigned long count, long limbs, const uint32_t *h_in, char *buf,
+                       size_t bufsize, size_t *needed);
+
+/* Parse `count` whitespace-separated hexadecimal floats ("%Ra" format, as
+ * accepted by mpfr_strtofr in base 16) into a SoA-limb array, rounding each
+ * to `prec` bits, to nearest with ties to even.  MPFFT_ERR_VALUE on
+ * malformed text or an exponent outside the accepted range. */
+int mpfft_hex_to_limbs(unsigned long count, long limbs, long prec, const char *text,
+                       uint32_t *h_out);
+
 /* ---- device arithmetic check ---------------------------------------------- */
 
 /* z[i] = x[i] op y[i] correctly rounded at prec bits on the GPU; op 0 add,

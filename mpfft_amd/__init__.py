@@ -33,7 +33,7 @@ EXPORTS = [
     "mpfft_plan_sync", "mpfft_mpfr_to_limbs", "mpfft_limbs_to_mpfr",
     "mpfft_mp_binop_host", "mpfft_limbs_for_precision",
     "mpfft_execute_inverse_limbs", "mpfft_roundtrip_stats", "mpfft_roundtrip_host",
-    "mpfft_plan_last_timing",
+    "mpfft_plan_last_timing", "mpfft_limbs_to_hex", "mpfft_hex_to_limbs",
 ]
 
 _lib = None
@@ -76,6 +76,9 @@ def lib():
     L.mpfft_roundtrip_stats.argtypes = [VP, P, P, P, P]
     L.mpfft_roundtrip_host.argtypes = [VP, P, P]
     L.mpfft_plan_last_timing.argtypes = [VP, P]
+    L.mpfft_limbs_to_hex.argtypes = [U, L_, P, P, ctypes.c_size_t,
+                                     ctypes.POINTER(ctypes.c_size_t)]
+    L.mpfft_hex_to_limbs.argtypes = [U, L_, L_, ctypes.c_char_p, P]
     L.mpfft_mp_binop_host.argtypes = [I, U, L_, P, P, P]
     L.mpfft_limbs_for_precision.argtypes = [L_]
     L.mpfft_limbs_for_precision.restype = L_
@@ -182,6 +185,26 @@ class Plan:
         out = np.zeros(8 * self.batch)
         check(lib().mpfft_roundtrip_host(self.h, _ptr(x), _ptr(out)), "roundtrip_host")
         return out.reshape(self.batch, 8)
+
+
+def to_hex(x, count, L):
+    """the reals of a SoA-limb array (count reals, L limbs) as MPFR "%Ra" text"""
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    need = ctypes.c_size_t(0)
+    lib().mpfft_limbs_to_hex(count, L, _ptr(x), None, 0, ctypes.byref(need))
+    buf = ctypes.create_string_buffer(need.value)
+    check(lib().mpfft_limbs_to_hex(count, L, _ptr(x), buf, need.value, ctypes.byref(need)),
+          "limbs_to_hex")
+    return buf.value.decode().split()
+
+
+def from_hex(strings, L, P):
+    """SoA-limb array (L limbs) of hexadecimal floats rounded to P bits"""
+    count = len(strings)
+    y = np.zeros(count * (L + 1), dtype=np.uint32)
+    check(lib().mpfft_hex_to_limbs(count, L, P, " ".join(strings).encode(), _ptr(y)),
+          "hex_to_limbs")
+    return y
 
 
 def mp_binop(op, x, y, count, P):

@@ -29,7 +29,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           f"--offload-arch={ARCH}", f"-I{MPFR_INC}", f"-I{os.path.join(ROOT, 'include')}"]
 
-SOURCES = ["mp_fft.hip", "d_fft.hip", "mpfft.cpp"]
+SOURCES = ["mp_fft.hip", "d_fft.hip", "mpfft.cpp", "hexio.cpp"]
 HEADERS = ["mp_arith.h", "kernels.h"]
 
 
