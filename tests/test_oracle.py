@@ -214,6 +214,28 @@ def test_roundtrip_rrmse():
     assert np.all(r[:2] < 2.0 ** -(P - 6)) and np.all(r[:2] > 0)
 
 
+def test_err_stats_closed_forms():
+    """the error measure itself: zero against an exact copy held at 2P bits,
+    exactly one ulp after changing the last bit of one component, and the
+    zero-mismatch count"""
+    n, P = 64, 128
+    L, L2 = O.limbs(P), O.limbs(2 * P)
+    y = O.mp_fft(n, P, I.mp_random(n, P, seed=4))
+    star = I.relimb(y, n, L, L2)                 # same values, 2P-bit container
+    assert O.err_stats(n, P, y, L, star, L2) == (0.0, 0.0, 0)
+    bad = y.copy()
+    h, lim = I.split(bad, n, L)
+    k = int(np.nonzero((h & ~1) != -2**31)[0][5])
+    lim[L - 1, k] ^= 1                           # one ulp at P = 128 = 32 L
+    mu, nw, zb = O.err_stats(n, P, bad, L, star, L2)
+    assert mu == 1.0 and 0 < nw < 1.0 and zb == 0
+    z = star.copy()
+    hz, lz = I.split(z, n, L2)
+    hz[k] = -2**31
+    lz[:, k] = 0
+    assert O.err_stats(n, P, y, L, z, L2)[2] == 1
+
+
 def test_planted_mistake_is_caught(mp_golden, digests):
     """a one-ulp change anywhere in a 1024-point output must fail the
     bit-exact comparison (the comparison is not vacuous)."""
