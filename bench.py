@@ -95,6 +95,34 @@ def measure(M, I, torch, n, P, batch, steps, warmup, seed):
                 out0=I.relimb(out0, n, plan_L, Lc), x0=I.relimb(x0, n, plan_L, Lc), L=plan_L)
 
 
+def measure_form(M, I, torch, n, P, batch, steps, warmup, seed, form):
+    """FFT/s of a limb-layout form timed as whole transforms (device
+    buffers resident), plus its first output for the error check"""
+    plan = M.Plan(n, P, form, batch)
+    Lc = I.limbs_for(P)
+    host = np.concatenate([I.relimb(I.mp_random(n, P, seed=seed + b), n, Lc, plan.L)
+                           for b in range(batch)])
+    x = torch.from_numpy(host.view(np.int32)).cuda()
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    for _ in range(warmup):
+        plan.execute_device(x.data_ptr(), y.data_ptr(), sp)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(steps):
+        plan.execute_device(x.data_ptr(), y.data_ptr(), sp)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    plan.check()
+    ms = t0.elapsed_time(t1)
+    out0 = I.relimb(y[: plan.words].cpu().numpy().view(np.uint32), n, plan.L, Lc)
+    plan.close()
+    return dict(total_ms=ms, fft_s=batch * steps / (ms * 1e-3), out0=out0)
+
+
 def measure_mpfr(M, I, n, P, seed, reps=3):
     """the drop-in path: one transform from and to mpfr_t arrays
     (marshalling, PCIe transfers and the GPU transform), per call"""
@@ -137,6 +165,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-err", action="store_true", help="skip the 2P-bit oracle error")
     ap.add_argument("--no-mpfr", action="store_true", help="skip the mpfr_t-array call timing")
+    ap.add_argument("--fixed", action="store_true", help="also measure the fixed form")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -208,6 +237,14 @@ def main():
         line["forms"][f"plain_P{a.prec2}"] = {
             "fft_per_s": round(B * max(2, a.steps // 2) / (r2["total_ms"] * 1e-3), 3)}
 
+    if a.fixed:
+        fx = measure_form(M, I, torch, n, P, B, a.steps, a.warmup, a.seed, M.FORM_FIXED)
+        line["forms"]["fixed"] = {"fft_per_s": round(fx["fft_s"], 3)}
+        fx_out0 = fx["out0"]
+        if a.prec2:
+            fx2 = measure_form(M, I, torch, n, a.prec2, B, max(2, a.steps // 2), 1, a.seed,
+                               M.FORM_FIXED)
+            line["forms"][f"fixed_P{a.prec2}"] = {"fft_per_s": round(fx2["fft_s"], 3)}
     if not a.no_mpfr:
         line["mpfr_path"] = measure_mpfr(M, I, n, P, a.seed)
 
@@ -221,6 +258,10 @@ def main():
         line["error_vs_2P_oracle"] = {"max_ulp": mu, "normwise_2^-P": nw, "zero_mismatch": zb,
                                       "bit_identical_to_P_oracle": same,
                                       "seconds": round(time.time() - t0, 2)}
+        if a.fixed:
+            mu2, nw2, zb2 = O.err_stats(n, P, fx_out0, Lc, star, I.limbs_for(2 * P))
+            line["forms"]["fixed"]["error_vs_2P_oracle"] = {"max_ulp": mu2, "normwise_2^-P": nw2,
+                                                           "zero_mismatch": zb2}
     if not a.no_cpu:
         # bounded sample: the reference's OpenMP CODE 2 (4 threads, hard-coded
         # at parallel_mp_fft2.c:262) when its build is present, and the oracle

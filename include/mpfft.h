@@ -89,6 +89,12 @@ typedef struct mpfft_plan_s *mpfft_plan;
                               butterfly for every k (serial_mp_fft1.c)      */
 #define MPFFT_FORM_MP_C2 2 /* multiprecision, CODE 2 op order: 4m2a, special
                               butterflies k = 0 and n/8 (serial_mp_fft2.c)  */
+#define MPFFT_FORM_FIXED 8 /* multiprecision "fixed" form: radix-4 Stockham on
+                              fixed-point reals with 32 (L+1) fraction bits;
+                              same SoA-limb I/O as MPFFT_FORM_MP, results at
+                              the plain form's accuracy or better instead of
+                              the reference's bit pattern; |input| < 2^(29 -
+                              log2 n), checked by mpfft_plan_check        */
 #define MPFFT_FORM_D5  5   /* double, CODE 5 op order (4m2a, 3 butterflies) */
 #define MPFFT_FORM_D6  6   /* double, CODE 6/7 op order (1 butterfly)      */
 
@@ -157,6 +163,11 @@ int mpfft_execute_f64_host(mpfft_plan plan, const double *h_in, double *h_out);
 
 /* Wait for the plan's stream. */
 int mpfft_plan_sync(mpfft_plan plan);
+
+/* Wait for the plan's stream and report input-range violations of the
+ * fixed form since the last check (MPFFT_ERR_VALUE); MPFFT_OK otherwise.
+ * The host and mpfr_t entry points check by themselves. */
+int mpfft_plan_check(mpfft_plan plan);
 
 /* ---- conversion ----------------------------------------------------------- */
 

@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(HERE, "libmpfft.so")
 
 FORM_MP, FORM_D5, FORM_D6 = 0, 5, 6
 FORM_MP_C1, FORM_MP_C2 = 1, 2          # the reference's CODE 1 / CODE 2 op orders
-MP_FORMS = (FORM_MP, FORM_MP_C1, FORM_MP_C2)
+FORM_FIXED = 8                          # fixed-point radix-4 form (same limb I/O)
+MP_FORMS = (FORM_MP, FORM_MP_C1, FORM_MP_C2, FORM_FIXED)
 OK = 0
 ERRORS = {
     -1: "MPFFT_ERR_SIZE", -2: "MPFFT_ERR_PREC", -3: "MPFFT_ERR_VALUE",
@@ -34,6 +35,7 @@ EXPORTS = [
     "mpfft_mp_binop_host", "mpfft_limbs_for_precision",
     "mpfft_execute_inverse_limbs", "mpfft_roundtrip_stats", "mpfft_roundtrip_host",
     "mpfft_plan_last_timing", "mpfft_limbs_to_hex", "mpfft_hex_to_limbs",
+    "mpfft_plan_check",
 ]
 
 _lib = None
@@ -76,6 +78,7 @@ def lib():
     L.mpfft_roundtrip_stats.argtypes = [VP, P, P, P, P]
     L.mpfft_roundtrip_host.argtypes = [VP, P, P]
     L.mpfft_plan_last_timing.argtypes = [VP, P]
+    L.mpfft_plan_check.argtypes = [VP]
     L.mpfft_limbs_to_hex.argtypes = [U, L_, P, P, ctypes.c_size_t,
                                      ctypes.POINTER(ctypes.c_size_t)]
     L.mpfft_hex_to_limbs.argtypes = [U, L_, L_, ctypes.c_char_p, P]
@@ -162,6 +165,10 @@ class Plan:
 
     def sync(self):
         check(lib().mpfft_plan_sync(self.h), "plan_sync")
+
+    def check(self):
+        """wait, then raise if a fixed-form input was out of range"""
+        check(lib().mpfft_plan_check(self.h), "plan_check")
 
     def execute_inverse(self, d_in, d_out, stream=None):
         """inverse transform by the reference's swap trick (device pointers)"""

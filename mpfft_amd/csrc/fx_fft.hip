@@ -1,0 +1,296 @@
+// The "fixed" form of the multiprecision FFT: a radix-4 Stockham
+// (self-sorting, out-of-place) transform on fixed-point multiprecision
+// reals (fx_arith.h), for callers that want the transform at the plain
+// form's accuracy or better rather than the reference's bit pattern.
+//
+// Why it is cheaper than the plain form: every radix-4 pass covers two
+// levels with one read and one write of the data (the plain form reads and
+// writes every element once per level), additions are exact W-word carry
+// chains instead of aligned, normalised and rounded floating-point sums, and
+// a product needs no normalisation or rounding test.  The cost is an
+// absolute error of about 2^-F per operation, with F = 32 bits more than the
+// floating precision.
+//
+// Data: two ping-pong buffers of W rows of 2N words per transform (real
+// parts at [0, N), imaginary at [N, 2N)); word 0 least significant.  The
+// input conversion reads the plain form's SoA-limb layout, the output
+// conversion rounds back to it at P bits.
+//
+// Pass (radix R = 4 or 2, Ns = size of the sub-transforms already done):
+//   for j < N/R, k = j mod Ns:
+//     v_r = x[j + r N/R] * w^(r k), w = exp(-2 pi i / (R Ns))
+//     y = DFT_R(v)
+//     out[(j - k) R + k + r Ns] = y_r
+// Lanes: R neighbouring lanes per j (a DPP quad); lane r loads v_r, forms
+// its twiddle product, then the quad exchanges values twice (radix 4) or
+// once (radix 2) through DPP.
+#include "fx_arith.h"
+#include "kernels.h"
+
+#include <utility>
+
+namespace mpfft {
+
+namespace {
+
+template <int W>
+__device__ __forceinline__ void fx_dpp(Fx<W> &o, const Fx<W> &x, int ctrl)
+{
+    // quad permutation of every word; ctrl is a compile-time constant at
+    // each call site (0x4E: [2,3,0,1], 0xB1: [1,0,3,2])
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        o.w[j] = ctrl == 0x4E
+                     ? (uint32_t)__builtin_amdgcn_mov_dpp((int)x.w[j], 0x4E, 0xF, 0xF, false)
+                     : (uint32_t)__builtin_amdgcn_mov_dpp((int)x.w[j], 0xB1, 0xF, 0xF, false);
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void fx_select(Fx<W> &o, uint32_t c, const Fx<W> &a, const Fx<W> &b)
+{
+#pragma unroll
+    for (int j = 0; j < W; j++) o.w[j] = c ? a.w[j] : b.w[j];
+}
+
+// z = A + (-1)^s B with (A, B) = (p, x) when sw, else (x, p)
+template <int W>
+__device__ __forceinline__ void fx_pair(Fx<W> &z, const Fx<W> &x, const Fx<W> &p, uint32_t sw,
+                                        uint32_t s)
+{
+    const uint32_t m = s ? 0xffffffffu : 0u;
+    uint32_t c = s;
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        const uint32_t a = sw ? p.w[j] : x.w[j], b = sw ? x.w[j] : p.w[j];
+        z.w[j] = add_cc(a, b ^ m, c);
+    }
+}
+
+// LDS parking slot k of this lane: word j at lds[(k W + j) nt + t]
+template <int W>
+__device__ __forceinline__ void fx_put(uint32_t *lds, uint32_t nt, uint32_t t, int k, const Fx<W> &x)
+{
+#pragma unroll
+    for (int j = 0; j < W; j++) lds[((uint32_t)k * W + j) * nt + t] = x.w[j];
+}
+
+template <int W>
+__device__ __forceinline__ void fx_get(Fx<W> &x, const uint32_t *lds, uint32_t nt, uint32_t t, int k)
+{
+#pragma unroll
+    for (int j = 0; j < W; j++) x.w[j] = lds[((uint32_t)k * W + j) * nt + t];
+}
+
+// twiddle real q of the table: DT digits then the sign word (SoA, MT reals)
+template <int W>
+__device__ __forceinline__ uint32_t tw_load(uint32_t (&T)[FxDig<W>::DT], const uint32_t *tw,
+                                            uint32_t MT, uint32_t q)
+{
+#pragma unroll
+    for (int j = 0; j < FxDig<W>::DT; j++) T[j] = (tw + (size_t)j * MT)[q];
+    return (tw + (size_t)FxDig<W>::DT * MT)[q];
+}
+
+}  // namespace
+
+// float SoA (hdr + L limbs) -> fixed point; flags inputs beyond the range
+template <int L, int W>
+__global__ void __launch_bounds__(256)
+fx_in_kernel(FxArgs a)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t M = 2 * a.N;
+    if (i >= M) return;
+    Mp<L> x;
+    mp_load(x, a.in + a.in_stride * blockIdx.y, M, i);
+    Fx<W> z;
+    if (fx_from_mp<L, W>(z, x, a.lim)) atomicOr(a.flag, 1u);
+    fx_store(z, a.buf0 + a.stride * blockIdx.y, M, i);
+}
+
+// fixed point -> float SoA at P bits, rounded to nearest
+template <int W, int L, bool FULL>
+__global__ void __launch_bounds__(256)
+fx_out_kernel(FxArgs a, const uint32_t *src)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t M = 2 * a.N;
+    if (i >= M) return;
+    Fx<W> x;
+    fx_load(x, src + a.stride * blockIdx.y, M, i);
+    Mp<L> z;
+    fx_to_mp<W, L, FULL>(z, x, a.P);
+    mp_store(z, a.out + a.out_stride * blockIdx.y, M, i);
+}
+
+// One radix-4 pass.  Blocks are ordered transform-fastest, so the blocks
+// that read one twiddle for the different transforms of a batch run side
+// by side and share it in L2.
+template <int W>
+__global__ void __launch_bounds__(256, 2)
+fx_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns)
+{
+    const uint32_t N = a.N, M = 2 * N, Q = N >> 2;
+    const uint32_t b = blockIdx.x % a.batch, jb = blockIdx.x / a.batch;
+    const uint32_t r = threadIdx.x & 3u;
+    const uint32_t j = jb * (blockDim.x >> 2) + (threadIdx.x >> 2);
+    if (j >= Q) return;                            // whole quads leave together
+    src += a.stride * b;
+    dst += a.stride * b;
+    Fx<W> re, im;
+    fx_load(re, src, M, j + r * Q);
+    fx_load(im, src, M, N + j + r * Q);
+    const uint32_t k = j & (Ns - 1);
+    if (r != 0 && k != 0) {
+        // v_r *= w^t, t = r k N / (4 Ns), by three products (Gauss):
+        //   k3 = im (c + d), k2 = re (d - c), k1 = (re + im) c,
+        //   re' = k1 - k3, im' = k1 + k2   for w^t = c + i d.
+        // re and im wait in LDS slots 0 and 1 and are replaced there by k2
+        // and k3, so only s = re + im stays in registers across a product.
+        extern __shared__ uint32_t lds[];
+        const uint32_t lt = threadIdx.x, nt = blockDim.x;
+        const uint32_t t = r * k * (N / (4 * Ns));
+        const uint32_t T3 = 3 * Q, MT = 3 * T3;
+        Fx<W> s;
+        fx_add(s, re, im);
+        fx_put(lds, nt, lt, 0, re);
+        fx_put(lds, nt, lt, 1, im);
+        uint32_t T[FxDig<W>::DT];
+#pragma unroll 1
+        for (int it = 0; it < 3; it++) {
+            // table rows: c at [0, T3), d - c at [T3, 2 T3), c + d at [2 T3, 3 T3)
+            const uint32_t u = it == 0 ? 2u : (it == 1 ? 1u : 0u);
+            const uint32_t tneg = tw_load<W>(T, a.tw, MT, u * T3 + t);
+            if (it < 2) fx_get(re, lds, nt, lt, 1 - it);    // im, then re
+            else re = s;
+            fx_mul(re, re, T, tneg);
+            if (it < 2) fx_put(lds, nt, lt, 1 - it, re);     // k3 -> 1, k2 -> 0
+        }
+        // re = k1 here
+        fx_get(im, lds, nt, lt, 1);                          // k3
+        fx_sub(s, re, im);
+        fx_get(im, lds, nt, lt, 0);                          // k2
+        fx_add(im, re, im);
+        re = s;
+    }
+    // DFT4 over the quad.  Stage 1 (partner r ^ 2): lanes 0, 1 form
+    // t0 = x0 + x2, t2 = x1 + x3; lanes 2, 3 form t1 = x0 - x2, t3 = x1 - x3.
+    {
+        const uint32_t hi = r >> 1;
+        Fx<W> p;
+        fx_dpp(p, re, 0x4E);
+        fx_pair(re, re, p, hi, hi);
+        fx_dpp(p, im, 0x4E);
+        fx_pair(im, im, p, hi, hi);
+    }
+    // Stage 2 (partner r ^ 1): lane 0: y0 = t0 + t2, lane 1: y2 = t0 - t2,
+    // lane 2: y1 = t1 - i t3, lane 3: y3 = t1 + i t3.
+    //   lanes 0, 1: re = own.re +- p.re, im likewise (lane 1: p - own)
+    //   lane 2 (own t1, p t3): re = own.re + p.im, im = own.im - p.re
+    //   lane 3 (own t3, p t1): re = p.re - own.im, im = p.im + own.re
+    {
+        const uint32_t odd = r & 1u, cross = r >> 1;
+        Fx<W> pre, pim, nre;
+        fx_dpp(pre, re, 0xB1);
+        fx_dpp(pim, im, 0xB1);
+        // real part: lanes 0/1 pair re with pre; lane 2 re with pim; lane 3 pre with im
+        {
+            Fx<W> x, y;
+            fx_select(x, cross, odd ? pre : re, re);
+            fx_select(y, cross, odd ? im : pim, pre);
+            // lanes 0, 2: x + y; lanes 1, 3: x - y with (x, y) ordered above
+            fx_pair(nre, x, y, cross ? 0u : odd, odd);
+        }
+        // imaginary part: lanes 0/1 pair im with pim; lane 2 im - pre; lane 3 pim + re
+        {
+            Fx<W> x, y;
+            fx_select(x, cross, odd ? pim : im, im);
+            fx_select(y, cross, odd ? re : pre, pim);
+            fx_pair(im, x, y, cross ? 0u : odd, cross ? (odd ^ 1u) : odd);
+        }
+        re = nre;
+    }
+    const uint32_t rout = (r == 1) ? 2u : (r == 2 ? 1u : r);
+    const uint32_t o = (j - k) * 4 + k + rout * Ns;
+    fx_store(re, dst, M, o);
+    fx_store(im, dst, M, N + o);
+}
+
+// The radix-2 pass (first pass when log2 N is odd: Ns = 1, no twiddles).
+template <int W>
+__global__ void __launch_bounds__(256, 2)
+fx_r2_kernel(FxArgs a, const uint32_t *src, uint32_t *dst)
+{
+    const uint32_t N = a.N, M = 2 * N, H = N >> 1;
+    const uint32_t b = blockIdx.x % a.batch, jb = blockIdx.x / a.batch;
+    const uint32_t r = threadIdx.x & 1u;
+    const uint32_t j = jb * (blockDim.x >> 1) + (threadIdx.x >> 1);
+    if (j >= H) return;
+    src += a.stride * b;
+    dst += a.stride * b;
+    Fx<W> re, im, pre, pim, x, y;
+    fx_load(re, src, M, j + r * H);
+    fx_load(im, src, M, N + j + r * H);
+    fx_dpp(pre, re, 0xB1);
+    fx_dpp(pim, im, 0xB1);
+    // lane 0: x0 + x1; lane 1: x0 - x1 (partner - own)
+    fx_select(x, r, pre, re);
+    fx_select(y, r, re, pre);
+    fx_addsub(re, x, y, r);
+    fx_select(x, r, pim, im);
+    fx_select(y, r, im, pim);
+    fx_addsub(im, x, y, r);
+    fx_store(re, dst, M, 2 * j + r);
+    fx_store(im, dst, M, N + 2 * j + r);
+}
+
+// ---------------------------------------------------------------------------
+
+template <int L, int W, bool FULL>
+static hipError_t launch_fx(const FxArgs &a, hipStream_t st)
+{
+    const uint32_t N = a.N, M = 2 * N;
+    dim3 gc((M + 255) / 256, a.batch);
+    hipLaunchKernelGGL((fx_in_kernel<L, W>), gc, dim3(256), 0, st, a);
+    uint32_t *src = a.buf0, *dst = a.buf1;
+    uint32_t lg = 0;
+    while ((1u << lg) < N) lg++;
+    uint32_t Ns = 1;
+    if (lg & 1) {
+        dim3 g(((N / 2) * 2 + 255) / 256 * a.batch);
+        hipLaunchKernelGGL((fx_r2_kernel<W>), g, dim3(256), 0, st, a, (const uint32_t *)src, dst);
+        std::swap(src, dst);
+        Ns = 2;
+    }
+    for (; Ns < N; Ns *= 4) {
+        dim3 g(((N / 4) * 4 + 255) / 256 * a.batch);
+        const size_t sh = sizeof(uint32_t) * 2 * W * 256;     // two parking slots per lane
+        hipLaunchKernelGGL((fx_r4_kernel<W>), g, dim3(256), sh, st, a, (const uint32_t *)src, dst, Ns);
+        std::swap(src, dst);
+    }
+    hipLaunchKernelGGL((fx_out_kernel<W, L, FULL>), gc, dim3(256), 0, st, a, (const uint32_t *)src);
+    return hipGetLastError();
+}
+
+hipError_t fx_fft_launch(int L, int P, const FxArgs &a, hipStream_t st)
+{
+    const bool full = (P == 32 * L);
+#define CALLX(Lv) (full ? launch_fx<Lv, Lv + 2, true>(a, st) : launch_fx<Lv, Lv + 2, false>(a, st))
+    switch (L) {
+    case 1: return CALLX(1);
+    case 2: return CALLX(2);
+    case 3: return CALLX(3);
+    case 4: return CALLX(4);
+    case 6: return CALLX(6);
+    case 8: return CALLX(8);
+    case 12: return CALLX(12);
+    case 16: return CALLX(16);
+    case 24: return CALLX(24);
+    case 32: return CALLX(32);
+    default: return hipErrorInvalidValue;
+    }
+#undef CALLX
+}
+
+}  // namespace mpfft

@@ -58,6 +58,28 @@ hipError_t mp_err_launch(int L, int P, const uint32_t *x, const uint32_t *y, uin
 hipError_t mp_binop_launch(int L, int P, int op, uint32_t count, const uint32_t *x,
                            const uint32_t *y, uint32_t *z, hipStream_t st);
 
+// ---- fixed form (fx_fft.hip) ----
+
+struct FxArgs {
+    const uint32_t *in;         // float SoA input of transform 0 (hdr + L limbs)
+    uint64_t in_stride;
+    uint32_t *out;              // float SoA output of transform 0
+    uint64_t out_stride;
+    uint32_t *buf0, *buf1;      // fixed-point ping-pong buffers (W rows of 2N words)
+    uint64_t stride;            // words per transform in buf0 / buf1
+    const uint32_t *tw;         // twiddles w^t, t < 3N/4: rows c, d - c, c + d (w^t =
+                                // c + i d), each DT digit rows + a sign row
+    uint32_t *flag;             // set to nonzero when an input exceeds the range
+    uint32_t N, batch;
+    int lim;                    // inputs must have exponent <= lim (|x| < 2^lim)
+    int P;
+};
+
+// fixed-point words per real for limb count L (F = 32 (L + 1) fraction bits)
+constexpr int fx_words(int L) { return L + 2; }
+
+hipError_t fx_fft_launch(int L, int P, const FxArgs &a, hipStream_t st);
+
 // ---- double path (d_fft.hip) ----
 
 struct DTables {

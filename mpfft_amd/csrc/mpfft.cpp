@@ -45,6 +45,9 @@ bool is_mp(int form)
     return form == MPFFT_FORM_MP || form == MPFFT_FORM_MP_C1 || form == MPFFT_FORM_MP_C2;
 }
 
+// forms whose device data use the SoA-limb layout of include/mpfft.h
+bool is_limb_form(int form) { return is_mp(form) || form == MPFFT_FORM_FIXED; }
+
 // the reference's operation order of a multiprecision form: CODE 3
 // (serial_mp_fft3.c), CODE 1 (serial_mp_fft1.c) or CODE 2 (serial_mp_fft2.c)
 int mp_code(int form)
@@ -225,6 +228,9 @@ struct mpfft_plan_s {
     DevBuf lut, pos_sn, pos_nr, nodes, aos;
     LeafArgs leaf{};
     std::vector<LevelArgs> levels;
+    // fixed form
+    DevBuf fx_tw, fx_buf0, fx_buf1, fx_flag;
+    int fx_lim = 0;
     // double forms
     DevBuf tw, dleaf_sn, dleaf_off, dleaf_n, dlvl_off, dlvl_begin;
     DTables dt{};
@@ -293,6 +299,101 @@ int build_mp_plan(mpfft_plan_s *p)
         a.N = (uint32_t)N; a.n = n; a.cnt = cnt;
         a.P = (int)p->P;
         p->levels.push_back(a);
+    }
+    return MPFFT_OK;
+}
+
+// Fixed form: twiddles w^t = c + i d = exp(-2 pi i t / N), t < 3N/4, as
+// c, d - c and c + d rounded to F = 32 (L + 1) fraction bits, each stored as
+// DT 29-bit digits of its magnitude (least significant first) and a sign
+// word; SoA over the 3 * (3N/4) reals.
+int build_fx_plan(mpfft_plan_s *p)
+{
+    const unsigned long N = p->N;
+    const long W = p->L + 2, F = 32 * (W - 1);
+    const long DT = (F + 1 + 28) / 29;
+    const size_t T3 = 3 * (N / 4), MT = 3 * T3;
+    std::vector<uint32_t> tab(MT > 0 ? MT * (DT + 1) : 1, 0);
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = std::max(1u, std::min(nt, 16u));
+    if (T3 < 256) nt = 1;
+    auto work = [&](size_t t0, size_t t1) {
+        mpfr_t th, c, d, v;
+        mpfr_inits2(F + 64, th, c, d, v, (mpfr_ptr)0);
+        mpz_t z;
+        mpz_init(z);
+        std::vector<uint32_t> words((size_t)(F / 32 + 4));
+        for (size_t t = t0; t < t1; t++) {
+            mpfr_const_pi(th, MPFR_RNDN);
+            mpfr_mul_ui(th, th, 2 * t, MPFR_RNDN);
+            mpfr_div_ui(th, th, N, MPFR_RNDN);
+            mpfr_sin_cos(d, c, th, MPFR_RNDN);
+            mpfr_neg(d, d, MPFR_RNDN);                    // w = cos - i sin
+            for (int u = 0; u < 3; u++) {
+                if (u == 0) mpfr_set(v, c, MPFR_RNDN);
+                else if (u == 1) mpfr_sub(v, d, c, MPFR_RNDN);
+                else mpfr_add(v, c, d, MPFR_RNDN);
+                mpfr_mul_2si(v, v, F, MPFR_RNDN);
+                mpfr_get_z(z, v, MPFR_RNDN);
+                const size_t q = u * T3 + t;
+                tab[(size_t)DT * MT + q] = mpz_sgn(z) < 0 ? 1u : 0u;
+                mpz_abs(z, z);
+                std::fill(words.begin(), words.end(), 0u);
+                size_t cnt = 0;
+                mpz_export(words.data(), &cnt, -1, sizeof(uint32_t), 0, 0, z);
+                for (long k = 0; k < DT; k++) {
+                    const long bit = 29 * k, wi = bit >> 5, sh = bit & 31;
+                    const uint64_t lo = words[wi], hi = words[wi + 1];
+                    tab[(size_t)k * MT + q] = (uint32_t)(((hi << 32 | lo) >> sh) & 0x1fffffffu);
+                }
+            }
+        }
+        mpz_clear(z);
+        mpfr_clears(th, c, d, v, (mpfr_ptr)0);
+        mpfr_free_cache();
+    };
+    std::vector<std::thread> th;
+    for (unsigned i = 0; i < nt; i++) th.emplace_back(work, T3 * i / nt, T3 * (i + 1) / nt);
+    for (auto &x : th) x.join();
+    int rc;
+    if ((rc = p->fx_tw.upload(tab))) return rc;
+    const size_t fxw = 2 * N * (size_t)W * p->batch;
+    if ((rc = p->fx_buf0.alloc(sizeof(uint32_t) * fxw)) ||
+        (rc = p->fx_buf1.alloc(sizeof(uint32_t) * fxw)) ||
+        (rc = p->fx_flag.upload(std::vector<uint32_t>{0u})))
+        return rc;
+    int lg = 0;
+    while ((1UL << lg) < N) lg++;
+    p->fx_lim = 29 - lg;
+    return MPFFT_OK;
+}
+
+int run_fx(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t st)
+{
+    FxArgs a{};
+    a.in = d_in; a.in_stride = p->words;
+    a.out = d_out; a.out_stride = p->words;
+    a.buf0 = p->fx_buf0.as<uint32_t>(); a.buf1 = p->fx_buf1.as<uint32_t>();
+    a.stride = 2 * p->N * (uint64_t)(p->L + 2);
+    a.tw = p->fx_tw.as<uint32_t>();
+    a.flag = p->fx_flag.as<uint32_t>();
+    a.N = (uint32_t)p->N; a.batch = p->batch;
+    a.lim = p->fx_lim; a.P = (int)p->P;
+    hipError_t e = fx_fft_launch((int)p->L, (int)p->P, a, st);
+    return e == hipSuccess ? MPFFT_OK : MPFFT_ERR_HIP;
+}
+
+// read and clear the fixed form's range flag (waits for the plan's stream)
+int fx_check(mpfft_plan_s *p)
+{
+    if (p->form != MPFFT_FORM_FIXED) return MPFFT_OK;
+    uint32_t f = 0;
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    HIP_TRY(hipMemcpy(&f, p->fx_flag.p, sizeof f, hipMemcpyDeviceToHost));
+    if (f) {
+        const uint32_t zero = 0;
+        HIP_TRY(hipMemcpy(p->fx_flag.p, &zero, sizeof zero, hipMemcpyHostToDevice));
+        return MPFFT_ERR_VALUE;
     }
     return MPFFT_OK;
 }
@@ -372,6 +473,12 @@ int run_mp(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t s
     return e == hipSuccess ? MPFFT_OK : MPFFT_ERR_HIP;
 }
 
+// one whole transform of a limb-layout form
+int run_limbs(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t st)
+{
+    return p->form == MPFFT_FORM_FIXED ? run_fx(p, d_in, d_out, st) : run_mp(p, d_in, d_out, st);
+}
+
 // Combine the per-block partials of mp_err_kernel into, per transform,
 // {rRMSE re, rRMSE im, max|aE| re, max|aE| im} and their log2.
 void combine_err(const double *part, uint32_t nblk, uint32_t batch, double *out)
@@ -417,7 +524,7 @@ void combine_err(const double *part, uint32_t nblk, uint32_t batch, double *out)
 
 int ensure_stage(mpfft_plan_s *p)
 {
-    const size_t bytes = (is_mp(p->form) ? sizeof(uint32_t) : sizeof(double)) *
+    const size_t bytes = (is_limb_form(p->form) ? sizeof(uint32_t) : sizeof(double)) *
                          p->words * p->batch;
     if (p->d_in.bytes != bytes) {
         int rc;
@@ -509,7 +616,7 @@ int mpfft_plan_create(mpfft_plan *plan, unsigned long n, long prec, int form, un
     if (!plan || batch == 0) return MPFFT_ERR_ARG;
     *plan = nullptr;
     if (!is_pow2(n) || n > kMaxN) return MPFFT_ERR_SIZE;
-    if (!is_mp(form) && form != MPFFT_FORM_D5 && form != MPFFT_FORM_D6)
+    if (!is_limb_form(form) && form != MPFFT_FORM_D5 && form != MPFFT_FORM_D6)
         return MPFFT_ERR_FORM;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MPFFT_ERR_NODEV;
@@ -518,12 +625,12 @@ int mpfft_plan_create(mpfft_plan *plan, unsigned long n, long prec, int form, un
     HIP_TRY(hipGetDevice(&p->device));
     HIP_TRY(hipStreamCreate(&p->stream));
     int rc;
-    if (is_mp(form)) {
+    if (is_limb_form(form)) {
         if (prec < 2 || prec > kMaxPrec) return MPFFT_ERR_PREC;
         p->P = prec;
         p->L = limbs_for_precision(prec);
         p->words = 2 * n * (size_t)(p->L + 1);
-        rc = build_mp_plan(p.get());
+        rc = form == MPFFT_FORM_FIXED ? build_fx_plan(p.get()) : build_mp_plan(p.get());
     } else {
         p->P = 53;
         p->words = 2 * n;
@@ -559,9 +666,17 @@ int mpfft_plan_sync(mpfft_plan p)
 int mpfft_execute_limbs(mpfft_plan p, const uint32_t *d_in, uint32_t *d_out, void *stream)
 {
     if (!p || !d_in || !d_out) return MPFFT_ERR_ARG;
-    if (!is_mp(p->form)) return MPFFT_ERR_FORM;
+    if (!is_limb_form(p->form)) return MPFFT_ERR_FORM;
     HIP_TRY(hipSetDevice(p->device));
-    return run_mp(p, d_in, d_out, stream ? (hipStream_t)stream : p->stream);
+    return run_limbs(p, d_in, d_out, stream ? (hipStream_t)stream : p->stream);
+}
+
+int mpfft_plan_check(mpfft_plan p)
+{
+    if (!p) return MPFFT_ERR_ARG;
+    HIP_TRY(hipSetDevice(p->device));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return fx_check(p);
 }
 
 int mpfft_execute_inverse_limbs(mpfft_plan p, const uint32_t *d_in, uint32_t *d_out,
@@ -637,17 +752,18 @@ int mpfft_plan_stages(mpfft_plan p)
 int mpfft_execute_limbs_host(mpfft_plan p, const uint32_t *h_in, uint32_t *h_out)
 {
     if (!p || !h_in || !h_out) return MPFFT_ERR_ARG;
-    if (!is_mp(p->form)) return MPFFT_ERR_FORM;
+    if (!is_limb_form(p->form)) return MPFFT_ERR_FORM;
     std::lock_guard<std::mutex> g(p->mu);
     HIP_TRY(hipSetDevice(p->device));
     int rc = ensure_stage(p);
     if (rc) return rc;
     const size_t bytes = sizeof(uint32_t) * p->words * p->batch;
     HIP_TRY(hipMemcpyAsync(p->d_in.p, h_in, bytes, hipMemcpyHostToDevice, p->stream));
-    if ((rc = run_mp(p, p->d_in.as<uint32_t>(), p->d_out.as<uint32_t>(), p->stream))) return rc;
+    if ((rc = run_limbs(p, p->d_in.as<uint32_t>(), p->d_out.as<uint32_t>(), p->stream)))
+        return rc;
     HIP_TRY(hipMemcpyAsync(h_out, p->d_out.p, bytes, hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
-    return MPFFT_OK;
+    return fx_check(p);
 }
 
 int mpfft_mpfr_to_limbs(unsigned long n, long limbs, mpfr_t *re, mpfr_t *im, uint32_t *h_out)
@@ -678,7 +794,7 @@ int mpfft_limbs_to_mpfr(unsigned long n, long limbs, const uint32_t *h_in, mpfr_
 int mpfft_execute_mpfr(mpfft_plan p, mpfr_t *inr, mpfr_t *ini, mpfr_t *outr, mpfr_t *outi)
 {
     if (!p || !inr || !ini || !outr || !outi) return MPFFT_ERR_ARG;
-    if (!is_mp(p->form)) return MPFFT_ERR_FORM;
+    if (!is_limb_form(p->form)) return MPFFT_ERR_FORM;
     if (p->batch != 1) return MPFFT_ERR_ARG;
     const unsigned long n = p->N;
     for (unsigned long k = 0; k < n; k++) {
@@ -748,7 +864,7 @@ int mpfft_execute_mpfr(mpfft_plan p, mpfr_t *inr, mpfr_t *ini, mpfr_t *outr, mpf
     HIP_TRY(hipEventCreate(&c0));
     HIP_TRY(hipEventCreate(&c1));
     HIP_TRY(hipEventRecord(c0, p->stream));
-    rc = run_mp(p, di, dout, p->stream);
+    rc = run_limbs(p, di, dout, p->stream);
     HIP_TRY(hipEventRecord(c1, p->stream));
     if (rc) { (void)hipEventDestroy(c0); (void)hipEventDestroy(c1); return rc; }
     for (unsigned c = 0; c < C; c++) {
@@ -784,7 +900,8 @@ int mpfft_execute_mpfr(mpfft_plan p, mpfr_t *inr, mpfr_t *ini, mpfr_t *outr, mpf
     p->timing[1] = ms;
     p->timing[2] = ttot - tin - ms;
     p->timing[3] = ttot;
-    return err.load();
+    if (err.load()) return err.load();
+    return fx_check(p);
 }
 
 int mpfft_plan_last_timing(mpfft_plan p, double *ms)

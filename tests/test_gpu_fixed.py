@@ -1,0 +1,77 @@
+"""GPU check of the fixed form (MPFFT_FORM_FIXED: radix-4 Stockham on
+fixed-point reals), through the C ABI.
+
+This form does not reproduce the reference's bit pattern; its contract is
+accuracy: against the 2P-bit oracle its error must be no worse than the
+plain form's (which is the reference's, bit for bit), both componentwise
+(max ulp) and normwise.  A wrong index or twiddle anywhere would show as an
+O(1) error.
+"""
+import numpy as np
+import pytest
+
+import mpfft_amd as M
+from mpfft_amd import inputs as I
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def run_fixed(n, P, x, batch=1):
+    Lo = O.limbs(P)
+    plan = M.Plan(n, P, M.FORM_FIXED, batch)
+    xs = np.concatenate([I.relimb(t, n, Lo, plan.L) for t in np.split(x, batch)])
+    y = plan.execute_host(xs)
+    plan.close()
+    return np.concatenate([I.relimb(t, n, plan.L, Lo) for t in np.split(y, batch)])
+
+
+def errors(n, P, y, star):
+    return O.err_stats(n, P, y, O.limbs(P), star, O.limbs(2 * P))
+
+
+@pytest.mark.parametrize("n,P", [(1, 128), (2, 128), (4, 256), (8, 256), (16, 1024), (32, 64),
+                                 (64, 1024), (128, 100), (1024, 256), (2048, 1024),
+                                 (4096, 53), (8192, 512)])
+def test_fixed_error_no_worse_than_plain(n, P):
+    x = I.mp_random(n, P, seed=1000 + n + P)
+    star = O.mp_fft(n, P, x, Pw=2 * P)
+    plain = O.mp_fft(n, P, x)                    # = the GPU plain form, bit for bit
+    fixed = run_fixed(n, P, x)
+    mu_f, nw_f, zb_f = errors(n, P, fixed, star)
+    mu_p, nw_p, zb_p = errors(n, P, plain, star)
+    assert zb_f == 0
+    assert nw_f <= max(nw_p, 0.5), (nw_f, nw_p)
+    assert mu_f <= max(mu_p, 1.0), (mu_f, mu_p)
+
+
+def test_fixed_full_size_1024_bits():
+    n, P = 1 << 16, 1024
+    x = I.mp_random(n, P, seed=77)
+    star = O.mp_fft(n, P, x, Pw=2 * P)
+    plain = O.mp_fft(n, P, x)
+    fixed = run_fixed(n, P, x)
+    mu_f, nw_f, _ = errors(n, P, fixed, star)
+    mu_p, nw_p, _ = errors(n, P, plain, star)
+    assert nw_f <= nw_p and mu_f <= mu_p, (mu_f, nw_f, mu_p, nw_p)
+
+
+def test_fixed_batch():
+    n, P, b = 512, 256, 3
+    xs = [I.mp_random(n, P, seed=300 + i) for i in range(b)]
+    got = np.split(run_fixed(n, P, np.concatenate(xs), batch=b), b)
+    for i in range(b):
+        assert np.array_equal(got[i], run_fixed(n, P, xs[i]))
+
+
+def test_fixed_input_range_checked():
+    n, P = 1024, 256
+    x = I.mp_random(n, P, seed=5)
+    h, _ = I.split(x, n, O.limbs(P))
+    h[17] = np.int32((40 << 1) | (h[17] & 1))    # |x| ~ 2^40, beyond 2^(29 - 10)
+    with pytest.raises(M.MpfftError) as e:
+        run_fixed(n, P, x)
+    assert e.value.code == -3
+    # the plan recovers: a good call right after succeeds
+    y = run_fixed(n, P, I.mp_random(n, P, seed=6))
+    assert y.size == x.size
