@@ -1,12 +1,15 @@
 #!/usr/bin/env python3
-"""Benchmark of the multiprecision split-radix conjugate-pair FFT on one
-MI355X: mp-FFTs/s at N = 2^16 and 1024-bit precision (plain GPU form,
-bit-identical to the reference's MPFR path), with the 256-bit point, the
-kernel roofline, the error against a 2P-bit oracle and the CPU baseline.
+"""Benchmark of the multiprecision split-radix FFT on one MI355X:
+mp-FFTs/s at N = 2^16, 1024 and 256 bits, for the plain form (bit-identical
+to the reference's MPFR path) and the fixed form (radix-4 Stockham on
+fixed-point reals); the "chosen" form is the fastest one whose error
+against a 2P-bit oracle is no worse than the plain form's, componentwise
+(max ulp) and normwise.  Also: the dominant kernel's roofline, the CPU
+baseline and the drop-in call from mpfr_t arrays.
 
     python bench.py [--steps K] [--warmup W] [--n N] [--prec P] [--batch B]
 
-A step is one pass of the FFT over one batch of B synthetic transforms
+A step is one pass of a form over one batch of B synthetic transforms
 whose inputs are already resident in HBM.  Prints ONE JSON line.
 """
 import argparse
@@ -27,28 +30,37 @@ HBM_PEAK_GBS = 8000.0
 # addend) issues at that full rate on gfx950 (mpfft_amd/valu_probe).
 VALU_LANE_OPS = 157.3e12 / 4
 
+PROFILES = {
+    "plain": ("mp_level_kernel_N65536_P1024.txt", "mpfft::mp_level_kernel"),
+    "fixed": ("fx_r4_kernel_N65536_P1024.txt", "mpfft::fx_r4_kernel"),
+}
+
 
 def cmul(n):
     lg = n.bit_length() - 1
     return n * lg - 3 * n + 4 if n >= 8 else 0
 
 
-def cadd(n):
+def fixed_products(n):
+    """real twiddle products of the fixed form per transform: a radix-4
+    pass with sub-transform size Ns multiplies lanes r = 1..3 of every j
+    with j mod Ns != 0 by a twiddle, three real products each"""
     lg = n.bit_length() - 1
-    return 3 * n * lg - 3 * n + 4 if n >= 8 else {1: 0, 2: 4, 4: 16}[n]
+    ns, tot = (2 if lg & 1 else 1), 0
+    while ns < n:
+        tot += 9 * (n // 4) * (ns - 1) // ns
+        ns *= 4
+    return tot
 
 
-PROFILE = os.path.join(ROOT, "profiles", "mp_level_kernel_N65536_P1024.txt")
-
-
-def profiled_traffic(n, P, batch):
-    """HBM bytes per level-kernel launch (FETCH_SIZE + WRITE_SIZE, scaled by
-    the copy-kernel calibration) from the committed rocprofv3 summary, when
-    it covers this workload; None otherwise."""
+def profiled_traffic(form, n, P, batch):
+    """HBM bytes per launch of the form's dominant kernel (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration) from the committed
+    rocprofv3 summary, when it covers this workload; None otherwise."""
     try:
-        for line in open(PROFILE):
+        for line in open(os.path.join(ROOT, "profiles", PROFILES[form][0])):
             f = line.split()
-            if f and f[0] == "level_kernel_traffic_bytes_per_launch":
+            if f and f[0] == "dominant_kernel_traffic_bytes_per_launch":
                 if f[2:] == [f"N={n}", f"P={P}", f"batch={batch}"]:
                     return int(f[1])
     except OSError:
@@ -56,48 +68,11 @@ def profiled_traffic(n, P, batch):
     return None
 
 
-def measure(M, I, torch, n, P, batch, steps, warmup, seed):
-    plan = M.Plan(n, P, M.FORM_MP, batch)
-    Lc = I.limbs_for(P)
-    host = np.concatenate([I.relimb(I.mp_random(n, P, seed=seed + b), n, Lc, plan.L)
-                           for b in range(batch)])
-    x = torch.from_numpy(host.view(np.int32)).cuda()
-    y = torch.empty_like(x)
-    stream = torch.cuda.current_stream()
-    sp = stream.cuda_stream
-    S = plan.stages()
-    for _ in range(warmup):
-        plan.execute_range(x.data_ptr(), y.data_ptr(), 0, S, sp)
-    torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    w0 = time.perf_counter()
-    t_start.record(stream)
-    for i in range(steps):
-        ev[i][0].record(stream)
-        plan.execute_range(x.data_ptr(), y.data_ptr(), 0, 1, sp)     # leaf pass
-        ev[i][1].record(stream)
-        plan.execute_range(x.data_ptr(), y.data_ptr(), 1, S, sp)     # combine levels
-        ev[i][2].record(stream)
-    t_end.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - w0
-    total_ms = t_start.elapsed_time(t_end)
-    leaf_ms = sum(e[0].elapsed_time(e[1]) for e in ev)
-    lvl_ms = sum(e[1].elapsed_time(e[2]) for e in ev)
-    out0 = y[: plan.words].cpu().numpy().view(np.uint32)
-    x0 = host[: plan.words]
-    plan_L = plan.L
-    plan.close()
-    return dict(total_ms=total_ms, leaf_ms=leaf_ms, lvl_ms=lvl_ms, wall_s=wall, S=S,
-                out0=I.relimb(out0, n, plan_L, Lc), x0=I.relimb(x0, n, plan_L, Lc), L=plan_L)
-
-
-def measure_form(M, I, torch, n, P, batch, steps, warmup, seed, form):
-    """FFT/s of a limb-layout form timed as whole transforms (device
-    buffers resident), plus its first output for the error check"""
+def measure(M, I, torch, n, P, batch, steps, warmup, seed, form):
+    """Time `steps` transforms of `batch` on device-resident data, with HIP
+    events (on the launch stream) around the parts of every step: plain
+    form: leaf pass | combine levels; fixed form: input conversion | radix
+    passes | output conversion."""
     plan = M.Plan(n, P, form, batch)
     Lc = I.limbs_for(P)
     host = np.concatenate([I.relimb(I.mp_random(n, P, seed=seed + b), n, Lc, plan.L)
@@ -106,21 +81,30 @@ def measure_form(M, I, torch, n, P, batch, steps, warmup, seed, form):
     y = torch.empty_like(x)
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
+    S = plan.stages()
+    cuts = [0, 1, S] if form != M.FORM_FIXED else [0, 1, S - 1, S]
     for _ in range(warmup):
-        plan.execute_device(x.data_ptr(), y.data_ptr(), sp)
+        plan.execute_range(x.data_ptr(), y.data_ptr(), 0, S, sp)
     torch.cuda.synchronize()
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
-    t0.record(stream)
-    for _ in range(steps):
-        plan.execute_device(x.data_ptr(), y.data_ptr(), sp)
-    t1.record(stream)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in cuts] for _ in range(steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    t_start.record(stream)
+    for i in range(steps):
+        ev[i][0].record(stream)
+        for c in range(len(cuts) - 1):
+            plan.execute_range(x.data_ptr(), y.data_ptr(), cuts[c], cuts[c + 1], sp)
+            ev[i][c + 1].record(stream)
+    t_end.record(stream)
     torch.cuda.synchronize()
     plan.check()
-    ms = t0.elapsed_time(t1)
+    total_ms = t_start.elapsed_time(t_end)
+    parts = [sum(e[c].elapsed_time(e[c + 1]) for e in ev) / steps for c in range(len(cuts) - 1)]
     out0 = I.relimb(y[: plan.words].cpu().numpy().view(np.uint32), n, plan.L, Lc)
+    x0 = I.relimb(host[: plan.words], n, plan.L, Lc)
     plan.close()
-    return dict(total_ms=ms, fft_s=batch * steps / (ms * 1e-3), out0=out0)
+    return dict(total_ms=total_ms, fft_s=batch * steps / (total_ms * 1e-3), parts_ms=parts,
+                S=S, out0=out0, x0=x0)
 
 
 def measure_mpfr(M, I, n, P, seed, reps=3):
@@ -162,10 +146,10 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--prec2", type=int, default=256, help="second precision (0 = skip)")
     ap.add_argument("--seed", type=int, default=0x6D7046465420)
+    ap.add_argument("--forms", default="plain,fixed", help="forms to measure")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-err", action="store_true", help="skip the 2P-bit oracle error")
     ap.add_argument("--no-mpfr", action="store_true", help="skip the mpfr_t-array call timing")
-    ap.add_argument("--fixed", action="store_true", help="also measure the fixed form")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -180,24 +164,72 @@ def main():
     from mpfft_amd import inputs as I
 
     n, P, B = a.n, a.prec, a.batch
+    Lc = I.limbs_for(P)
+    form_ids = {"plain": M.FORM_MP, "fixed": M.FORM_FIXED}
+    forms = [f for f in a.forms.split(",") if f]
+    if "plain" not in forms:
+        forms.insert(0, "plain")
     if a.calib_copy_mib:
         src = torch.empty(a.calib_copy_mib << 18, dtype=torch.int32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    r = measure(M, I, torch, n, P, B, a.steps, a.warmup, a.seed)
-    ms_step = r["total_ms"] / a.steps
-    fft_s = B * a.steps / (r["total_ms"] * 1e-3)
 
-    # roofline of the dominant kernel (the combine-level kernel): algorithmic
-    # schoolbook 32x32->64 limb products = cmul(N) * L^2 per transform
-    Lc = I.limbs_for(P)
-    nlev = r["S"] - 1
-    prod_per_fft = cmul(n) * Lc * Lc
-    lvl_s_per_step = r["lvl_ms"] * 1e-3 / a.steps
-    achieved = prod_per_fft * B / lvl_s_per_step / 1e12          # T products / s
-    # peak: one v_mad_u64_u32 per 32x32->64 product at the full VALU rate
+    res = {f: measure(M, I, torch, n, P, B, a.steps, a.warmup, a.seed, form_ids[f])
+           for f in forms}
+    out = {}
+    for f in forms:
+        r = res[f]
+        out[f] = {"fft_per_s": round(r["fft_s"], 3)}
+        if f == "plain":
+            out[f]["kernel_ms_per_step"] = {"leaf": round(r["parts_ms"][0], 4),
+                                            "levels": round(r["parts_ms"][1], 4),
+                                            "level_launches": r["S"] - 1}
+        else:
+            out[f]["kernel_ms_per_step"] = {"convert_in": round(r["parts_ms"][0], 4),
+                                            "passes": round(r["parts_ms"][1], 4),
+                                            "convert_out": round(r["parts_ms"][2], 4),
+                                            "pass_launches": r["S"] - 2}
+    if a.prec2:
+        for f in forms:
+            r2 = measure(M, I, torch, n, a.prec2, B, max(2, a.steps // 2), 1, a.seed, form_ids[f])
+            out[f][f"fft_per_s_P{a.prec2}"] = round(r2["fft_s"], 3)
+
+    # accuracy of every form against the 2P-bit oracle on transform 0; the
+    # chosen form is the fastest one no worse than the plain form
+    from oracle import oracle as O
+    chosen = "plain"
+    if not a.no_err:
+        t0 = time.time()
+        x0 = res["plain"]["x0"]
+        star = O.mp_fft(n, P, x0, Pw=2 * P)
+        for f in forms:
+            mu, nw, zb = O.err_stats(n, P, res[f]["out0"], Lc, star, I.limbs_for(2 * P))
+            out[f]["error_vs_2P_oracle"] = {"max_ulp": mu, "normwise_2^-P": nw,
+                                            "zero_mismatch": zb}
+        out["plain"]["bit_identical_to_reference_order"] = bool(
+            np.array_equal(res["plain"]["out0"], O.mp_fft(n, P, x0)))
+        ep = out["plain"]["error_vs_2P_oracle"]
+        for f in forms:
+            e = out[f]["error_vs_2P_oracle"]
+            if (e["max_ulp"] <= ep["max_ulp"] and e["normwise_2^-P"] <= ep["normwise_2^-P"]
+                    and e["zero_mismatch"] == 0 and res[f]["fft_s"] > res[chosen]["fft_s"]):
+                chosen = f
+        err_s = round(time.time() - t0, 2)
+
+    rc = res[chosen]
+    fft_s = rc["fft_s"]
+    ms_step = rc["total_ms"] / a.steps
+    # roofline of the chosen form's dominant kernel: algorithmic schoolbook
+    # 32x32->64 limb products at the precision's limb count L, i.e. the
+    # form's real products per transform times L^2, over that kernel's event
+    # time (plain: cmul(N) products, the reference's own count, in the
+    # combine levels; fixed: the twiddle products of the radix passes)
+    prod_per_fft = (cmul(n) if chosen == "plain" else fixed_products(n)) * Lc * Lc
+    kern_s = rc["parts_ms"][1] * 1e-3
+    launches = rc["S"] - (1 if chosen == "plain" else 2)
+    achieved = prod_per_fft * B / kern_s / 1e12
     peak = VALU_LANE_OPS / 1e12
     bytes_fft = (4 * n + 3 * n // 4) * (4 * Lc + 4)
     line = {
@@ -213,55 +245,27 @@ def main():
         "vs_baseline": None,
         "dtype": f"mp{P} (u32 limbs)",
         "data": "synthetic",
-        "config": {"workload": f"plain multiprecision CP split-radix FFT, N={n}, P={P} bits, "
-                               f"batch {B} transforms per step",
-                   "N": n, "precision_bits": P, "batch": B, "form": "plain",
+        "config": {"workload": f"multiprecision FFT N={n}, P={P} bits, batch {B} transforms "
+                               f"per step, {chosen} form",
+                   "N": n, "precision_bits": P, "batch": B, "form": chosen,
                    "parallelism": "single GPU"},
-        "forms": {"plain": {"fft_per_s": round(fft_s, 3)}, "chosen": "plain",
-                  "chosen_vs_plain": 1.0},
-        "kernel_ms": {"leaf_per_step": round(r["leaf_ms"] / a.steps, 4),
-                      "levels_per_step": round(r["lvl_ms"] / a.steps, 4),
-                      "level_launches_per_step": nlev,
-                      "mp_level_kernel_avg_us": round(r["lvl_ms"] / a.steps / nlev * 1e3, 2)},
+        "forms": out,
+        "chosen": chosen,
+        "chosen_vs_plain": round(fft_s / res["plain"]["fft_s"], 4),
         "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": round(peak, 4),
                      "unit": "T(32x32->64 limb products)/s", "frac": round(achieved / peak, 4),
-                     "traffic": profiled_traffic(n, P, B),
-                     "kernel": "mpfft::mp_level_kernel",
-                     "algorithmic_per_fft": prod_per_fft},
+                     "traffic": profiled_traffic(chosen, n, P, B),
+                     "kernel": PROFILES[chosen][1],
+                     "kernel_avg_us": round(kern_s / launches * 1e6, 2),
+                     "algorithmic_products_per_fft": prod_per_fft},
         "roofline_hbm": {"achieved": round(bytes_fft * fft_s / 1e9, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(bytes_fft * fft_s / 1e9 / HBM_PEAK_GBS, 5),
                          "compulsory_bytes_per_fft": bytes_fft},
     }
-    if a.prec2:
-        r2 = measure(M, I, torch, n, a.prec2, B, max(2, a.steps // 2), 1, a.seed)
-        line["forms"][f"plain_P{a.prec2}"] = {
-            "fft_per_s": round(B * max(2, a.steps // 2) / (r2["total_ms"] * 1e-3), 3)}
-
-    if a.fixed:
-        fx = measure_form(M, I, torch, n, P, B, a.steps, a.warmup, a.seed, M.FORM_FIXED)
-        line["forms"]["fixed"] = {"fft_per_s": round(fx["fft_s"], 3)}
-        fx_out0 = fx["out0"]
-        if a.prec2:
-            fx2 = measure_form(M, I, torch, n, a.prec2, B, max(2, a.steps // 2), 1, a.seed,
-                               M.FORM_FIXED)
-            line["forms"][f"fixed_P{a.prec2}"] = {"fft_per_s": round(fx2["fft_s"], 3)}
+    if not a.no_err:
+        line["error_seconds"] = err_s
     if not a.no_mpfr:
         line["mpfr_path"] = measure_mpfr(M, I, n, P, a.seed)
-
-    from oracle import oracle as O
-    if not a.no_err:
-        t0 = time.time()
-        x0 = r["x0"]
-        star = O.mp_fft(n, P, x0, Pw=2 * P)
-        mu, nw, zb = O.err_stats(n, P, r["out0"], Lc, star, I.limbs_for(2 * P))
-        same = bool(np.array_equal(r["out0"], O.mp_fft(n, P, x0)))
-        line["error_vs_2P_oracle"] = {"max_ulp": mu, "normwise_2^-P": nw, "zero_mismatch": zb,
-                                      "bit_identical_to_P_oracle": same,
-                                      "seconds": round(time.time() - t0, 2)}
-        if a.fixed:
-            mu2, nw2, zb2 = O.err_stats(n, P, fx_out0, Lc, star, I.limbs_for(2 * P))
-            line["forms"]["fixed"]["error_vs_2P_oracle"] = {"max_ulp": mu2, "normwise_2^-P": nw2,
-                                                           "zero_mismatch": zb2}
     if not a.no_cpu:
         # bounded sample: the reference's OpenMP CODE 2 (4 threads, hard-coded
         # at parallel_mp_fft2.c:262) when its build is present, and the oracle

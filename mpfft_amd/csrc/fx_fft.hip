@@ -16,6 +16,9 @@
 // input conversion reads the plain form's SoA-limb layout, the output
 // conversion rounds back to it at P bits.
 //
+// Twiddles: one table per radix-4 pass, entry 3k + r - 1 = w^(r k) of that
+// pass (k < Ns, r = 1..3), so the lanes of a wave read consecutive entries.
+//
 // Pass (radix R = 4 or 2, Ns = size of the sub-transforms already done):
 //   for j < N/R, k = j mod Ns:
 //     v_r = x[j + r N/R] * w^(r k), w = exp(-2 pi i / (R Ns))
@@ -129,7 +132,7 @@ fx_out_kernel(FxArgs a, const uint32_t *src)
 // by side and share it in L2.
 template <int W>
 __global__ void __launch_bounds__(256, 2)
-fx_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns)
+fx_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t toff)
 {
     const uint32_t N = a.N, M = 2 * N, Q = N >> 2;
     const uint32_t b = blockIdx.x % a.batch, jb = blockIdx.x / a.batch;
@@ -148,10 +151,11 @@ fx_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns)
         //   re' = k1 - k3, im' = k1 + k2   for w^t = c + i d.
         // re and im wait in LDS slots 0 and 1 and are replaced there by k2
         // and k3, so only s = re + im stays in registers across a product.
+        // The pass's twiddles are stored in (k, r) order from toff, so the
+        // lanes of a wave read consecutive entries.
         extern __shared__ uint32_t lds[];
         const uint32_t lt = threadIdx.x, nt = blockDim.x;
-        const uint32_t t = r * k * (N / (4 * Ns));
-        const uint32_t T3 = 3 * Q, MT = 3 * T3;
+        const uint32_t q = toff + 3 * k + (r - 1), TT = a.ntw, MT = 3 * TT;
         Fx<W> s;
         fx_add(s, re, im);
         fx_put(lds, nt, lt, 0, re);
@@ -159,9 +163,9 @@ fx_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns)
         uint32_t T[FxDig<W>::DT];
 #pragma unroll 1
         for (int it = 0; it < 3; it++) {
-            // table rows: c at [0, T3), d - c at [T3, 2 T3), c + d at [2 T3, 3 T3)
+            // table reals: c at [0, TT), d - c at [TT, 2 TT), c + d at [2 TT, 3 TT)
             const uint32_t u = it == 0 ? 2u : (it == 1 ? 1u : 0u);
-            const uint32_t tneg = tw_load<W>(T, a.tw, MT, u * T3 + t);
+            const uint32_t tneg = tw_load<W>(T, a.tw, MT, u * TT + q);
             if (it < 2) fx_get(re, lds, nt, lt, 1 - it);    // im, then re
             else re = s;
             fx_mul(re, re, T, tneg);
@@ -247,36 +251,50 @@ fx_r2_kernel(FxArgs a, const uint32_t *src, uint32_t *dst)
 
 // ---------------------------------------------------------------------------
 
+// Stages [s0, s1) of the transform: stage 0 converts the input, stages
+// 1 .. npass run the passes (buffer 0 -> 1 -> 0 ...), stage npass + 1
+// converts the output.
 template <int L, int W, bool FULL>
-static hipError_t launch_fx(const FxArgs &a, hipStream_t st)
+static hipError_t launch_fx(const FxArgs &a, int s0, int s1, hipStream_t st)
 {
     const uint32_t N = a.N, M = 2 * N;
     dim3 gc((M + 255) / 256, a.batch);
-    hipLaunchKernelGGL((fx_in_kernel<L, W>), gc, dim3(256), 0, st, a);
-    uint32_t *src = a.buf0, *dst = a.buf1;
     uint32_t lg = 0;
     while ((1u << lg) < N) lg++;
+    const int npass = fx_passes(N);
+    auto in_range = [&](int s) { return s >= s0 && s < s1; };
+    if (in_range(0)) hipLaunchKernelGGL((fx_in_kernel<L, W>), gc, dim3(256), 0, st, a);
+    uint32_t *src = a.buf0, *dst = a.buf1;
+    int stage = 1;
     uint32_t Ns = 1;
     if (lg & 1) {
         dim3 g(((N / 2) * 2 + 255) / 256 * a.batch);
-        hipLaunchKernelGGL((fx_r2_kernel<W>), g, dim3(256), 0, st, a, (const uint32_t *)src, dst);
+        if (in_range(stage))
+            hipLaunchKernelGGL((fx_r2_kernel<W>), g, dim3(256), 0, st, a, (const uint32_t *)src,
+                               dst);
         std::swap(src, dst);
         Ns = 2;
+        stage++;
     }
-    for (; Ns < N; Ns *= 4) {
+    for (uint32_t toff = 0; Ns < N; toff += 3 * Ns, Ns *= 4, stage++) {
         dim3 g(((N / 4) * 4 + 255) / 256 * a.batch);
         const size_t sh = sizeof(uint32_t) * 2 * W * 256;     // two parking slots per lane
-        hipLaunchKernelGGL((fx_r4_kernel<W>), g, dim3(256), sh, st, a, (const uint32_t *)src, dst, Ns);
+        if (in_range(stage))
+            hipLaunchKernelGGL((fx_r4_kernel<W>), g, dim3(256), sh, st, a, (const uint32_t *)src,
+                               dst, Ns, toff);
         std::swap(src, dst);
     }
-    hipLaunchKernelGGL((fx_out_kernel<W, L, FULL>), gc, dim3(256), 0, st, a, (const uint32_t *)src);
+    if (in_range(npass + 1))
+        hipLaunchKernelGGL((fx_out_kernel<W, L, FULL>), gc, dim3(256), 0, st, a,
+                           (const uint32_t *)src);
     return hipGetLastError();
 }
 
-hipError_t fx_fft_launch(int L, int P, const FxArgs &a, hipStream_t st)
+hipError_t fx_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st)
 {
     const bool full = (P == 32 * L);
-#define CALLX(Lv) (full ? launch_fx<Lv, Lv + 2, true>(a, st) : launch_fx<Lv, Lv + 2, false>(a, st))
+#define CALLX(Lv) (full ? launch_fx<Lv, Lv + 2, true>(a, s0, s1, st) \
+                        : launch_fx<Lv, Lv + 2, false>(a, s0, s1, st))
     switch (L) {
     case 1: return CALLX(1);
     case 2: return CALLX(2);

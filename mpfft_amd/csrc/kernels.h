@@ -10,9 +10,10 @@ namespace mpfft {
 
 struct LevelArgs {
     uint32_t *data;             // out buffer of transform 0
-    const uint32_t *lut;        // twiddle buffer: SoA of lut_m reals, rows of Q = N/4:
-                                // r, s, d (code 3) or r, i (codes 1, 2)
-    uint32_t lut_m;
+    const uint32_t *lut;        // twiddle buffer: SoA of lut_m reals, rows of lut_e
+                                // entries: r, s, d (code 3) or r, i (codes 1, 2);
+                                // level n uses entries lut_off + k = LUT[k N/n]
+    uint32_t lut_m, lut_e, lut_off;
     int code;                   // operation order: 3 (3m3a), 2 (4m2a), 1 (4m2a, no
                                 // special butterflies)
     const uint32_t *node_off;   // output offsets of the size-n nodes
@@ -67,8 +68,11 @@ struct FxArgs {
     uint64_t out_stride;
     uint32_t *buf0, *buf1;      // fixed-point ping-pong buffers (W rows of 2N words)
     uint64_t stride;            // words per transform in buf0 / buf1
-    const uint32_t *tw;         // twiddles w^t, t < 3N/4: rows c, d - c, c + d (w^t =
-                                // c + i d), each DT digit rows + a sign row
+    const uint32_t *tw;         // twiddles of the radix-4 passes, pass after pass,
+                                // entry 3k + r - 1 = w^(r k N / (4 Ns)) = c + i d;
+                                // reals c, d - c, c + d (3 ntw of them), each DT
+                                // digit rows + a sign row
+    uint32_t ntw;               // twiddle entries in all
     uint32_t *flag;             // set to nonzero when an input exceeds the range
     uint32_t N, batch;
     int lim;                    // inputs must have exponent <= lim (|x| < 2^lim)
@@ -78,7 +82,17 @@ struct FxArgs {
 // fixed-point words per real for limb count L (F = 32 (L + 1) fraction bits)
 constexpr int fx_words(int L) { return L + 2; }
 
-hipError_t fx_fft_launch(int L, int P, const FxArgs &a, hipStream_t st);
+// passes of the fixed form: a radix-2 pass when log2 N is odd, then radix 4
+inline int fx_passes(uint32_t N)
+{
+    int lg = 0;
+    while ((1u << lg) < N) lg++;
+    return (lg & 1) + lg / 2;
+}
+
+// stages [s0, s1): 0 input conversion, 1 .. fx_passes(N) passes, then the
+// output conversion
+hipError_t fx_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st);
 
 // ---- double path (d_fft.hip) ----
 

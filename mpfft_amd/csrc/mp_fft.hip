@@ -8,10 +8,12 @@
 // Data layout in HBM (per transform): buffer = hdr int32[2N] then limb
 // uint32[L][2N]; element i < N is the real part of X[i], N+i its imaginary
 // part.  Limb-major SoA: a wave reading limb j of 32 consecutive elements
-// touches one 128-byte segment.  Twiddles (Q = N/4 per row, SoA): for the
-// 3m3a order of CODE 3, LUTr at [0,Q), LUTs at [Q,2Q), LUTd at [2Q,3Q)
-// (LUTi is not used, serial_mp_fft3.c:79-80); for the 4m2a orders of CODE 1
-// and CODE 2 (serial_mp_fft1.c, serial_mp_fft2.c), LUTr then LUTi.
+// touches one 128-byte segment.  Twiddles (SoA): every level n has its own
+// run of n/4 entries, entry k = LUT[k N/n], so the lanes of a wave (k
+// consecutive) read consecutive entries; rows of E entries: for the 3m3a
+// order of CODE 3, LUTr, LUTs, LUTd (LUTi is not used,
+// serial_mp_fft3.c:79-80); for the 4m2a orders of CODE 1 and CODE 2
+// (serial_mp_fft1.c, serial_mp_fft2.c), LUTr then LUTi.
 //
 // Work mapping: two lanes (a DPP quad pair) per butterfly.  The even lane
 // computes m1 = Z+ * w^k, the odd lane m2 = Z- * conj(w^k), with the same
@@ -137,7 +139,7 @@ mp_level_kernel(LevelArgs a)
     const uint32_t o = a.node_off[node];
     const uint32_t zi = o + k + (B ? 3 * q : 2 * q);  // Z+ or Z-
     const uint32_t ui = o + k + (B ? q : 0);          // U0 or U1
-    const uint32_t kn = k * (N / n);
+    const uint32_t kt = a.lut_off + k;                // this level's twiddle entry k
 
     // o1, o2: m1 = Z+ * w^k on the Z+ lane, m2 = Z- * conj(w^k) on the Z- lane.
     // Intermediates that must survive a multiplication wait in LDS slots
@@ -176,10 +178,11 @@ mp_level_kernel(LevelArgs a)
             if (it == 0 && !m4) f = r;
             else if (kind == 1) lds_get(f, lds, nt, lt, 1);
             else mp_load(f, out, M, (m4 ? (it == 0 || it == 3) : it == 2) ? zi : N + zi);
+            const uint32_t E = a.lut_e;
             if (kind == 1) row = 0;
-            else if (m4) row = (it & 1) ? Q : 0;
-            else row = it == 0 ? 0 : (it == 1 ? 2 * Q : Q);
-            mp_load(w, a.lut, a.lut_m, row + kn);
+            else if (m4) row = (it & 1) ? E : 0;
+            else row = it == 0 ? 0 : (it == 1 ? 2 * E : E);
+            mp_load(w, a.lut, a.lut_m, row + kt);
             mp_mul<L, FULL>(r, f, w, P);
             if (m4 ? (it & 1) : (kind == 0 && it > 0)) {
                 Mp<L> T;

@@ -231,6 +231,7 @@ struct mpfft_plan_s {
     // fixed form
     DevBuf fx_tw, fx_buf0, fx_buf1, fx_flag;
     int fx_lim = 0;
+    uint32_t fx_ntw = 0;
     // double forms
     DevBuf tw, dleaf_sn, dleaf_off, dleaf_n, dlvl_off, dlvl_begin;
     DTables dt{};
@@ -271,10 +272,27 @@ int build_mp_plan(mpfft_plan_s *p)
         all.insert(all.end(), t.level[l].begin(), t.level[l].end());
     }
     if ((rc = p->nodes.upload(all))) return rc;
+    // per level n >= 8: entries k < n/4 gathered from the base table
+    // (entry k of level n = base entry k N/n), so a wave's lanes read
+    // consecutive entries
+    const int rows = mp_code(p->form) == 3 ? 3 : 2;
+    size_t E = 0;
+    std::vector<uint32_t> lvl_off(t.lgN + 2, 0);
+    for (uint32_t l = 3; l <= t.lgN; l++) { lvl_off[l] = (uint32_t)E; E += (1UL << l) / 4; }
     if (N >= 8) {
-        std::vector<uint32_t> lutbuf;
-        build_mp_lut(N, p->P, p->L, mp_code(p->form), lutbuf);
-        if ((rc = p->lut.upload(lutbuf))) return rc;
+        std::vector<uint32_t> base;
+        build_mp_lut(N, p->P, p->L, mp_code(p->form), base);
+        const size_t Q = N / 4, MB = rows * Q, ML = rows * E;
+        std::vector<uint32_t> lut(ML * (p->L + 1));
+        for (uint32_t l = 3; l <= t.lgN; l++) {
+            const size_t n = 1UL << l, step = N / n;
+            for (size_t k = 0; k < n / 4; k++)
+                for (int u = 0; u < rows; u++) {
+                    const size_t src = u * Q + k * step, dst = u * E + lvl_off[l] + k;
+                    for (long j = 0; j <= p->L; j++) lut[j * ML + dst] = base[j * MB + src];
+                }
+        }
+        if ((rc = p->lut.upload(lut))) return rc;
     }
     // element-major staging copy of the input for the leaf gathers
     const uint32_t aw = (uint32_t)((p->L + 1 + 3) & ~3L);
@@ -293,7 +311,9 @@ int build_mp_plan(mpfft_plan_s *p)
         const uint32_t n = 1u << l, cnt = (uint32_t)t.level[l].size();
         a.lut = p->lut.as<uint32_t>();
         a.code = mp_code(p->form);
-        a.lut_m = (uint32_t)((a.code == 3 ? 3 : 2) * (N / 4));
+        a.lut_e = (uint32_t)E;
+        a.lut_m = (uint32_t)(rows * E);
+        a.lut_off = lvl_off[l];
         a.node_off = p->nodes.as<uint32_t>() + first[l];
         a.stride = p->words;
         a.N = (uint32_t)N; a.n = n; a.cnt = cnt;
@@ -303,16 +323,24 @@ int build_mp_plan(mpfft_plan_s *p)
     return MPFFT_OK;
 }
 
-// Fixed form: twiddles w^t = c + i d = exp(-2 pi i t / N), t < 3N/4, as
-// c, d - c and c + d rounded to F = 32 (L + 1) fraction bits, each stored as
-// DT 29-bit digits of its magnitude (least significant first) and a sign
-// word; SoA over the 3 * (3N/4) reals.
+// Fixed form: for every radix-4 pass (Ns = 1 or 2, then times 4 while
+// Ns < N), entries 3k + r - 1 (k < Ns, r = 1..3) hold w^t = c + i d =
+// exp(-2 pi i t / N), t = r k N / (4 Ns), as c, d - c and c + d rounded to
+// F = 32 (L + 1) fraction bits, each stored as DT 29-bit digits of its
+// magnitude (least significant first) and a sign word; SoA over the 3 T3
+// reals.
 int build_fx_plan(mpfft_plan_s *p)
 {
     const unsigned long N = p->N;
     const long W = p->L + 2, F = 32 * (W - 1);
     const long DT = (F + 1 + 28) / 29;
-    const size_t T3 = 3 * (N / 4), MT = 3 * T3;
+    int lg0 = 0;
+    while ((1UL << lg0) < N) lg0++;
+    std::vector<unsigned long> ent;               // t of every entry, in table order
+    for (unsigned long Ns = (lg0 & 1) ? 2 : 1; Ns < N; Ns *= 4)
+        for (unsigned long k = 0; k < Ns; k++)
+            for (unsigned long r = 1; r <= 3; r++) ent.push_back(r * k * (N / (4 * Ns)));
+    const size_t T3 = ent.size(), MT = 3 * T3;
     std::vector<uint32_t> tab(MT > 0 ? MT * (DT + 1) : 1, 0);
     unsigned nt = std::thread::hardware_concurrency();
     nt = std::max(1u, std::min(nt, 16u));
@@ -323,7 +351,8 @@ int build_fx_plan(mpfft_plan_s *p)
         mpz_t z;
         mpz_init(z);
         std::vector<uint32_t> words((size_t)(F / 32 + 4));
-        for (size_t t = t0; t < t1; t++) {
+        for (size_t e = t0; e < t1; e++) {
+            const size_t t = ent[e];
             mpfr_const_pi(th, MPFR_RNDN);
             mpfr_mul_ui(th, th, 2 * t, MPFR_RNDN);
             mpfr_div_ui(th, th, N, MPFR_RNDN);
@@ -335,7 +364,7 @@ int build_fx_plan(mpfft_plan_s *p)
                 else mpfr_add(v, c, d, MPFR_RNDN);
                 mpfr_mul_2si(v, v, F, MPFR_RNDN);
                 mpfr_get_z(z, v, MPFR_RNDN);
-                const size_t q = u * T3 + t;
+                const size_t q = u * T3 + e;
                 tab[(size_t)DT * MT + q] = mpz_sgn(z) < 0 ? 1u : 0u;
                 mpz_abs(z, z);
                 std::fill(words.begin(), words.end(), 0u);
@@ -362,24 +391,26 @@ int build_fx_plan(mpfft_plan_s *p)
         (rc = p->fx_buf1.alloc(sizeof(uint32_t) * fxw)) ||
         (rc = p->fx_flag.upload(std::vector<uint32_t>{0u})))
         return rc;
-    int lg = 0;
-    while ((1UL << lg) < N) lg++;
-    p->fx_lim = 29 - lg;
+    p->fx_ntw = (uint32_t)T3;
+    p->fx_lim = 29 - lg0;
     return MPFFT_OK;
 }
 
-int run_fx(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t st)
+int run_fx(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t st,
+           int s0 = 0, int s1 = 1 << 30)
 {
+    if (s0 < 0 || s0 > s1) return MPFFT_ERR_ARG;
     FxArgs a{};
     a.in = d_in; a.in_stride = p->words;
     a.out = d_out; a.out_stride = p->words;
     a.buf0 = p->fx_buf0.as<uint32_t>(); a.buf1 = p->fx_buf1.as<uint32_t>();
     a.stride = 2 * p->N * (uint64_t)(p->L + 2);
     a.tw = p->fx_tw.as<uint32_t>();
+    a.ntw = p->fx_ntw;
     a.flag = p->fx_flag.as<uint32_t>();
     a.N = (uint32_t)p->N; a.batch = p->batch;
     a.lim = p->fx_lim; a.P = (int)p->P;
-    hipError_t e = fx_fft_launch((int)p->L, (int)p->P, a, st);
+    hipError_t e = fx_fft_launch((int)p->L, (int)p->P, a, s0, s1, st);
     return e == hipSuccess ? MPFFT_OK : MPFFT_ERR_HIP;
 }
 
@@ -737,15 +768,17 @@ int mpfft_execute_limbs_range(mpfft_plan p, const uint32_t *d_in, uint32_t *d_ou
                               void *stream, int stage_begin, int stage_end)
 {
     if (!p || !d_in || !d_out) return MPFFT_ERR_ARG;
-    if (!is_mp(p->form)) return MPFFT_ERR_FORM;
+    if (!is_limb_form(p->form)) return MPFFT_ERR_FORM;
     HIP_TRY(hipSetDevice(p->device));
-    return run_mp(p, d_in, d_out, stream ? (hipStream_t)stream : p->stream,
-                  stage_begin, stage_end);
+    hipStream_t st = stream ? (hipStream_t)stream : p->stream;
+    if (p->form == MPFFT_FORM_FIXED) return run_fx(p, d_in, d_out, st, stage_begin, stage_end);
+    return run_mp(p, d_in, d_out, st, stage_begin, stage_end);
 }
 
 int mpfft_plan_stages(mpfft_plan p)
 {
     if (!p) return MPFFT_ERR_ARG;
+    if (p->form == MPFFT_FORM_FIXED) return fx_passes((uint32_t)p->N) + 2;
     return is_mp(p->form) ? 1 + (int)p->levels.size() : MPFFT_ERR_FORM;
 }
 
