@@ -71,8 +71,8 @@ def profiled_traffic(form, n, P, batch):
 def measure(M, I, torch, n, P, batch, steps, warmup, seed, form):
     """Time `steps` transforms of `batch` on device-resident data, with HIP
     events (on the launch stream) around the parts of every step: plain
-    form: leaf pass | combine levels; fixed form: input conversion | radix
-    passes | output conversion."""
+    form: leaf pass | combine levels; fixed form: the radix passes (the
+    first and last also convert from and to the float layout)."""
     plan = M.Plan(n, P, form, batch)
     Lc = I.limbs_for(P)
     host = np.concatenate([I.relimb(I.mp_random(n, P, seed=seed + b), n, Lc, plan.L)
@@ -82,7 +82,7 @@ def measure(M, I, torch, n, P, batch, steps, warmup, seed, form):
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     S = plan.stages()
-    cuts = [0, 1, S] if form != M.FORM_FIXED else [0, 1, S - 1, S]
+    cuts = [0, 1, S] if form != M.FORM_FIXED else [0, S]
     for _ in range(warmup):
         plan.execute_range(x.data_ptr(), y.data_ptr(), 0, S, sp)
     torch.cuda.synchronize()
@@ -187,10 +187,8 @@ def main():
                                             "levels": round(r["parts_ms"][1], 4),
                                             "level_launches": r["S"] - 1}
         else:
-            out[f]["kernel_ms_per_step"] = {"convert_in": round(r["parts_ms"][0], 4),
-                                            "passes": round(r["parts_ms"][1], 4),
-                                            "convert_out": round(r["parts_ms"][2], 4),
-                                            "pass_launches": r["S"] - 2}
+            out[f]["kernel_ms_per_step"] = {"passes": round(r["parts_ms"][0], 4),
+                                            "pass_launches": r["S"]}
     if a.prec2:
         for f in forms:
             r2 = measure(M, I, torch, n, a.prec2, B, max(2, a.steps // 2), 1, a.seed, form_ids[f])
@@ -227,8 +225,8 @@ def main():
     # time (plain: cmul(N) products, the reference's own count, in the
     # combine levels; fixed: the twiddle products of the radix passes)
     prod_per_fft = (cmul(n) if chosen == "plain" else fixed_products(n)) * Lc * Lc
-    kern_s = rc["parts_ms"][1] * 1e-3
-    launches = rc["S"] - (1 if chosen == "plain" else 2)
+    kern_s = rc["parts_ms"][1 if chosen == "plain" else 0] * 1e-3
+    launches = rc["S"] - (1 if chosen == "plain" else 0)
     achieved = prod_per_fft * B / kern_s / 1e12
     peak = VALU_LANE_OPS / 1e12
     bytes_fft = (4 * n + 3 * n // 4) * (4 * Lc + 4)

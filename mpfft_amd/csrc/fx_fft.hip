@@ -13,8 +13,9 @@
 //
 // Data: two ping-pong buffers of W rows of 2N words per transform (real
 // parts at [0, N), imaginary at [N, 2N)); word 0 least significant.  The
-// input conversion reads the plain form's SoA-limb layout, the output
-// conversion rounds back to it at P bits.
+// first pass reads the plain form's SoA-limb layout and converts on the fly,
+// the last pass rounds back to it at P bits, so the conversions cost no
+// extra pass over the data.
 //
 // Twiddles: one table per radix-4 pass, entry 3k + r - 1 = w^(r k) of that
 // pass (k < Ns, r = 1..3), so the lanes of a wave read consecutive entries.
@@ -97,6 +98,46 @@ __device__ __forceinline__ uint32_t tw_load(uint32_t (&T)[FxDig<W>::DT], const u
 
 }  // namespace
 
+// Pass I/O.  The first pass reads the float SoA input and converts on the
+// fly (IO bit 1), the last pass rounds to P bits and writes the float SoA
+// output (IO bit 2); the passes between read and write fixed point.
+template <int L, int W, int IO>
+__device__ __forceinline__ void fx_pass_load(Fx<W> &re, Fx<W> &im, const FxArgs &a,
+                                             const uint32_t *src, uint32_t b, uint32_t i)
+{
+    const uint32_t N = a.N, M = 2 * N;
+    if (IO & 1) {
+        const uint32_t *in = a.in + a.in_stride * b;
+        Mp<L> x;
+        mp_load(x, in, M, i);
+        uint32_t bad = fx_from_mp<L, W>(re, x, a.lim);
+        mp_load(x, in, M, N + i);
+        bad |= fx_from_mp<L, W>(im, x, a.lim);
+        if (bad) atomicOr(a.flag, 1u);
+    } else {
+        fx_load(re, src, M, i);
+        fx_load(im, src, M, N + i);
+    }
+}
+
+template <int L, int W, bool FULL, int IO>
+__device__ __forceinline__ void fx_pass_store(const Fx<W> &re, const Fx<W> &im, const FxArgs &a,
+                                              uint32_t *dst, uint32_t b, uint32_t o)
+{
+    const uint32_t N = a.N, M = 2 * N;
+    if (IO & 2) {
+        uint32_t *out = a.out + a.out_stride * b;
+        Mp<L> z;
+        fx_to_mp<W, L, FULL>(z, re, a.P);
+        mp_store(z, out, M, o);
+        fx_to_mp<W, L, FULL>(z, im, a.P);
+        mp_store(z, out, M, N + o);
+    } else {
+        fx_store(re, dst, M, o);
+        fx_store(im, dst, M, N + o);
+    }
+}
+
 // float SoA (hdr + L limbs) -> fixed point; flags inputs beyond the range
 template <int L, int W>
 __global__ void __launch_bounds__(256)
@@ -130,11 +171,11 @@ fx_out_kernel(FxArgs a, const uint32_t *src)
 // One radix-4 pass.  Blocks are ordered transform-fastest, so the blocks
 // that read one twiddle for the different transforms of a batch run side
 // by side and share it in L2.
-template <int W>
+template <int L, int W, bool FULL, int IO>
 __global__ void __launch_bounds__(256, 2)
 fx_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t toff)
 {
-    const uint32_t N = a.N, M = 2 * N, Q = N >> 2;
+    const uint32_t N = a.N, Q = N >> 2;
     const uint32_t b = blockIdx.x % a.batch, jb = blockIdx.x / a.batch;
     const uint32_t r = threadIdx.x & 3u;
     const uint32_t j = jb * (blockDim.x >> 2) + (threadIdx.x >> 2);
@@ -142,8 +183,7 @@ fx_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t
     src += a.stride * b;
     dst += a.stride * b;
     Fx<W> re, im;
-    fx_load(re, src, M, j + r * Q);
-    fx_load(im, src, M, N + j + r * Q);
+    fx_pass_load<L, W, IO>(re, im, a, src, b, j + r * Q);
     const uint32_t k = j & (Ns - 1);
     if (r != 0 && k != 0) {
         // v_r *= w^t, t = r k N / (4 Ns), by three products (Gauss):
@@ -217,16 +257,15 @@ fx_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t
     }
     const uint32_t rout = (r == 1) ? 2u : (r == 2 ? 1u : r);
     const uint32_t o = (j - k) * 4 + k + rout * Ns;
-    fx_store(re, dst, M, o);
-    fx_store(im, dst, M, N + o);
+    fx_pass_store<L, W, FULL, IO>(re, im, a, dst, b, o);
 }
 
 // The radix-2 pass (first pass when log2 N is odd: Ns = 1, no twiddles).
-template <int W>
+template <int L, int W, bool FULL, int IO>
 __global__ void __launch_bounds__(256, 2)
 fx_r2_kernel(FxArgs a, const uint32_t *src, uint32_t *dst)
 {
-    const uint32_t N = a.N, M = 2 * N, H = N >> 1;
+    const uint32_t N = a.N, H = N >> 1;
     const uint32_t b = blockIdx.x % a.batch, jb = blockIdx.x / a.batch;
     const uint32_t r = threadIdx.x & 1u;
     const uint32_t j = jb * (blockDim.x >> 1) + (threadIdx.x >> 1);
@@ -234,8 +273,7 @@ fx_r2_kernel(FxArgs a, const uint32_t *src, uint32_t *dst)
     src += a.stride * b;
     dst += a.stride * b;
     Fx<W> re, im, pre, pim, x, y;
-    fx_load(re, src, M, j + r * H);
-    fx_load(im, src, M, N + j + r * H);
+    fx_pass_load<L, W, IO>(re, im, a, src, b, j + r * H);
     fx_dpp(pre, re, 0xB1);
     fx_dpp(pim, im, 0xB1);
     // lane 0: x0 + x1; lane 1: x0 - x1 (partner - own)
@@ -245,33 +283,43 @@ fx_r2_kernel(FxArgs a, const uint32_t *src, uint32_t *dst)
     fx_select(x, r, pim, im);
     fx_select(y, r, im, pim);
     fx_addsub(im, x, y, r);
-    fx_store(re, dst, M, 2 * j + r);
-    fx_store(im, dst, M, N + 2 * j + r);
+    fx_pass_store<L, W, FULL, IO>(re, im, a, dst, b, 2 * j + r);
 }
 
 // ---------------------------------------------------------------------------
 
-// Stages [s0, s1) of the transform: stage 0 converts the input, stages
-// 1 .. npass run the passes (buffer 0 -> 1 -> 0 ...), stage npass + 1
-// converts the output.
+// Stages [s0, s1) of the transform: one per pass (buffer 0 -> 1 -> 0 ...),
+// the first converting the input on the fly, the last the output.  N = 1
+// has no pass: stage 0 converts in, stage 1 out.
 template <int L, int W, bool FULL>
 static hipError_t launch_fx(const FxArgs &a, int s0, int s1, hipStream_t st)
 {
     const uint32_t N = a.N, M = 2 * N;
-    dim3 gc((M + 255) / 256, a.batch);
     uint32_t lg = 0;
     while ((1u << lg) < N) lg++;
     const int npass = fx_passes(N);
     auto in_range = [&](int s) { return s >= s0 && s < s1; };
-    if (in_range(0)) hipLaunchKernelGGL((fx_in_kernel<L, W>), gc, dim3(256), 0, st, a);
-    uint32_t *src = a.buf0, *dst = a.buf1;
-    int stage = 1;
+    if (npass == 0) {
+        dim3 gc((M + 255) / 256, a.batch);
+        if (in_range(0)) hipLaunchKernelGGL((fx_in_kernel<L, W>), gc, dim3(256), 0, st, a);
+        if (in_range(1))
+            hipLaunchKernelGGL((fx_out_kernel<W, L, FULL>), gc, dim3(256), 0, st, a,
+                               (const uint32_t *)a.buf0);
+        return hipGetLastError();
+    }
+    uint32_t *src = a.buf1, *dst = a.buf0;
+    int stage = 0;
     uint32_t Ns = 1;
     if (lg & 1) {
         dim3 g(((N / 2) * 2 + 255) / 256 * a.batch);
-        if (in_range(stage))
-            hipLaunchKernelGGL((fx_r2_kernel<W>), g, dim3(256), 0, st, a, (const uint32_t *)src,
-                               dst);
+        if (in_range(stage)) {
+            if (npass == 1)
+                hipLaunchKernelGGL((fx_r2_kernel<L, W, FULL, 3>), g, dim3(256), 0, st, a,
+                                   (const uint32_t *)src, dst);
+            else
+                hipLaunchKernelGGL((fx_r2_kernel<L, W, FULL, 1>), g, dim3(256), 0, st, a,
+                                   (const uint32_t *)src, dst);
+        }
         std::swap(src, dst);
         Ns = 2;
         stage++;
@@ -279,14 +327,24 @@ static hipError_t launch_fx(const FxArgs &a, int s0, int s1, hipStream_t st)
     for (uint32_t toff = 0; Ns < N; toff += 3 * Ns, Ns *= 4, stage++) {
         dim3 g(((N / 4) * 4 + 255) / 256 * a.batch);
         const size_t sh = sizeof(uint32_t) * 2 * W * 256;     // two parking slots per lane
-        if (in_range(stage))
-            hipLaunchKernelGGL((fx_r4_kernel<W>), g, dim3(256), sh, st, a, (const uint32_t *)src,
-                               dst, Ns, toff);
+        const bool first = stage == 0, last = stage == npass - 1;
+        if (in_range(stage)) {
+            const uint32_t *s_ = src;
+            if (first && last)
+                hipLaunchKernelGGL((fx_r4_kernel<L, W, FULL, 3>), g, dim3(256), sh, st, a, s_,
+                                   dst, Ns, toff);
+            else if (first)
+                hipLaunchKernelGGL((fx_r4_kernel<L, W, FULL, 1>), g, dim3(256), sh, st, a, s_,
+                                   dst, Ns, toff);
+            else if (last)
+                hipLaunchKernelGGL((fx_r4_kernel<L, W, FULL, 2>), g, dim3(256), sh, st, a, s_,
+                                   dst, Ns, toff);
+            else
+                hipLaunchKernelGGL((fx_r4_kernel<L, W, FULL, 0>), g, dim3(256), sh, st, a, s_,
+                                   dst, Ns, toff);
+        }
         std::swap(src, dst);
     }
-    if (in_range(npass + 1))
-        hipLaunchKernelGGL((fx_out_kernel<W, L, FULL>), gc, dim3(256), 0, st, a,
-                           (const uint32_t *)src);
     return hipGetLastError();
 }
 

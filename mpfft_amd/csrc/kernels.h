@@ -90,8 +90,11 @@ inline int fx_passes(uint32_t N)
     return (lg & 1) + lg / 2;
 }
 
-// stages [s0, s1): 0 input conversion, 1 .. fx_passes(N) passes, then the
-// output conversion
+// stages of the fixed form: one per pass (the first converts the input, the
+// last the output); N = 1: input and output conversion
+inline int fx_stages(uint32_t N) { return fx_passes(N) > 0 ? fx_passes(N) : 2; }
+
+// stages [s0, s1) of the fixed form
 hipError_t fx_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st);
 
 // ---- double path (d_fft.hip) ----

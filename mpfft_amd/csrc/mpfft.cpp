@@ -778,7 +778,7 @@ int mpfft_execute_limbs_range(mpfft_plan p, const uint32_t *d_in, uint32_t *d_ou
 int mpfft_plan_stages(mpfft_plan p)
 {
     if (!p) return MPFFT_ERR_ARG;
-    if (p->form == MPFFT_FORM_FIXED) return fx_passes((uint32_t)p->N) + 2;
+    if (p->form == MPFFT_FORM_FIXED) return fx_stages((uint32_t)p->N);
     return is_mp(p->form) ? 1 + (int)p->levels.size() : MPFFT_ERR_FORM;
 }
 
