@@ -2,7 +2,8 @@
 """Benchmark of the multiprecision split-radix FFT on one MI355X:
 mp-FFTs/s at N = 2^16, 1024 and 256 bits, for the plain form (bit-identical
 to the reference's MPFR path) and the fixed form (radix-4 Stockham on
-fixed-point reals); the "chosen" form is the fastest one whose error
+fixed-point reals), on panels (twiddle products on the int8 matrix cores,
+"fixed") and on lanes ("fixed_lane"); the "chosen" form is the fastest one whose error
 against a 2P-bit oracle is no worse than the plain form's, componentwise
 (max ulp) and normwise.  Also: the dominant kernel's roofline, the CPU
 baseline and the drop-in call from mpfr_t arrays.
@@ -29,10 +30,12 @@ HBM_PEAK_GBS = 8000.0
 # quarter of it.  v_mad_u64_u32 (one 32x32->64 product, plus a 64-bit
 # addend) issues at that full rate on gfx950 (mpfft_amd/valu_probe).
 VALU_LANE_OPS = 157.3e12 / 4
+MFMA_I8_PEAK_T = 5000.0          # dense int8 matrix-core peak, TOP/s
 
 PROFILES = {
     "plain": ("mp_level_kernel_N65536_P1024.txt", "mpfft::mp_level_kernel"),
-    "fixed": ("fx_r4_kernel_N65536_P1024.txt", "mpfft::fx_r4_kernel"),
+    "fixed": ("fxp_r4_kernel_N65536_P1024.txt", "mpfft::fxp_r4_kernel"),
+    "fixed_lane": ("fx_r4_kernel_N65536_P1024.txt", "mpfft::fx_r4_kernel"),
 }
 
 
@@ -53,6 +56,21 @@ def fixed_products(n):
     return tot
 
 
+def mfma_tiles(W):
+    """32x32x32 int8 tiles of one real twiddle product at W words: tile
+    (p, q) of the Toeplitz product is issued when it holds a twiddle digit
+    (mf_live in mpfft_amd/csrc/kernels.h)"""
+    nxb, ntb, c0, ncol = 4 * W, 4 * W - 3, 4 * W - 8, 4 * W + 4
+    t = 0
+    for p in range((ncol + 31) // 32):
+        for q in range((nxb + 31) // 32):
+            mmax = min(ncol - 32 * p, 32) - 1
+            kmax = min(nxb - 32 * q, 32) - 1
+            lo, hi = c0 + 32 * (p - q) - kmax, c0 + 32 * (p - q) + mmax
+            t += mmax >= 0 and kmax >= 0 and hi >= 0 and lo <= ntb - 1
+    return t
+
+
 def profiled_traffic(form, n, P, batch):
     """HBM bytes per launch of the form's dominant kernel (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy-kernel calibration) from the committed
@@ -71,8 +89,9 @@ def profiled_traffic(form, n, P, batch):
 def measure(M, I, torch, n, P, batch, steps, warmup, seed, form):
     """Time `steps` transforms of `batch` on device-resident data, with HIP
     events (on the launch stream) around the parts of every step: plain
-    form: leaf pass | combine levels; fixed form: the radix passes (the
-    first and last also convert from and to the float layout)."""
+    form: leaf pass | combine levels; fixed form on panels: input
+    conversion | radix passes | output conversion; fixed form on lanes: the
+    radix passes (the first and last also convert from and to floats)."""
     plan = M.Plan(n, P, form, batch)
     Lc = I.limbs_for(P)
     host = np.concatenate([I.relimb(I.mp_random(n, P, seed=seed + b), n, Lc, plan.L)
@@ -82,7 +101,8 @@ def measure(M, I, torch, n, P, batch, steps, warmup, seed, form):
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     S = plan.stages()
-    cuts = [0, 1, S] if form != M.FORM_FIXED else [0, S]
+    panel = form == M.FORM_FIXED and batch % 64 == 0 and n >= 16
+    cuts = [0, S] if form in M.FIXED_FORMS and not panel else ([0, 1, S - 1, S] if panel else [0, 1, S])
     for _ in range(warmup):
         plan.execute_range(x.data_ptr(), y.data_ptr(), 0, S, sp)
     torch.cuda.synchronize()
@@ -104,7 +124,7 @@ def measure(M, I, torch, n, P, batch, steps, warmup, seed, form):
     x0 = I.relimb(host[: plan.words], n, plan.L, Lc)
     plan.close()
     return dict(total_ms=total_ms, fft_s=batch * steps / (total_ms * 1e-3), parts_ms=parts,
-                S=S, out0=out0, x0=x0)
+                S=S, out0=out0, x0=x0, panel=panel, L=plan.L)
 
 
 def measure_mpfr(M, I, n, P, seed, reps=3):
@@ -146,7 +166,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--prec2", type=int, default=256, help="second precision (0 = skip)")
     ap.add_argument("--seed", type=int, default=0x6D7046465420)
-    ap.add_argument("--forms", default="plain,fixed", help="forms to measure")
+    ap.add_argument("--forms", default="plain,fixed,fixed_lane", help="forms to measure")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-err", action="store_true", help="skip the 2P-bit oracle error")
     ap.add_argument("--no-mpfr", action="store_true", help="skip the mpfr_t-array call timing")
@@ -165,7 +185,7 @@ def main():
 
     n, P, B = a.n, a.prec, a.batch
     Lc = I.limbs_for(P)
-    form_ids = {"plain": M.FORM_MP, "fixed": M.FORM_FIXED}
+    form_ids = {"plain": M.FORM_MP, "fixed": M.FORM_FIXED, "fixed_lane": M.FORM_FIXED_LANE}
     forms = [f for f in a.forms.split(",") if f]
     if "plain" not in forms:
         forms.insert(0, "plain")
@@ -186,6 +206,11 @@ def main():
             out[f]["kernel_ms_per_step"] = {"leaf": round(r["parts_ms"][0], 4),
                                             "levels": round(r["parts_ms"][1], 4),
                                             "level_launches": r["S"] - 1}
+        elif r["panel"]:
+            out[f]["kernel_ms_per_step"] = {"in": round(r["parts_ms"][0], 4),
+                                            "passes": round(r["parts_ms"][1], 4),
+                                            "out": round(r["parts_ms"][2], 4),
+                                            "pass_launches": r["S"] - 2}
         else:
             out[f]["kernel_ms_per_step"] = {"passes": round(r["parts_ms"][0], 4),
                                             "pass_launches": r["S"]}
@@ -219,16 +244,51 @@ def main():
     rc = res[chosen]
     fft_s = rc["fft_s"]
     ms_step = rc["total_ms"] / a.steps
-    # roofline of the chosen form's dominant kernel: algorithmic schoolbook
-    # 32x32->64 limb products at the precision's limb count L, i.e. the
-    # form's real products per transform times L^2, over that kernel's event
-    # time (plain: cmul(N) products, the reference's own count, in the
-    # combine levels; fixed: the twiddle products of the radix passes)
-    prod_per_fft = (cmul(n) if chosen == "plain" else fixed_products(n)) * Lc * Lc
-    kern_s = rc["parts_ms"][1 if chosen == "plain" else 0] * 1e-3
-    launches = rc["S"] - (1 if chosen == "plain" else 0)
-    achieved = prod_per_fft * B / kern_s / 1e12
-    peak = VALU_LANE_OPS / 1e12
+    # roofline of the chosen form's dominant kernel.
+    #   plain, fixed on lanes (VALU-bound): algorithmic schoolbook 32x32->64
+    #     limb products at the precision's limb count L, i.e. the form's real
+    #     products per transform times L^2 (plain: cmul(N), the reference's
+    #     own count, in the combine levels; fixed: the twiddle products of
+    #     the radix passes), over that kernel's event time;
+    #   fixed on panels (fxp_r4_kernel): HBM bytes a radix pass must move,
+    #     one read and one write of 2N fixed-point reals of W = L + 2 words
+    #     per transform, over the average pass time; the matrix-core side is
+    #     reported beside it (int8 MFMA operations the pass issues).
+    if rc["panel"]:
+        W = rc["L"] + 2
+        S = rc["S"]
+        kern_s = rc["parts_ms"][1] * 1e-3
+        launches = S - 2
+        bytes_launch = 2 * (2 * n * W * 4) * B
+        achieved = bytes_launch * launches / kern_s / 1e9
+        tiles = mfma_tiles(W)
+        lg = n.bit_length() - 1
+        ns, prod_waves = (2 if lg & 1 else 1), 0
+        while ns < n:
+            prod_waves += 3 * (n // 4) * (ns - 1) // ns * (B // 64)
+            ns *= 4
+        mfma_ops = prod_waves * tiles * 4 * 2 * (32 * 32 * 32 * 2)
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": profiled_traffic(chosen, n, P, B),
+                "kernel": PROFILES[chosen][1],
+                "kernel_avg_us": round(kern_s / launches * 1e6, 2),
+                "algorithmic_bytes_per_launch": bytes_launch,
+                "mfma": {"achieved": round(mfma_ops / kern_s / 1e12, 2), "peak": MFMA_I8_PEAK_T,
+                         "unit": "TOP/s (int8)", "frac": round(mfma_ops / kern_s / 1e12
+                                                               / MFMA_I8_PEAK_T, 4)}}
+    else:
+        prod_per_fft = (cmul(n) if chosen == "plain" else fixed_products(n)) * Lc * Lc
+        kern_s = rc["parts_ms"][1 if chosen == "plain" else 0] * 1e-3
+        launches = rc["S"] - (1 if chosen == "plain" else 0)
+        achieved = prod_per_fft * B / kern_s / 1e12
+        peak = VALU_LANE_OPS / 1e12
+        roof = {"bound": "valu", "achieved": round(achieved, 4), "peak": round(peak, 4),
+                "unit": "T(32x32->64 limb products)/s", "frac": round(achieved / peak, 4),
+                "traffic": profiled_traffic(chosen, n, P, B),
+                "kernel": PROFILES[chosen][1],
+                "kernel_avg_us": round(kern_s / launches * 1e6, 2),
+                "algorithmic_products_per_fft": prod_per_fft}
     bytes_fft = (4 * n + 3 * n // 4) * (4 * Lc + 4)
     line = {
         "metric": "mp-FFTs/s at N=2^16 @256/1024 bit, chosen vs plain GPU form at matched ulp error",
@@ -250,12 +310,7 @@ def main():
         "forms": out,
         "chosen": chosen,
         "chosen_vs_plain": round(fft_s / res["plain"]["fft_s"], 4),
-        "roofline": {"bound": "valu", "achieved": round(achieved, 4), "peak": round(peak, 4),
-                     "unit": "T(32x32->64 limb products)/s", "frac": round(achieved / peak, 4),
-                     "traffic": profiled_traffic(chosen, n, P, B),
-                     "kernel": PROFILES[chosen][1],
-                     "kernel_avg_us": round(kern_s / launches * 1e6, 2),
-                     "algorithmic_products_per_fft": prod_per_fft},
+        "roofline": roof,
         "roofline_hbm": {"achieved": round(bytes_fft * fft_s / 1e9, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(bytes_fft * fft_s / 1e9 / HBM_PEAK_GBS, 5),
                          "compulsory_bytes_per_fft": bytes_fft},

@@ -94,7 +94,14 @@ typedef struct mpfft_plan_s *mpfft_plan;
                               same SoA-limb I/O as MPFFT_FORM_MP, results at
                               the plain form's accuracy or better instead of
                               the reference's bit pattern; |input| < 2^(29 -
-                              log2 n), checked by mpfft_plan_check        */
+                              log2 n), checked by mpfft_plan_check.  With
+                              batch a multiple of 64 and n >= 16 the
+                              twiddle products run on the int8 matrix cores
+                              (transforms interleaved in panels); otherwise
+                              as MPFFT_FORM_FIXED_LANE                     */
+#define MPFFT_FORM_FIXED_LANE 9 /* the fixed form with per-lane twiddle
+                              products (29-bit digit multiply-adds) at any
+                              batch; same format and error bound          */
 #define MPFFT_FORM_D5  5   /* double, CODE 5 op order (4m2a, 3 butterflies) */
 #define MPFFT_FORM_D6  6   /* double, CODE 6/7 op order (1 butterfly)      */
 

@@ -16,7 +16,9 @@ LIB_PATH = os.path.join(HERE, "libmpfft.so")
 FORM_MP, FORM_D5, FORM_D6 = 0, 5, 6
 FORM_MP_C1, FORM_MP_C2 = 1, 2          # the reference's CODE 1 / CODE 2 op orders
 FORM_FIXED = 8                          # fixed-point radix-4 form (same limb I/O)
-MP_FORMS = (FORM_MP, FORM_MP_C1, FORM_MP_C2, FORM_FIXED)
+FORM_FIXED_LANE = 9                     # the fixed form on its per-lane kernels only
+FIXED_FORMS = (FORM_FIXED, FORM_FIXED_LANE)
+MP_FORMS = (FORM_MP, FORM_MP_C1, FORM_MP_C2, FORM_FIXED, FORM_FIXED_LANE)
 OK = 0
 ERRORS = {
     -1: "MPFFT_ERR_SIZE", -2: "MPFFT_ERR_PREC", -3: "MPFFT_ERR_VALUE",

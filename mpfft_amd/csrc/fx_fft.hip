@@ -29,6 +29,7 @@
 // its twiddle product, then the quad exchanges values twice (radix 4) or
 // once (radix 2) through DPP.
 #include "fx_arith.h"
+#include "fx_lane.h"
 #include "kernels.h"
 
 #include <utility>
@@ -36,40 +37,6 @@
 namespace mpfft {
 
 namespace {
-
-template <int W>
-__device__ __forceinline__ void fx_dpp(Fx<W> &o, const Fx<W> &x, int ctrl)
-{
-    // quad permutation of every word; ctrl is a compile-time constant at
-    // each call site (0x4E: [2,3,0,1], 0xB1: [1,0,3,2])
-#pragma unroll
-    for (int j = 0; j < W; j++) {
-        o.w[j] = ctrl == 0x4E
-                     ? (uint32_t)__builtin_amdgcn_mov_dpp((int)x.w[j], 0x4E, 0xF, 0xF, false)
-                     : (uint32_t)__builtin_amdgcn_mov_dpp((int)x.w[j], 0xB1, 0xF, 0xF, false);
-    }
-}
-
-template <int W>
-__device__ __forceinline__ void fx_select(Fx<W> &o, uint32_t c, const Fx<W> &a, const Fx<W> &b)
-{
-#pragma unroll
-    for (int j = 0; j < W; j++) o.w[j] = c ? a.w[j] : b.w[j];
-}
-
-// z = A + (-1)^s B with (A, B) = (p, x) when sw, else (x, p)
-template <int W>
-__device__ __forceinline__ void fx_pair(Fx<W> &z, const Fx<W> &x, const Fx<W> &p, uint32_t sw,
-                                        uint32_t s)
-{
-    const uint32_t m = s ? 0xffffffffu : 0u;
-    uint32_t c = s;
-#pragma unroll
-    for (int j = 0; j < W; j++) {
-        const uint32_t a = sw ? p.w[j] : x.w[j], b = sw ? x.w[j] : p.w[j];
-        z.w[j] = add_cc(a, b ^ m, c);
-    }
-}
 
 // LDS parking slot k of this lane: word j at lds[(k W + j) nt + t]
 template <int W>
@@ -218,45 +185,8 @@ fx_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t
         fx_add(im, re, im);
         re = s;
     }
-    // DFT4 over the quad.  Stage 1 (partner r ^ 2): lanes 0, 1 form
-    // t0 = x0 + x2, t2 = x1 + x3; lanes 2, 3 form t1 = x0 - x2, t3 = x1 - x3.
-    {
-        const uint32_t hi = r >> 1;
-        Fx<W> p;
-        fx_dpp(p, re, 0x4E);
-        fx_pair(re, re, p, hi, hi);
-        fx_dpp(p, im, 0x4E);
-        fx_pair(im, im, p, hi, hi);
-    }
-    // Stage 2 (partner r ^ 1): lane 0: y0 = t0 + t2, lane 1: y2 = t0 - t2,
-    // lane 2: y1 = t1 - i t3, lane 3: y3 = t1 + i t3.
-    //   lanes 0, 1: re = own.re +- p.re, im likewise (lane 1: p - own)
-    //   lane 2 (own t1, p t3): re = own.re + p.im, im = own.im - p.re
-    //   lane 3 (own t3, p t1): re = p.re - own.im, im = p.im + own.re
-    {
-        const uint32_t odd = r & 1u, cross = r >> 1;
-        Fx<W> pre, pim, nre;
-        fx_dpp(pre, re, 0xB1);
-        fx_dpp(pim, im, 0xB1);
-        // real part: lanes 0/1 pair re with pre; lane 2 re with pim; lane 3 pre with im
-        {
-            Fx<W> x, y;
-            fx_select(x, cross, odd ? pre : re, re);
-            fx_select(y, cross, odd ? im : pim, pre);
-            // lanes 0, 2: x + y; lanes 1, 3: x - y with (x, y) ordered above
-            fx_pair(nre, x, y, cross ? 0u : odd, odd);
-        }
-        // imaginary part: lanes 0/1 pair im with pim; lane 2 im - pre; lane 3 pim + re
-        {
-            Fx<W> x, y;
-            fx_select(x, cross, odd ? pim : im, im);
-            fx_select(y, cross, odd ? re : pre, pim);
-            fx_pair(im, x, y, cross ? 0u : odd, cross ? (odd ^ 1u) : odd);
-        }
-        re = nre;
-    }
-    const uint32_t rout = (r == 1) ? 2u : (r == 2 ? 1u : r);
-    const uint32_t o = (j - k) * 4 + k + rout * Ns;
+    fx_dft4_quad(re, im, r);
+    const uint32_t o = (j - k) * 4 + k + fx_dft4_rout(r) * Ns;
     fx_pass_store<L, W, FULL, IO>(re, im, a, dst, b, o);
 }
 
@@ -272,17 +202,9 @@ fx_r2_kernel(FxArgs a, const uint32_t *src, uint32_t *dst)
     if (j >= H) return;
     src += a.stride * b;
     dst += a.stride * b;
-    Fx<W> re, im, pre, pim, x, y;
+    Fx<W> re, im;
     fx_pass_load<L, W, IO>(re, im, a, src, b, j + r * H);
-    fx_dpp(pre, re, 0xB1);
-    fx_dpp(pim, im, 0xB1);
-    // lane 0: x0 + x1; lane 1: x0 - x1 (partner - own)
-    fx_select(x, r, pre, re);
-    fx_select(y, r, re, pre);
-    fx_addsub(re, x, y, r);
-    fx_select(x, r, pim, im);
-    fx_select(y, r, im, pim);
-    fx_addsub(im, x, y, r);
+    fx_dft2_pair(re, im, r);
     fx_pass_store<L, W, FULL, IO>(re, im, a, dst, b, 2 * j + r);
 }
 

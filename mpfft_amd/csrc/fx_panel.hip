@@ -1,0 +1,525 @@
+// The fixed form of the FFT on panels of transforms, with the twiddle
+// products on the int8 matrix cores (v_mfma_i32_32x32x32_i8).
+//
+// Same transform and the same fixed-point format as fx_fft.hip (radix-4
+// Stockham passes, radix 2 first when log2 N is odd; W-word two's-complement
+// reals with F = 32 (W - 1) fraction bits); what changes is where a product
+// is computed.  In the per-lane kernels every lane multiplies its own operand
+// by its own twiddle with 29-bit digit products (v_mad_u64_u32); here a wave
+// holds the same element of 64 transforms, so the 64 operands of a wave
+// share one twiddle, and the product becomes a matrix product:
+//
+//   x = sum_i X_i 256^i, t = sum_j T_j 256^j   (balanced digits in [-128, 127])
+//   column c of x t:  S_c = sum_i T_(c-i) X_i = (Toeplitz(T) X)_c
+//
+// With the columns c in tiles of 32 (rows of an MFMA) and the operand digits
+// i in tiles of 32 (its reduction axis), tile (p, q) is a 32 x 32 Toeplitz
+// block of T that depends only on the diagonal p - q; the 32 MFMA columns are
+// 32 transforms.  Column sums stay below 2^23 in int32, and a carry pass
+// (7 VALU operations per output word) turns the columns at and above
+// 2^(F - 32) into the W result words: the result is floor(x t / 2^F) up to
+// the dropped columns, whose sum is below 2^(F - 17).
+//
+// A complex product (c + i d)(re + i im) takes two accumulating products per
+// part: re' = c re + (-d) im, im' = d re + c im, so the twiddle table holds
+// c, d and -d for each entry.
+//
+// Pass kernel: one workgroup = one j and 64 transforms; wave w holds
+// v_r = x[j + r N/4] with r = (w + block) mod 4 (rotated so that the waves
+// without products, r = 0, spread over the SIMDs).  After the products the
+// four waves exchange through LDS and each forms one output of the DFT4.
+#include "fx_arith.h"
+#include "fx_lane.h"
+#include "kernels.h"
+
+// Compile-time tuning of the pass kernel: FXP_OCC blocks per CU (persistent
+// grid), FXP_SETS LDS slot sets (2: alternate between items, one barrier per
+// item; needs one block per CU), FXP_ACACHE 1 keeps a product's A tiles in
+// registers (spills at two blocks per CU at W = 34).
+#ifndef FXP_OCC
+#define FXP_OCC 2
+#endif
+#ifndef FXP_SETS
+#define FXP_SETS 1
+#endif
+#ifndef FXP_ACACHE
+#define FXP_ACACHE 0
+#endif
+
+namespace mpfft {
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// XCD-aware block order: consecutive logical blocks run on the same XCD
+// (blocks are dealt to the 8 XCDs round robin), so blocks that share
+// cache lines share an L2
+__device__ __forceinline__ uint32_t xcd_block()
+{
+    const uint32_t nb = gridDim.x, bid = blockIdx.x;
+    return (nb & 7u) == 0 ? (bid & 7u) * (nb >> 3) + (bid >> 3) : bid;
+}
+
+// word w of transform b at element e in a panel buffer: pan_base + w kPanel
+__device__ __forceinline__ size_t pan_base(uint32_t b, uint32_t e, uint32_t N, int W)
+{
+    return ((size_t)(b / kPanel) * (2 * (size_t)N) + e) * (size_t)W * kPanel + (b % kPanel);
+}
+
+template <int W>
+__device__ __forceinline__ void pan_load(Fx<W> &re, Fx<W> &im, const uint32_t *buf, uint32_t b,
+                                         uint32_t e, uint32_t N)
+{
+    const uint32_t *p = buf + pan_base(b, e, N, W);
+#pragma unroll
+    for (int j = 0; j < W; j++) re.w[j] = p[j * kPanel];
+    p += (size_t)N * W * kPanel;
+#pragma unroll
+    for (int j = 0; j < W; j++) im.w[j] = p[j * kPanel];
+}
+
+template <int W>
+__device__ __forceinline__ void pan_store(const Fx<W> &re, const Fx<W> &im, uint32_t *buf,
+                                          uint32_t b, uint32_t e, uint32_t N)
+{
+    uint32_t *p = buf + pan_base(b, e, N, W);
+#pragma unroll
+    for (int j = 0; j < W; j++) p[j * kPanel] = re.w[j];
+    p += (size_t)N * W * kPanel;
+#pragma unroll
+    for (int j = 0; j < W; j++) p[j * kPanel] = im.w[j];
+}
+
+// The same element e of the 64 transforms of group grp (one panel): lane l
+// holds transform 64 grp + l.  The element's words are one contiguous block
+// of W x 256 bytes; every address is a wave-uniform base plus 4 lane bytes
+// plus an immediate word offset.
+template <int W>
+__device__ __forceinline__ void row_load(Fx<W> &re, Fx<W> &im, const uint32_t *buf, uint32_t grp,
+                                         uint32_t e, uint32_t N, uint32_t lane)
+{
+    const uint32_t *p = buf + ((size_t)grp * (2 * (size_t)N) + e) * (size_t)W * kPanel;
+#pragma unroll
+    for (int j = 0; j < W; j++) re.w[j] = (p + j * kPanel)[lane];
+    p += (size_t)N * W * kPanel;
+#pragma unroll
+    for (int j = 0; j < W; j++) im.w[j] = (p + j * kPanel)[lane];
+}
+
+template <int W>
+__device__ __forceinline__ void row_store(const Fx<W> &re, const Fx<W> &im, uint32_t *buf,
+                                          uint32_t grp, uint32_t e, uint32_t N, uint32_t lane)
+{
+    uint32_t *p = buf + ((size_t)grp * (2 * (size_t)N) + e) * (size_t)W * kPanel;
+#pragma unroll
+    for (int j = 0; j < W; j++) (p + j * kPanel)[lane] = re.w[j];
+    p += (size_t)N * W * kPanel;
+#pragma unroll
+    for (int j = 0; j < W; j++) (p + j * kPanel)[lane] = im.w[j];
+}
+
+// Output R of the DFT4 over the four slots: with a = v0 +- v2, c = v1 +- v3
+// (+ for R even), y0 = a + c, y2 = a - c, y1 = a - i c, y3 = a + i c.
+// vs[s] points at lane's word 0 of v_s in the LDS slots.
+template <bool SUB>
+__device__ __forceinline__ uint32_t addsub_c(uint32_t a, uint32_t b, uint32_t &c)
+{
+    return SUB ? sub_bb(a, b, c) : add_cc(a, b, c);
+}
+
+template <int W, int R>
+__device__ __forceinline__ void dft4_out(Fx<W> &yr, Fx<W> &yi, const uint32_t *const (&vs)[4])
+{
+    constexpr bool odd = R & 1;
+    constexpr bool sre = (R == 2 || R == 3), sim = (R == 1 || R == 2);
+    uint32_t ca_r = 0, ca_i = 0, cc_r = 0, cc_i = 0, cy_r = 0, cy_i = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        const uint32_t ar = addsub_c<odd>(vs[0][i * 64], vs[2][i * 64], ca_r);
+        const uint32_t ai = addsub_c<odd>(vs[0][(W + i) * 64], vs[2][(W + i) * 64], ca_i);
+        const uint32_t cr = addsub_c<odd>(vs[1][i * 64], vs[3][i * 64], cc_r);
+        const uint32_t ci = addsub_c<odd>(vs[1][(W + i) * 64], vs[3][(W + i) * 64], cc_i);
+        yr.w[i] = addsub_c<sre>(ar, odd ? ci : cr, cy_r);
+        yi.w[i] = addsub_c<sim>(ai, odd ? cr : ci, cy_i);
+    }
+}
+
+// Balanced base-256 digits of x as MFMA B operands.  Digit i of x is byte i
+// of (x + 0x8080...80) minus 128, i.e. that byte XOR 0x80 read as int8.
+// K tile q covers digits [32 q, 32 q + 32) = words [8 q, 8 q + 8); in an MFMA
+// the lanes n and n + 32 supply column n (digits 16 g + [0, 16) from lane
+// n + 32 g), so group 0 (the operands of lanes 0..31) needs words 8q + 4..7
+// of lane n in lane n + 32, and group 1 words 8q..8q+3 of lane n + 32 in lane
+// n: one permlane32 swap per word pair.
+template <int W, int KT>
+__device__ __forceinline__ void mf_bdigits(const Fx<W> &x, v4i (&B0)[KT], v4i (&B1)[KT])
+{
+    uint32_t d[8 * KT];
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 8 * KT; i++)
+        d[i] = i < W ? (add_cc(x.w[i], 0x80808080u, c) ^ 0x80808080u) : 0u;
+#pragma unroll
+    for (int q = 0; q < KT; q++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (8 * q + i >= W) {            // both halves zero
+                B0[q][i] = 0;
+                B1[q][i] = 0;
+                continue;
+            }
+            const auto s = __builtin_amdgcn_permlane32_swap(d[8 * q + i], d[8 * q + 4 + i], false,
+                                                            false);
+            B0[q][i] = (int)s[0];
+            B1[q][i] = (int)s[1];
+        }
+    }
+}
+
+// A operand of diagonal tile delta: lane (m = lane % 32, g = lane / 32) holds
+// T[C0 + 32 delta + m - 16 g - b] in byte b, read from the twiddle real's
+// digit table R (byte u = digit U - u) at byte offset off.
+__device__ __forceinline__ v4i mf_atile(const uint32_t *R, uint32_t off)
+{
+    const uint32_t b4 = off >> 2, sh = off & 3u;
+    const uint32_t d0 = R[b4], d1 = R[b4 + 1], d2 = R[b4 + 2], d3 = R[b4 + 3], d4 = R[b4 + 4];
+    v4i a;
+    a[0] = (int)__builtin_amdgcn_alignbyte(d1, d0, sh);
+    a[1] = (int)__builtin_amdgcn_alignbyte(d2, d1, sh);
+    a[2] = (int)__builtin_amdgcn_alignbyte(d3, d2, sh);
+    a[3] = (int)__builtin_amdgcn_alignbyte(d4, d3, sh);
+    return a;
+}
+
+// The B operands of a complex operand (re, im): its digit tiles for both
+// 32-lane groups.
+template <int W>
+struct MfOperand {
+    static constexpr int KT = mf_geom(W).KT;
+    v4i r0[KT], r1[KT], i0[KT], i1[KT];
+};
+
+template <int W>
+__device__ __forceinline__ void mf_operand(MfOperand<W> &B, const Fx<W> &re, const Fx<W> &im)
+{
+    mf_bdigits<W, MfOperand<W>::KT>(re, B.r0, B.r1);
+    mf_bdigits<W, MfOperand<W>::KT>(im, B.i0, B.i1);
+}
+
+// (re, im) *= (c + i d) for the 64 operands of the wave (one per lane), given
+// as B operands; the wave's twiddle reals c, d, -d are digit tables in LDS.
+// Results go to the wave's LDS slot: word j of re' at slot[j 64 + lane], of
+// im' at slot[(W + j) 64 + lane].  The twiddle tables may lie in the im' part
+// of the slot: all of them are read before the first word of im' is written.
+template <int W>
+__device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *Rc,
+                                        const uint32_t *Rd, const uint32_t *Rnd, uint32_t lane,
+                                        uint32_t *slot)
+{
+    constexpr MfGeom G = mf_geom(W);
+    constexpr int KT = G.KT, MT = G.MT, ND = G.DHI - G.DLO + 1;
+    const uint32_t m = lane & 31u, g = lane >> 5;
+    // byte offset of diagonal delta: U - (C0 + 32 delta + m - 16 g)
+    const uint32_t off0 = (uint32_t)(G.U - G.C0 - 32 * G.DLO) - m + 16 * g;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        // h = 0: re' = c re + (-d) im;  h = 1: im' = d re + c im
+        const uint32_t *R1 = h ? Rd : Rc;
+        const uint32_t *R2 = h ? Rc : Rnd;
+#if FXP_ACACHE
+        v4i A1[ND], A2[ND];
+#pragma unroll
+        for (int dd = 0; dd < ND; dd++) {
+            A1[dd] = mf_atile(R1, off0 - 32 * dd);
+            A2[dd] = mf_atile(R2, off0 - 32 * dd);
+        }
+#endif
+        int32_t carry = 0;
+        uint32_t *out = slot + (uint32_t)h * W * 64 + lane;
+#pragma unroll
+        for (int p = 0; p < MT; p++) {
+            v16i D0 = {}, D1 = {};
+#pragma unroll
+            for (int q = 0; q < KT; q++) {
+                if (!mf_live(W, p, q)) continue;
+                const int dd = p - q - G.DLO;
+#if FXP_ACACHE
+                const v4i a1 = A1[dd], a2 = A2[dd];
+#else
+                const v4i a1 = mf_atile(R1, off0 - 32 * dd), a2 = mf_atile(R2, off0 - 32 * dd);
+#endif
+                D0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, B.r0[q], D0, 0, 0, 0);
+                D1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, B.r1[q], D1, 0, 0, 0);
+                D0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a2, B.i0[q], D0, 0, 0, 0);
+                D1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a2, B.i1[q], D1, 0, 0, 0);
+            }
+            // D0 lane l: rows 8 (v/4) + 4 (l/32) + v%4 of operand l % 32,
+            // D1 the same of operand l % 32 + 32.  After one swap per v the
+            // lane holds rows 8 (v/4) + v%4 (S0) and those + 4 (S1) of its
+            // own operand.
+            int32_t S0[16], S1[16];
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int row = 8 * (v / 4) + v % 4;
+                if (32 * p + row >= G.NCOL) { S0[v] = 0; S1[v] = 0; continue; }
+                const auto s = __builtin_amdgcn_permlane32_swap((uint32_t)D0[v], (uint32_t)D1[v],
+                                                                false, false);
+                S0[v] = (int32_t)s[0];
+                S1[v] = (int32_t)s[1];
+            }
+            // rows 4u..4u+3 = output word t = 8 p + u - 1 (t = -1: the four
+            // columns below 2^-F, which only carry)
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int t = 8 * p + u - 1;
+                if (t >= W) break;
+                const int32_t *S = (u & 1) ? &S1[4 * (u >> 1)] : &S0[4 * (u >> 1)];
+                // V = S0 + 2^8 S1 + 2^16 S2 + 2^24 S3 + carry = P0 + 2^16 P1
+                const int32_t P0 = (int32_t)((uint32_t)S[0] + ((uint32_t)S[1] << 8)) + carry;
+                const int32_t P1 = (int32_t)((uint32_t)S[2] + ((uint32_t)S[3] << 8));
+                if (t >= 0) out[(uint32_t)t * 64] = (uint32_t)P0 + ((uint32_t)P1 << 16);
+                carry = (P1 + (P0 >> 16)) >> 16;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// Input conversion: float SoA -> fixed point in the panel layout, through an
+// LDS transpose so that both sides move 64-byte runs.  A block covers 16
+// elements (of the 2N re/im positions) of 16 transforms; blocks are ordered
+// transform group fastest, so the four blocks that fill one 256-byte panel
+// row run next to each other on one XCD.
+template <int L, int W>
+__global__ void __launch_bounds__(256)
+fxp_in_kernel(FxArgs a, uint32_t *dst)
+{
+    __shared__ uint32_t tile[W * 16 * 17];
+    const uint32_t N = a.N, M = 2 * N, t = threadIdx.x;
+    const uint32_t lb = xcd_block(), ng = a.batch / 16;
+    const uint32_t g16 = lb % ng, eb = lb / ng;
+    {
+        // lane = (element t % 16, transform t / 16): 64-byte runs of the input
+        const uint32_t el = t & 15u, bl = t >> 4;
+        Mp<L> x;
+        mp_load(x, a.in + a.in_stride * (g16 * 16 + bl), M, eb * 16 + el);
+        Fx<W> z;
+        if (fx_from_mp<L, W>(z, x, a.lim)) atomicOr(a.flag, 1u);
+#pragma unroll
+        for (int w = 0; w < W; w++) tile[(w * 16 + el) * 17 + bl] = z.w[w];
+    }
+    __syncthreads();
+    // lane = (transform t % 16, element t / 16): 64-byte runs of a panel row
+    const uint32_t bl = t & 15u, el = t >> 4;
+    uint32_t *p = dst + pan_base(g16 * 16 + bl, eb * 16 + el, N, W);
+#pragma unroll
+    for (int w = 0; w < W; w++) p[w * kPanel] = tile[(w * 16 + el) * 17 + bl];
+}
+
+// A radix-2 pass without twiddles (the first pass when log2 N is odd): a
+// lane holds the same j of one transform for both halves.
+template <int W>
+__global__ void __launch_bounds__(256)
+fxp_r2_kernel(FxArgs a, const uint32_t *src, uint32_t *dst)
+{
+    const uint32_t N = a.N, H = N >> 1;
+    const uint32_t wid = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    const uint32_t j = wid % H, grp = wid / H, b = grp * 64 + lane;
+    Fx<W> r0, i0, r1, i1, yr, yi;
+    pan_load(r0, i0, src, b, j, N);
+    pan_load(r1, i1, src, b, j + H, N);
+    fx_add(yr, r0, r1);
+    fx_add(yi, i0, i1);
+    pan_store(yr, yi, dst, b, 2 * j, N);
+    fx_sub(yr, r0, r1);
+    fx_sub(yi, i0, i1);
+    pan_store(yr, yi, dst, b, 2 * j + 1, N);
+}
+
+// A radix-4 pass with twiddles, panel layout in and out (see the top of the
+// file).  Persistent: a block runs items it = block, block + grid, ...
+// (item = one j and 64 transforms) and loads the next item's operands and
+// twiddle while it multiplies the current one, so the HBM traffic of a pass
+// overlaps its matrix-core and VALU work even at one wave per SIMD.  LDS:
+// two sets (alternating items) of four slots of 2 W words x 64 lanes; one
+// barrier per item.
+template <int W>
+__global__ void __launch_bounds__(256, FXP_OCC)
+fxp_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t toff,
+              uint32_t nitems)
+{
+    extern __shared__ uint32_t lds[];
+    constexpr MfGeom G = mf_geom(W);
+    constexpr uint32_t SLOT = 2 * W * 64, TWN = 3 * G.RW, TWL = (TWN + 63) / 64;
+    const uint32_t N = a.N, Q = N >> 2;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    uint32_t it = blockIdx.x;
+    if (it >= nitems) return;
+    // operands and twiddle words of item it (prefetched one item ahead)
+    auto item = [&](uint32_t i, uint32_t &j, uint32_t &grp, uint32_t &rot, uint32_t &r,
+                    uint32_t &k) {
+        j = i % Q; grp = i / Q; rot = i & 3u; r = (wv + rot) & 3u; k = j & (Ns - 1);
+    };
+    auto fetch = [&](uint32_t i, Fx<W> &re, Fx<W> &im, uint32_t (&tw)[TWL]) {
+        uint32_t j, grp, rot, r, k;
+        item(i, j, grp, rot, r, k);
+        if (r != 0 && k != 0) {
+            const uint32_t *t = a.tw + (size_t)(toff + 3 * k + r - 1) * TWN;
+#pragma unroll
+            for (uint32_t q = 0; q < TWL; q++)
+                tw[q] = (lane + 64 * q < TWN) ? (t + 64 * q)[lane] : 0u;
+        }
+        row_load(re, im, src, grp, j + r * Q, N, lane);
+    };
+    Fx<W> re, im;
+    uint32_t tw[TWL];
+    fetch(it, re, im, tw);
+    for (uint32_t n = 0;; n++) {
+        uint32_t j, grp, rot, r, k;
+        item(it, j, grp, rot, r, k);
+        const uint32_t nx = it + gridDim.x;
+        uint32_t *set = lds + (FXP_SETS == 2 ? (n & 1u) : 0u) * 4 * SLOT;
+        if (FXP_SETS == 1 && n > 0) __syncthreads();   // the slots of the last item are read
+        uint32_t *slot = set + wv * SLOT;
+        uint32_t *R = slot + SLOT - TWN;          // twiddle tables at the end of the slot
+        const bool prod = r != 0 && k != 0;
+        // consume this item's operands before the next item's loads are
+        // issued: loads and stores share one in-order counter (at most 63
+        // outstanding), so a wait for these operands placed after the next
+        // item's loads would wait for those too
+        MfOperand<W> B;
+        if (prod) {
+#pragma unroll
+            for (uint32_t q = 0; q < TWL; q++)
+                if (lane + 64 * q < TWN) R[lane + 64 * q] = tw[q];
+            mf_operand<W>(B, re, im);
+        } else {
+#pragma unroll
+            for (int i = 0; i < W; i++) slot[i * 64 + lane] = re.w[i];
+#pragma unroll
+            for (int i = 0; i < W; i++) slot[(W + i) * 64 + lane] = im.w[i];
+        }
+        if (nx < nitems) fetch(nx, re, im, tw);
+        if (prod) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            mf_cmul<W>(B, R, R + G.RW, R + 2 * G.RW, lane, slot);
+        }
+        __syncthreads();
+        // DFT4, output y_r of this wave; v_s sits in the slot of wave
+        // (s - rot) mod 4
+        const uint32_t *const vs[4] = {set + ((0u - rot) & 3u) * SLOT + lane,
+                                       set + ((1u - rot) & 3u) * SLOT + lane,
+                                       set + ((2u - rot) & 3u) * SLOT + lane,
+                                       set + ((3u - rot) & 3u) * SLOT + lane};
+        Fx<W> yr, yi;
+        switch (r) {
+        case 0: dft4_out<W, 0>(yr, yi, vs); break;
+        case 1: dft4_out<W, 1>(yr, yi, vs); break;
+        case 2: dft4_out<W, 2>(yr, yi, vs); break;
+        default: dft4_out<W, 3>(yr, yi, vs); break;
+        }
+        row_store(yr, yi, dst, grp, (j - k) * 4 + k + r * Ns, N, lane);
+        if (nx >= nitems) break;
+        it = nx;
+    }
+}
+
+// Output conversion: panel layout -> float SoA rounded to P bits, through
+// an LDS transpose (64-byte runs on both sides; block order as fxp_in_kernel).
+template <int W, int L, bool FULL>
+__global__ void __launch_bounds__(256)
+fxp_out_kernel(FxArgs a, const uint32_t *src)
+{
+    __shared__ uint32_t tile[(L + 1) * 16 * 17];
+    const uint32_t N = a.N, M = 2 * N, t = threadIdx.x;
+    const uint32_t lb = xcd_block(), ng = a.batch / 16;
+    const uint32_t g16 = lb % ng, eb = lb / ng;
+    {
+        // lane = (transform t % 16, element t / 16): 64-byte runs of a panel row
+        const uint32_t bl = t & 15u, el = t >> 4;
+        const uint32_t *p = src + pan_base(g16 * 16 + bl, eb * 16 + el, N, W);
+        Fx<W> x;
+#pragma unroll
+        for (int w = 0; w < W; w++) x.w[w] = p[w * kPanel];
+        Mp<L> z;
+        fx_to_mp<W, L, FULL>(z, x, a.P);
+        tile[el * 17 + bl] = z.zero ? ((uint32_t)kHdrZero | z.neg) : (((uint32_t)z.e << 1) | z.neg);
+#pragma unroll
+        for (int j = 0; j < L; j++) tile[((j + 1) * 16 + el) * 17 + bl] = z.zero ? 0u : z.m[j];
+    }
+    __syncthreads();
+    // lane = (element t % 16, transform t / 16): 64-byte runs of the output
+    const uint32_t el = t & 15u, bl = t >> 4;
+    uint32_t *o = a.out + a.out_stride * (g16 * 16 + bl) + eb * 16 + el;
+#pragma unroll
+    for (int j = 0; j <= L; j++) o[(size_t)j * M] = tile[(j * 16 + el) * 17 + bl];
+}
+
+// ---------------------------------------------------------------------------
+
+template <int L, int W, bool FULL>
+static hipError_t launch_fxp(const FxArgs &a, int s0, int s1, hipStream_t st)
+{
+    const uint32_t N = a.N;
+    uint32_t lg = 0;
+    while ((1u << lg) < N) lg++;
+    auto in_range = [&](int s) { return s >= s0 && s < s1; };
+    uint32_t *src = a.buf1, *dst = a.buf0;
+    const uint32_t blocks16 = (2 * N / 16) * (a.batch / 16);
+    // stage 0: input conversion into buf1
+    if (in_range(0))
+        hipLaunchKernelGGL((fxp_in_kernel<L, W>), dim3(blocks16), dim3(256), 0, st, a, src);
+    int stage = 1;
+    uint32_t Ns = 1, toff = 0;
+    if (lg & 1) {
+        if (in_range(stage))
+            hipLaunchKernelGGL((fxp_r2_kernel<W>), dim3((N / 2) * (a.batch / 64) / 4), dim3(256),
+                               0, st, a, (const uint32_t *)src, dst);
+        std::swap(src, dst);
+        stage++;
+        Ns = 2;
+    }
+    const size_t sh = sizeof(uint32_t) * FXP_SETS * 4 * 2 * W * 64;   // sets of four slots
+    const uint32_t nitems = (N / 4) * (a.batch / 64);
+    const uint32_t nblk = nitems < FXP_OCC * a.ncu ? nitems : FXP_OCC * a.ncu;
+    for (; Ns < N; toff += 3 * Ns, Ns *= 4, stage++) {
+        if (in_range(stage))
+            hipLaunchKernelGGL((fxp_r4_kernel<W>), dim3(nblk), dim3(256), sh, st, a,
+                               (const uint32_t *)src, dst, Ns, toff, nitems);
+        std::swap(src, dst);
+    }
+    // the last stage: output conversion
+    if (in_range(stage))
+        hipLaunchKernelGGL((fxp_out_kernel<W, L, FULL>), dim3(blocks16), dim3(256), 0, st, a,
+                           (const uint32_t *)src);
+    return hipGetLastError();
+}
+
+hipError_t fxp_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st)
+{
+    if (!fxp_eligible(a.N, a.batch)) return hipErrorInvalidValue;
+    const bool full = (P == 32 * L);
+#define CALLP(Lv) (full ? launch_fxp<Lv, Lv + 2, true>(a, s0, s1, st) \
+                        : launch_fxp<Lv, Lv + 2, false>(a, s0, s1, st))
+    switch (L) {
+    case 1: return CALLP(1);
+    case 2: return CALLP(2);
+    case 3: return CALLP(3);
+    case 4: return CALLP(4);
+    case 6: return CALLP(6);
+    case 8: return CALLP(8);
+    case 12: return CALLP(12);
+    case 16: return CALLP(16);
+    case 24: return CALLP(24);
+    case 32: return CALLP(32);
+    default: return hipErrorInvalidValue;
+    }
+#undef CALLP
+}
+
+}  // namespace mpfft

@@ -77,6 +77,7 @@ struct FxArgs {
     uint32_t N, batch;
     int lim;                    // inputs must have exponent <= lim (|x| < 2^lim)
     int P;
+    uint32_t ncu;               // compute units of the device (persistent grids)
 };
 
 // fixed-point words per real for limb count L (F = 32 (L + 1) fraction bits)
@@ -96,6 +97,74 @@ inline int fx_stages(uint32_t N) { return fx_passes(N) > 0 ? fx_passes(N) : 2; }
 
 // stages [s0, s1) of the fixed form
 hipError_t fx_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st);
+
+// ---- fixed form, panel kernels (fx_panel.hip) ----
+//
+// Transforms are interleaved in panels of kPanel: word w of element e of
+// transform b sits at ((b / 64) 2N + e) W 64 + 64 w + b % 64, so one element
+// of 64 transforms is a contiguous block of W x 256 bytes.  A twiddle
+// product runs on the int8 matrix cores: the twiddle t (|t| <= 1) and the
+// operand x are written in balanced base-256 digits (each in [-128, 127]),
+// and the columns of x t are a Toeplitz matrix of t's digits times the
+// digit vectors of 32 operands that share t (the same element of 32
+// transforms), one v_mfma_i32_32x32x32_i8 per 32 x 32 tile.
+
+constexpr uint32_t kPanel = 64;        // transforms per panel
+constexpr uint32_t kPanelBatch = 64;   // the panel kernels need batch % 64 == 0
+
+struct MfGeom {
+    int NXB;       // digits of an operand: 4 W
+    int NTB;       // digits of a twiddle real: 4 W - 3 (|t| <= 1 at F bits)
+    int C0;        // lowest product column kept: F / 8 - 4
+    int NCOL;      // columns kept: C0 .. C0 + NCOL - 1 (4 below 2^-F, 4 W above)
+    int KT, MT;    // 32-wide tiles over the operand digits / the kept columns
+    int DLO, DHI;  // tile diagonals p - q with a nonzero twiddle digit
+    int U;         // digit table of a twiddle real: byte u holds digit U - u
+    int RW;        // 32-bit words of that table (with 4 bytes of over-read)
+};
+
+// tile (p, q) holds some twiddle digit T[C0 + 32 (p - q) + m - k], m < 32
+// rows (kept columns), k < 32 operand digits
+constexpr bool mf_live(int W, int p, int q)
+{
+    const int NXB = 4 * W, NTB = 4 * W - 3, C0 = 4 * W - 8, NCOL = 4 * W + 4;
+    const int mmax = (NCOL - 32 * p < 32 ? NCOL - 32 * p : 32) - 1;
+    const int kmax = (NXB - 32 * q < 32 ? NXB - 32 * q : 32) - 1;
+    const int lo = C0 + 32 * (p - q) - kmax, hi = C0 + 32 * (p - q) + mmax;
+    return mmax >= 0 && kmax >= 0 && hi >= 0 && lo <= NTB - 1;
+}
+
+constexpr MfGeom mf_geom(int W)
+{
+    MfGeom g{};
+    g.NXB = 4 * W;
+    g.NTB = 4 * W - 3;
+    g.C0 = 4 * W - 8;
+    g.NCOL = 4 * W + 4;
+    g.KT = (g.NXB + 31) / 32;
+    g.MT = (g.NCOL + 31) / 32;
+    g.DLO = 1 << 20;
+    g.DHI = -(1 << 20);
+    for (int p = 0; p < g.MT; p++)
+        for (int q = 0; q < g.KT; q++)
+            if (mf_live(W, p, q)) {
+                if (p - q < g.DLO) g.DLO = p - q;
+                if (p - q > g.DHI) g.DHI = p - q;
+            }
+    g.U = g.C0 + 32 * g.DHI + 31;
+    g.RW = (32 * (g.DHI - g.DLO) + 67 + 3) / 4;
+    return g;
+}
+
+// the panel kernels apply: batch a multiple of 64, N >= 16
+inline bool fxp_eligible(uint32_t N, uint32_t batch)
+{
+    return N >= 16 && batch > 0 && batch % kPanelBatch == 0;
+}
+
+// stages [s0, s1) of the fixed form on panels: the input conversion, one
+// stage per pass, the output conversion (rounding back to floats)
+hipError_t fxp_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st);
 
 // ---- double path (d_fft.hip) ----
 

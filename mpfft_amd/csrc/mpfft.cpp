@@ -45,8 +45,10 @@ bool is_mp(int form)
     return form == MPFFT_FORM_MP || form == MPFFT_FORM_MP_C1 || form == MPFFT_FORM_MP_C2;
 }
 
+bool is_fixed(int form) { return form == MPFFT_FORM_FIXED || form == MPFFT_FORM_FIXED_LANE; }
+
 // forms whose device data use the SoA-limb layout of include/mpfft.h
-bool is_limb_form(int form) { return is_mp(form) || form == MPFFT_FORM_FIXED; }
+bool is_limb_form(int form) { return is_mp(form) || is_fixed(form); }
 
 // the reference's operation order of a multiprecision form: CODE 3
 // (serial_mp_fft3.c), CODE 1 (serial_mp_fft1.c) or CODE 2 (serial_mp_fft2.c)
@@ -232,6 +234,8 @@ struct mpfft_plan_s {
     DevBuf fx_tw, fx_buf0, fx_buf1, fx_flag;
     int fx_lim = 0;
     uint32_t fx_ntw = 0;
+    bool fx_panel = false;   // panel kernels (matrix-core twiddle products)
+    uint32_t ncu = 0;        // compute units of the plan's device
     // double forms
     DevBuf tw, dleaf_sn, dleaf_off, dleaf_n, dlvl_off, dlvl_begin;
     DTables dt{};
@@ -325,10 +329,14 @@ int build_mp_plan(mpfft_plan_s *p)
 
 // Fixed form: for every radix-4 pass (Ns = 1 or 2, then times 4 while
 // Ns < N), entries 3k + r - 1 (k < Ns, r = 1..3) hold w^t = c + i d =
-// exp(-2 pi i t / N), t = r k N / (4 Ns), as c, d - c and c + d rounded to
-// F = 32 (L + 1) fraction bits, each stored as DT 29-bit digits of its
-// magnitude (least significant first) and a sign word; SoA over the 3 T3
-// reals.
+// exp(-2 pi i t / N), t = r k N / (4 Ns), rounded to F = 32 (L + 1)
+// fraction bits (MPFR at F + 64 bits, then one rounding to nearest).
+//   per-lane kernels: c, d - c and c + d, each as DT 29-bit digits of its
+//     magnitude (least significant first) and a sign word; SoA over the
+//     3 T3 reals;
+//   panel kernels: c, d and -d per entry (entry-major), each as the byte
+//     table of its balanced base-256 digits that the matrix-core product
+//     reads (byte u = digit U - u, zero outside the digits; mf_geom).
 int build_fx_plan(mpfft_plan_s *p)
 {
     const unsigned long N = p->N;
@@ -336,12 +344,18 @@ int build_fx_plan(mpfft_plan_s *p)
     const long DT = (F + 1 + 28) / 29;
     int lg0 = 0;
     while ((1UL << lg0) < N) lg0++;
+    p->fx_panel = p->form == MPFFT_FORM_FIXED && fxp_eligible((uint32_t)N, p->batch);
+    int ncu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device));
+    p->ncu = ncu > 0 ? (uint32_t)ncu : 1u;
+    const MfGeom G = mf_geom((int)W);
     std::vector<unsigned long> ent;               // t of every entry, in table order
     for (unsigned long Ns = (lg0 & 1) ? 2 : 1; Ns < N; Ns *= 4)
         for (unsigned long k = 0; k < Ns; k++)
             for (unsigned long r = 1; r <= 3; r++) ent.push_back(r * k * (N / (4 * Ns)));
     const size_t T3 = ent.size(), MT = 3 * T3;
-    std::vector<uint32_t> tab(MT > 0 ? MT * (DT + 1) : 1, 0);
+    const size_t tabn = p->fx_panel ? T3 * 3 * (size_t)G.RW : MT * (DT + 1);
+    std::vector<uint32_t> tab(tabn > 0 ? tabn : 1, 0);
     unsigned nt = std::thread::hardware_concurrency();
     nt = std::max(1u, std::min(nt, 16u));
     if (T3 < 256) nt = 1;
@@ -351,6 +365,7 @@ int build_fx_plan(mpfft_plan_s *p)
         mpz_t z;
         mpz_init(z);
         std::vector<uint32_t> words((size_t)(F / 32 + 4));
+        std::vector<uint8_t> bytes((size_t)G.NTB + 8), R((size_t)G.RW * 4);
         for (size_t e = t0; e < t1; e++) {
             const size_t t = ent[e];
             mpfr_const_pi(th, MPFR_RNDN);
@@ -359,11 +374,51 @@ int build_fx_plan(mpfft_plan_s *p)
             mpfr_sin_cos(d, c, th, MPFR_RNDN);
             mpfr_neg(d, d, MPFR_RNDN);                    // w = cos - i sin
             for (int u = 0; u < 3; u++) {
-                if (u == 0) mpfr_set(v, c, MPFR_RNDN);
-                else if (u == 1) mpfr_sub(v, d, c, MPFR_RNDN);
-                else mpfr_add(v, c, d, MPFR_RNDN);
+                if (p->fx_panel) {
+                    if (u == 0) mpfr_set(v, c, MPFR_RNDN);
+                    else if (u == 1) mpfr_set(v, d, MPFR_RNDN);
+                    else mpfr_neg(v, d, MPFR_RNDN);
+                } else {
+                    if (u == 0) mpfr_set(v, c, MPFR_RNDN);
+                    else if (u == 1) mpfr_sub(v, d, c, MPFR_RNDN);
+                    else mpfr_add(v, c, d, MPFR_RNDN);
+                }
                 mpfr_mul_2si(v, v, F, MPFR_RNDN);
                 mpfr_get_z(z, v, MPFR_RNDN);
+                if (p->fx_panel) {
+                    // two's complement over NTB bytes, then balanced digits:
+                    // bytes of (T + 0x80..80) minus 128 (|T| <= 2^F, so the
+                    // sum stays in [0, 256^NTB))
+                    const bool neg = mpz_sgn(z) < 0;
+                    mpz_abs(z, z);
+                    std::fill(bytes.begin(), bytes.end(), 0);
+                    size_t cnt = 0;
+                    mpz_export(bytes.data(), &cnt, -1, 1, 0, 0, z);
+                    if (neg) {
+                        unsigned cy = 1;
+                        for (int i = 0; i < G.NTB; i++) {
+                            const unsigned x = (unsigned)(uint8_t)~bytes[i] + cy;
+                            bytes[i] = (uint8_t)x;
+                            cy = x >> 8;
+                        }
+                    }
+                    unsigned cy = 0;
+                    for (int i = 0; i < G.NTB; i++) {
+                        const unsigned x = (unsigned)bytes[i] + 0x80u + cy;
+                        bytes[i] = (uint8_t)((x & 0xffu) ^ 0x80u);   // digit as int8
+                        cy = x >> 8;
+                    }
+                    std::fill(R.begin(), R.end(), 0);
+                    for (int b = 0; b < 4 * G.RW; b++) {
+                        const int idx = G.U - b;
+                        if (idx >= 0 && idx < G.NTB) R[b] = bytes[idx];
+                    }
+                    uint32_t *dst = tab.data() + (e * 3 + u) * (size_t)G.RW;
+                    for (int w = 0; w < G.RW; w++)
+                        dst[w] = (uint32_t)R[4 * w] | (uint32_t)R[4 * w + 1] << 8 |
+                                 (uint32_t)R[4 * w + 2] << 16 | (uint32_t)R[4 * w + 3] << 24;
+                    continue;
+                }
                 const size_t q = u * T3 + e;
                 tab[(size_t)DT * MT + q] = mpz_sgn(z) < 0 ? 1u : 0u;
                 mpz_abs(z, z);
@@ -410,14 +465,16 @@ int run_fx(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t s
     a.flag = p->fx_flag.as<uint32_t>();
     a.N = (uint32_t)p->N; a.batch = p->batch;
     a.lim = p->fx_lim; a.P = (int)p->P;
-    hipError_t e = fx_fft_launch((int)p->L, (int)p->P, a, s0, s1, st);
+    a.ncu = p->ncu;
+    hipError_t e = p->fx_panel ? fxp_fft_launch((int)p->L, (int)p->P, a, s0, s1, st)
+                               : fx_fft_launch((int)p->L, (int)p->P, a, s0, s1, st);
     return e == hipSuccess ? MPFFT_OK : MPFFT_ERR_HIP;
 }
 
 // read and clear the fixed form's range flag (waits for the plan's stream)
 int fx_check(mpfft_plan_s *p)
 {
-    if (p->form != MPFFT_FORM_FIXED) return MPFFT_OK;
+    if (!is_fixed(p->form)) return MPFFT_OK;
     uint32_t f = 0;
     HIP_TRY(hipStreamSynchronize(p->stream));
     HIP_TRY(hipMemcpy(&f, p->fx_flag.p, sizeof f, hipMemcpyDeviceToHost));
@@ -507,7 +564,7 @@ int run_mp(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t s
 // one whole transform of a limb-layout form
 int run_limbs(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t st)
 {
-    return p->form == MPFFT_FORM_FIXED ? run_fx(p, d_in, d_out, st) : run_mp(p, d_in, d_out, st);
+    return is_fixed(p->form) ? run_fx(p, d_in, d_out, st) : run_mp(p, d_in, d_out, st);
 }
 
 // Combine the per-block partials of mp_err_kernel into, per transform,
@@ -661,7 +718,7 @@ int mpfft_plan_create(mpfft_plan *plan, unsigned long n, long prec, int form, un
         p->P = prec;
         p->L = limbs_for_precision(prec);
         p->words = 2 * n * (size_t)(p->L + 1);
-        rc = form == MPFFT_FORM_FIXED ? build_fx_plan(p.get()) : build_mp_plan(p.get());
+        rc = is_fixed(form) ? build_fx_plan(p.get()) : build_mp_plan(p.get());
     } else {
         p->P = 53;
         p->words = 2 * n;
@@ -771,14 +828,15 @@ int mpfft_execute_limbs_range(mpfft_plan p, const uint32_t *d_in, uint32_t *d_ou
     if (!is_limb_form(p->form)) return MPFFT_ERR_FORM;
     HIP_TRY(hipSetDevice(p->device));
     hipStream_t st = stream ? (hipStream_t)stream : p->stream;
-    if (p->form == MPFFT_FORM_FIXED) return run_fx(p, d_in, d_out, st, stage_begin, stage_end);
+    if (is_fixed(p->form)) return run_fx(p, d_in, d_out, st, stage_begin, stage_end);
     return run_mp(p, d_in, d_out, st, stage_begin, stage_end);
 }
 
 int mpfft_plan_stages(mpfft_plan p)
 {
     if (!p) return MPFFT_ERR_ARG;
-    if (p->form == MPFFT_FORM_FIXED) return fx_stages((uint32_t)p->N);
+    if (is_fixed(p->form))
+        return p->fx_panel ? fx_passes((uint32_t)p->N) + 2 : fx_stages((uint32_t)p->N);
     return is_mp(p->form) ? 1 + (int)p->levels.size() : MPFFT_ERR_FORM;
 }
 

@@ -1,0 +1,116 @@
+"""GPU check of the fixed form's panel kernels (MPFFT_FORM_FIXED with batch a
+multiple of 64: transforms interleaved in panels of 64, twiddle products on
+the int8 matrix cores), through the C ABI.
+
+The contract is the fixed form's (tests/test_gpu_fixed.py): against the
+2P-bit oracle the error is no worse than the plain form's.  Two more checks
+are specific to the panels:
+  - panel and per-lane kernels (MPFFT_FORM_FIXED_LANE) agree within 2 ulp on
+    every element of every transform: both stay within a small multiple of
+    2^-F (F = P + 32 bits and more) of the exact transform before their one
+    rounding to P bits, so they can differ only where the exact value lies
+    next to a rounding boundary;
+  - every transform of a batch has its own input, so a panel or lane index
+    that mixes transforms shows as an O(1) error.
+"""
+import numpy as np
+import pytest
+
+import mpfft_amd as M
+from mpfft_amd import inputs as I
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def run(n, P, xs, form):
+    Lo = O.limbs(P)
+    plan = M.Plan(n, P, form, len(xs))
+    y = plan.execute_host(np.concatenate([I.relimb(t, n, Lo, plan.L) for t in xs]))
+    plan.close()
+    return [I.relimb(t, n, plan.L, Lo) for t in np.split(y, len(xs))]
+
+
+def inputs(n, P, b, seed):
+    return [I.mp_random(n, P, seed=seed + 7919 * i) for i in range(b)]
+
+
+def check_against_lane(n, P, yp, yl):
+    Lo = O.limbs(P)
+    for i in range(len(yp)):
+        mu, nw, zb = O.err_stats(n, P, yp[i], Lo, yl[i], Lo)
+        assert zb == 0 and mu <= 2.0, (i, mu, nw)
+
+
+def check_against_oracle(n, P, x, y):
+    Lo, L2 = O.limbs(P), O.limbs(2 * P)
+    star = O.mp_fft(n, P, x, Pw=2 * P)
+    plain = O.mp_fft(n, P, x)                    # = the GPU plain form, bit for bit
+    mu_f, nw_f, zb_f = O.err_stats(n, P, y, Lo, star, L2)
+    mu_p, nw_p, _ = O.err_stats(n, P, plain, Lo, star, L2)
+    assert zb_f == 0
+    assert nw_f <= max(nw_p, 0.5), (nw_f, nw_p)
+    assert mu_f <= max(mu_p, 1.0), (mu_f, mu_p)
+
+
+@pytest.mark.parametrize("n,P", [(16, 1024), (32, 256), (64, 53), (128, 32), (256, 64),
+                                 (512, 100), (1024, 512), (2048, 1024), (4096, 128),
+                                 (8192, 256)])
+def test_panel_matches_lane_and_oracle(n, P):
+    xs = inputs(n, P, 64, seed=9000 + n + P)
+    yp = run(n, P, xs, M.FORM_FIXED)
+    yl = run(n, P, xs, M.FORM_FIXED_LANE)
+    check_against_lane(n, P, yp, yl)
+    for i in (0, 21, 63):
+        check_against_oracle(n, P, xs[i], yp[i])
+
+
+def test_panel_two_groups():
+    # batch 128: two groups of 64 transforms per pass
+    n, P = 256, 256
+    xs = inputs(n, P, 128, seed=4242)
+    yp = run(n, P, xs, M.FORM_FIXED)
+    yl = run(n, P, xs, M.FORM_FIXED_LANE)
+    check_against_lane(n, P, yp, yl)
+    check_against_oracle(n, P, xs[100], yp[100])
+
+
+def test_panel_stage_ranges():
+    # the stages run one at a time give the whole transform, bit for bit
+    import torch
+    n, P, b = 1024, 256, 64
+    xs = inputs(n, P, b, seed=77)
+    plan = M.Plan(n, P, M.FORM_FIXED, b)
+    x = np.concatenate([I.relimb(t, n, O.limbs(P), plan.L) for t in xs])
+    S = plan.stages()
+    assert S == 5 + 2                            # input conversion, five radix-4 passes, output
+    dx = torch.from_numpy(x.view(np.int32)).cuda()
+    d1, d2 = torch.empty_like(dx), torch.empty_like(dx)
+    plan.execute_device(dx.data_ptr(), d1.data_ptr())
+    for s in range(S):
+        plan.execute_range(dx.data_ptr(), d2.data_ptr(), s, s + 1)
+    plan.check()
+    assert torch.equal(d1, d2)
+    plan.close()
+
+
+def test_panel_input_range_checked():
+    n, P = 1024, 256
+    xs = inputs(n, P, 64, seed=5)
+    h, _ = I.split(xs[40], n, O.limbs(P))
+    h[17] = np.int32((40 << 1) | (h[17] & 1))    # |x| ~ 2^40, beyond 2^(29 - 10)
+    with pytest.raises(M.MpfftError) as e:
+        run(n, P, xs, M.FORM_FIXED)
+    assert e.value.code == -3
+    y = run(n, P, inputs(n, P, 64, seed=6), M.FORM_FIXED)
+    assert len(y) == 64
+
+
+def test_panel_full_size_1024_bits():
+    n, P = 1 << 16, 1024
+    xs = inputs(n, P, 64, seed=31)
+    yp = run(n, P, xs, M.FORM_FIXED)
+    yl = run(n, P, xs, M.FORM_FIXED_LANE)
+    check_against_lane(n, P, yp, yl)
+    for i in (0, 63):
+        check_against_oracle(n, P, xs[i], yp[i])
