@@ -210,10 +210,11 @@ __device__ __forceinline__ void mf_operand(MfOperand<W> &B, const Fx<W> &re, con
 
 // (re, im) *= (c + i d) for the 64 operands of the wave (one per lane), given
 // as B operands; the wave's twiddle reals c, d, -d are digit tables in LDS.
+// REAL: (re, im) *= c (a real twiddle: one product per part, Rd and Rnd unused).
 // Results go to the wave's LDS slot: word j of re' at slot[j 64 + lane], of
 // im' at slot[(W + j) 64 + lane].  The twiddle tables may lie in the im' part
 // of the slot: all of them are read before the first word of im' is written.
-template <int W>
+template <int W, bool REAL = false>
 __device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *Rc,
                                         const uint32_t *Rd, const uint32_t *Rnd, uint32_t lane,
                                         uint32_t *slot)
@@ -226,14 +227,15 @@ __device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *R
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         // h = 0: re' = c re + (-d) im;  h = 1: im' = d re + c im
-        const uint32_t *R1 = h ? Rd : Rc;
+        // (REAL: h = 0: re' = c re;  h = 1: im' = c im)
+        const uint32_t *R1 = REAL ? Rc : (h ? Rd : Rc);
         const uint32_t *R2 = h ? Rc : Rnd;
 #if FXP_ACACHE
         v4i A1[ND], A2[ND];
 #pragma unroll
         for (int dd = 0; dd < ND; dd++) {
             A1[dd] = mf_atile(R1, off0 - 32 * dd);
-            A2[dd] = mf_atile(R2, off0 - 32 * dd);
+            A2[dd] = REAL ? A1[dd] : mf_atile(R2, off0 - 32 * dd);
         }
 #endif
         int32_t carry = 0;
@@ -248,8 +250,14 @@ __device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *R
 #if FXP_ACACHE
                 const v4i a1 = A1[dd], a2 = A2[dd];
 #else
-                const v4i a1 = mf_atile(R1, off0 - 32 * dd), a2 = mf_atile(R2, off0 - 32 * dd);
+                const v4i a1 = mf_atile(R1, off0 - 32 * dd);
+                const v4i a2 = REAL ? a1 : mf_atile(R2, off0 - 32 * dd);
 #endif
+                if (REAL) {
+                    D0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, h ? B.i0[q] : B.r0[q], D0, 0, 0, 0);
+                    D1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, h ? B.i1[q] : B.r1[q], D1, 0, 0, 0);
+                    continue;
+                }
                 D0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, B.r0[q], D0, 0, 0, 0);
                 D1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, B.r1[q], D1, 0, 0, 0);
                 D0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a2, B.i0[q], D0, 0, 0, 0);
@@ -429,6 +437,151 @@ fxp_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
     }
 }
 
+// A radix-8 pass, panel layout in and out.  One item = one j and 64
+// transforms; the eight waves of a block hold v_r = x[j + r N/8] (r rotated
+// by the item index) and multiply it by w^(r k) on the matrix cores.  The
+// butterfly y_q = sum_r v_r w8^(r q) runs in three LDS-exchange stages:
+//   u_(r1, q0) = v_r1 + (-1)^q0 v_(r1+4)            (wave of input r1 + 4 q0)
+//   u'_(r1, 1) = w8^r1 u_(r1, 1)                     (r1 = 1, 3: two real products
+//                                                     by 1/sqrt(2), matrix cores)
+//   y_(q0 + 2 q1) = sum_r1 (-i)^(r1 q1) u'_(r1, q0)  (DFT4 of four slots)
+// LDS: eight slots of 2 W words x 64 lanes (139 KB at W = 34, one block per
+// CU, two waves per SIMD) and the table of 1/sqrt(2).
+template <int W>
+__global__ void __launch_bounds__(512, 1)
+fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t toff,
+              uint32_t nitems, uint32_t c8e)
+{
+    extern __shared__ uint32_t lds[];
+    constexpr MfGeom G = mf_geom(W);
+    constexpr uint32_t SLOT = 2 * W * 64, TWN = 3 * G.RW, TWL = (TWN + 63) / 64;
+    const uint32_t N = a.N, E8 = N >> 3;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    uint32_t *c8t = lds + 8 * SLOT;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)G.RW; i += blockDim.x)
+        c8t[i] = a.tw[(size_t)c8e * TWN + i];
+    uint32_t it = blockIdx.x;
+    if (it >= nitems) return;
+    auto item = [&](uint32_t i, uint32_t &j, uint32_t &grp, uint32_t &rot, uint32_t &r,
+                    uint32_t &k) {
+        j = i % E8; grp = i / E8; rot = i & 7u; r = (wv + rot) & 7u; k = j & (Ns - 1);
+    };
+    auto fetch = [&](uint32_t i, Fx<W> &re, Fx<W> &im, uint32_t (&tw)[TWL]) {
+        uint32_t j, grp, rot, r, k;
+        item(i, j, grp, rot, r, k);
+        if (r != 0 && k != 0) {
+            const uint32_t *t = a.tw + (size_t)(toff + 7 * k + r - 1) * TWN;
+#pragma unroll
+            for (uint32_t q = 0; q < TWL; q++)
+                tw[q] = (lane + 64 * q < TWN) ? (t + 64 * q)[lane] : 0u;
+        }
+        row_load(re, im, src, grp, j + r * E8, N, lane);
+    };
+    Fx<W> re, im;
+    uint32_t tw[TWL];
+    fetch(it, re, im, tw);
+    for (uint32_t n = 0;; n++) {
+        uint32_t j, grp, rot, r, k;
+        item(it, j, grp, rot, r, k);
+        const uint32_t nx = it + gridDim.x;
+        // slot of the wave holding input s: wave (s - rot) mod 8
+        auto sl = [&](uint32_t s_) { return lds + ((s_ - rot) & 7u) * SLOT + lane; };
+        uint32_t *slot = lds + wv * SLOT;
+        uint32_t *R = slot + SLOT - TWN;
+        __syncthreads();   // the slots of the last item are read (first item: c8t staged)
+        const bool prod = r != 0 && k != 0;
+        MfOperand<W> B;
+        if (prod) {
+#pragma unroll
+            for (uint32_t q = 0; q < TWL; q++)
+                if (lane + 64 * q < TWN) R[lane + 64 * q] = tw[q];
+            mf_operand<W>(B, re, im);
+        } else {
+#pragma unroll
+            for (int i = 0; i < W; i++) slot[i * 64 + lane] = re.w[i];
+#pragma unroll
+            for (int i = 0; i < W; i++) slot[(W + i) * 64 + lane] = im.w[i];
+        }
+        if (prod) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            mf_cmul<W>(B, R, R + G.RW, R + 2 * G.RW, lane, slot);
+        }
+        __syncthreads();
+        // stage 1: u = v_r + v_(r+4) (r < 4) or v_(r-4) - v_r (r >= 4)
+        Fx<W> ur, ui;
+        {
+            const uint32_t *lo = sl(r & 3u), *hi = sl((r & 3u) + 4);
+            uint32_t cr = 0, ci = 0;
+            if (r < 4) {
+#pragma unroll
+                for (int i = 0; i < W; i++) {
+                    ur.w[i] = add_cc(lo[i * 64], hi[i * 64], cr);
+                    ui.w[i] = add_cc(lo[(W + i) * 64], hi[(W + i) * 64], ci);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < W; i++) {
+                    ur.w[i] = sub_bb(lo[i * 64], hi[i * 64], cr);
+                    ui.w[i] = sub_bb(lo[(W + i) * 64], hi[(W + i) * 64], ci);
+                }
+            }
+        }
+        __syncthreads();
+        // stage 2: u' = w8^r1 u for the odd half (r = 4 + r1)
+        if (r == 5 || r == 7) {
+            // w8 (a + ib) = ((a + b) + i (b - a)) / sqrt2; w8^3 (a + ib) = ((b - a) - i (a + b)) / sqrt2
+            Fx<W> s_, d_;
+            fx_add(s_, ur, ui);
+            fx_sub(d_, ui, ur);
+            if (r == 7) {
+                Fx<W> z;
+#pragma unroll
+                for (int i = 0; i < W; i++) z.w[i] = 0;
+                fx_sub(ur, z, s_);            // -(a + b)
+                s_ = d_;
+                d_ = ur;
+            }
+            MfOperand<W> Bq;
+            mf_operand<W>(Bq, s_, d_);
+            mf_cmul<W, true>(Bq, c8t, c8t, c8t, lane, slot);
+        } else {
+            if (r == 6) {                     // -i (a + ib) = b - ia
+                Fx<W> z, t_;
+#pragma unroll
+                for (int i = 0; i < W; i++) z.w[i] = 0;
+                fx_sub(t_, z, ur);
+                ur = ui;
+                ui = t_;
+            }
+#pragma unroll
+            for (int i = 0; i < W; i++) slot[i * 64 + lane] = ur.w[i];
+#pragma unroll
+            for (int i = 0; i < W; i++) slot[(W + i) * 64 + lane] = ui.w[i];
+        }
+        // the next item's operands load during stage 3 and the stores (issued
+        // here, after both products, to keep them out of the products'
+        // registers)
+        if (nx < nitems) fetch(nx, re, im, tw);
+        __syncthreads();
+        // stage 3: output q = q0 + 2 q1 with q0 = r / 4, q1 = r % 4: the DFT4 of
+        // the slots of inputs 4 q0 + r1, r1 = 0..3
+        const uint32_t q0 = r >> 2, q1 = r & 3u;
+        const uint32_t *const vs[4] = {sl(4 * q0), sl(4 * q0 + 1), sl(4 * q0 + 2), sl(4 * q0 + 3)};
+        Fx<W> yr, yi;
+        switch (q1) {
+        case 0: dft4_out<W, 0>(yr, yi, vs); break;
+        case 1: dft4_out<W, 1>(yr, yi, vs); break;
+        case 2: dft4_out<W, 2>(yr, yi, vs); break;
+        default: dft4_out<W, 3>(yr, yi, vs); break;
+        }
+        row_store(yr, yi, dst, grp, (j - k) * 8 + k + (q0 + 2 * q1) * Ns, N, lane);
+        if (nx >= nitems) break;
+        it = nx;
+    }
+}
+
 // Output conversion: panel layout -> float SoA rounded to P bits, through
 // an LDS transpose (64-byte runs on both sides; block order as fxp_in_kernel).
 template <int W, int L, bool FULL>
@@ -466,31 +619,39 @@ template <int L, int W, bool FULL>
 static hipError_t launch_fxp(const FxArgs &a, int s0, int s1, hipStream_t st)
 {
     const uint32_t N = a.N;
-    uint32_t lg = 0;
-    while ((1u << lg) < N) lg++;
     auto in_range = [&](int s) { return s >= s0 && s < s1; };
     uint32_t *src = a.buf1, *dst = a.buf0;
     const uint32_t blocks16 = (2 * N / 16) * (a.batch / 16);
     // stage 0: input conversion into buf1
     if (in_range(0))
         hipLaunchKernelGGL((fxp_in_kernel<L, W>), dim3(blocks16), dim3(256), 0, st, a, src);
+    FxpPass ps[32];
+    uint32_t nent = 0;
+    const int np = fxp_plan(N, W, ps, &nent);
+    constexpr MfGeom G = mf_geom(W);
     int stage = 1;
-    uint32_t Ns = 1, toff = 0;
-    if (lg & 1) {
-        if (in_range(stage))
-            hipLaunchKernelGGL((fxp_r2_kernel<W>), dim3((N / 2) * (a.batch / 64) / 4), dim3(256),
-                               0, st, a, (const uint32_t *)src, dst);
-        std::swap(src, dst);
-        stage++;
-        Ns = 2;
-    }
-    const size_t sh = sizeof(uint32_t) * FXP_SETS * 4 * 2 * W * 64;   // sets of four slots
-    const uint32_t nitems = (N / 4) * (a.batch / 64);
-    const uint32_t nblk = nitems < FXP_OCC * a.ncu ? nitems : FXP_OCC * a.ncu;
-    for (; Ns < N; toff += 3 * Ns, Ns *= 4, stage++) {
-        if (in_range(stage))
-            hipLaunchKernelGGL((fxp_r4_kernel<W>), dim3(nblk), dim3(256), sh, st, a,
-                               (const uint32_t *)src, dst, Ns, toff, nitems);
+    for (int i = 0; i < np; i++, stage++) {
+        const FxpPass &P = ps[i];
+        if (in_range(stage)) {
+            if (P.R == 2) {
+                hipLaunchKernelGGL((fxp_r2_kernel<W>), dim3((N / 2) * (a.batch / 64) / 4),
+                                   dim3(256), 0, st, a, (const uint32_t *)src, dst);
+            } else if (P.R == 4) {
+                const size_t sh = sizeof(uint32_t) * FXP_SETS * 4 * 2 * W * 64;
+                const uint32_t nitems = (N / 4) * (a.batch / 64);
+                const uint32_t nblk = nitems < FXP_OCC * a.ncu ? nitems : FXP_OCC * a.ncu;
+                hipLaunchKernelGGL((fxp_r4_kernel<W>), dim3(nblk), dim3(256), sh, st, a,
+                                   (const uint32_t *)src, dst, P.Ns, P.toff, nitems);
+            } else if constexpr (W <= kPanelR8MaxW) {
+                const size_t sh = sizeof(uint32_t) * (8 * 2 * W * 64 + G.RW);
+                const uint32_t nitems = (N / 8) * (a.batch / 64);
+                const uint32_t nblk = nitems < a.ncu ? nitems : a.ncu;
+                hipLaunchKernelGGL((fxp_r8_kernel<W>), dim3(nblk), dim3(512), sh, st, a,
+                                   (const uint32_t *)src, dst, P.Ns, P.toff, nitems, nent);
+            } else {
+                return hipErrorInvalidValue;        // no radix-8 pass at this width
+            }
+        }
         std::swap(src, dst);
     }
     // the last stage: output conversion

@@ -162,6 +162,42 @@ inline bool fxp_eligible(uint32_t N, uint32_t batch)
     return N >= 16 && batch > 0 && batch % kPanelBatch == 0;
 }
 
+// Passes of the fixed form on panels.  Up to W = kPanelR8MaxW words per real:
+// radix 8, after a first pass of radix 2 or 4 when log2 N is not a multiple
+// of 3; wider reals: radix 4 after a radix-2 first pass when log2 N is odd
+// (a radix-8 pass keeps eight operands of W words per lane in LDS and
+// registers; at W = 34 it spills and runs one block per CU, and measured
+// slower than two radix-4 passes' worth of work).  Pass i has radix R and
+// sub-transform size Ns; its twiddles w^(r k) (k < Ns, r = 1 .. R-1) are
+// table entries toff + (R - 1) k + r - 1.  One more entry, at index *nent,
+// holds the real constant 1/sqrt(2) of the radix-8 butterflies.
+constexpr int kPanelR8MaxW = 18;
+
+struct FxpPass {
+    uint32_t R, Ns, toff;
+};
+
+inline int fxp_plan(uint32_t N, int W, FxpPass (&p)[32], uint32_t *nent)
+{
+    int lg = 0;
+    while ((1u << lg) < N) lg++;
+    const int rb = W <= kPanelR8MaxW ? 3 : 2;          // log2 of the main radix
+    int n = 0;
+    uint32_t Ns = 1, toff = 0;
+    if (lg % rb) {
+        const uint32_t R = 1u << (lg % rb);
+        p[n++] = {R, 1, toff};
+        toff += R - 1;
+        Ns = R;
+    }
+    for (; Ns < N; Ns <<= rb) {
+        p[n++] = {1u << rb, Ns, toff};
+        toff += ((1u << rb) - 1) * Ns;
+    }
+    *nent = toff;
+    return n;
+}
+
 // stages [s0, s1) of the fixed form on panels: the input conversion, one
 // stage per pass, the output conversion (rounding back to floats)
 hipError_t fxp_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st);

@@ -235,6 +235,7 @@ struct mpfft_plan_s {
     int fx_lim = 0;
     uint32_t fx_ntw = 0;
     bool fx_panel = false;   // panel kernels (matrix-core twiddle products)
+    int fx_npass = 0;        // passes of the panel plan
     uint32_t ncu = 0;        // compute units of the plan's device
     // double forms
     DevBuf tw, dleaf_sn, dleaf_off, dleaf_n, dlvl_off, dlvl_begin;
@@ -350,9 +351,24 @@ int build_fx_plan(mpfft_plan_s *p)
     p->ncu = ncu > 0 ? (uint32_t)ncu : 1u;
     const MfGeom G = mf_geom((int)W);
     std::vector<unsigned long> ent;               // t of every entry, in table order
-    for (unsigned long Ns = (lg0 & 1) ? 2 : 1; Ns < N; Ns *= 4)
-        for (unsigned long k = 0; k < Ns; k++)
-            for (unsigned long r = 1; r <= 3; r++) ent.push_back(r * k * (N / (4 * Ns)));
+    if (p->fx_panel) {
+        // the panel passes (fxp_plan), then the constant 1/sqrt(2) as the real
+        // part of w^(N/8)
+        FxpPass ps[32];
+        uint32_t nent = 0;
+        const int np = fxp_plan((uint32_t)N, (int)W, ps, &nent);
+        for (int i = 0; i < np; i++)
+            for (unsigned long k = 0; k < ps[i].Ns; k++)
+                for (unsigned long r = 1; r < ps[i].R; r++)
+                    ent.push_back(r * k * (N / ((unsigned long)ps[i].R * ps[i].Ns)));
+        if (ent.size() != nent) return MPFFT_ERR_ARG;
+        ent.push_back(N / 8);
+        p->fx_npass = np;
+    } else {
+        for (unsigned long Ns = (lg0 & 1) ? 2 : 1; Ns < N; Ns *= 4)
+            for (unsigned long k = 0; k < Ns; k++)
+                for (unsigned long r = 1; r <= 3; r++) ent.push_back(r * k * (N / (4 * Ns)));
+    }
     const size_t T3 = ent.size(), MT = 3 * T3;
     const size_t tabn = p->fx_panel ? T3 * 3 * (size_t)G.RW : MT * (DT + 1);
     std::vector<uint32_t> tab(tabn > 0 ? tabn : 1, 0);
@@ -836,7 +852,7 @@ int mpfft_plan_stages(mpfft_plan p)
 {
     if (!p) return MPFFT_ERR_ARG;
     if (is_fixed(p->form))
-        return p->fx_panel ? fx_passes((uint32_t)p->N) + 2 : fx_stages((uint32_t)p->N);
+        return p->fx_panel ? p->fx_npass + 2 : fx_stages((uint32_t)p->N);
     return is_mp(p->form) ? 1 + (int)p->levels.size() : MPFFT_ERR_FORM;
 }
 

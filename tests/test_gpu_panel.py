@@ -83,7 +83,10 @@ def test_panel_stage_ranges():
     plan = M.Plan(n, P, M.FORM_FIXED, b)
     x = np.concatenate([I.relimb(t, n, O.limbs(P), plan.L) for t in xs])
     S = plan.stages()
-    assert S == 5 + 2                            # input conversion, five radix-4 passes, output
+    # input conversion, a radix-2 pass (log2 n = 10 = 1 + 3 * 3), three radix-8
+    # passes (reals of W = 10 words), output conversion
+    assert S == 1 + 1 + 3 + 1
+    assert M.Plan(n, 1024, M.FORM_FIXED, b).stages() == 1 + 5 + 1   # W = 34: five radix-4
     dx = torch.from_numpy(x.view(np.int32)).cuda()
     d1, d2 = torch.empty_like(dx), torch.empty_like(dx)
     plan.execute_device(dx.data_ptr(), d1.data_ptr())
