@@ -117,3 +117,30 @@ def test_panel_full_size_1024_bits():
     check_against_lane(n, P, yp, yl)
     for i in (0, 63):
         check_against_oracle(n, P, xs[i], yp[i])
+
+
+def impulse(n, P, pos):
+    """x[pos] = 1 (0.1b * 2^1), every other real +0"""
+    L = O.limbs(P)
+    hdr = np.full(2 * n, np.int32(-2**31), dtype=np.int32)
+    limbs = np.zeros((L, 2 * n), dtype=np.uint32)
+    hdr[pos] = 2
+    limbs[0, pos] = 0x80000000
+    return I.join(hdr, limbs)
+
+
+@pytest.mark.parametrize("form,batch", [(M.FORM_FIXED, 64), (M.FORM_FIXED_LANE, 2)])
+@pytest.mark.parametrize("n,P", [(16, 1024), (32, 256), (1024, 64), (2048, 1024)])
+def test_fixed_impulse(form, batch, n, P):
+    # x = delta_0: every output exactly 1 + 0i (each pass adds zeros and
+    # multiplies only zeros by twiddles); x = delta_1: X[k] = w^k, within the
+    # plain form's error against the 2P-bit oracle
+    Lo = O.limbs(P)
+    for y in run(n, P, [impulse(n, P, 0)] * batch, form):
+        h, lm = I.split(y, n, Lo)
+        assert (h[:n] == 2).all() and (lm[0, :n] == 0x80000000).all() and (lm[1:, :n] == 0).all()
+        assert ((h[n:] & ~1) == np.int32(-2**31)).all() and (lm[:, n:] == 0).all()
+    x1 = impulse(n, P, 1)
+    ys = run(n, P, [x1] * batch, form)
+    check_against_oracle(n, P, x1, ys[0])
+    check_against_oracle(n, P, x1, ys[-1])
