@@ -4,7 +4,7 @@
 // significant, with value X / 2^F, F = 32 (W - 1): one integer word (sign
 // and 31 integer bits) above W - 1 fraction words.  Additions are exact
 // (W-word carry chains); a product with a twiddle keeps the bits at and
-// above 2^-F and drops the rest (error below 1.01 units of 2^-F).  No
+// above 2^-F and drops the rest (error below 1.04 units of 2^-F).  No
 // normalisation and no rounding step: that is where this form saves work
 // against correctly rounded floating point, at the price of an absolute
 // rather than relative error per operation (the FFT's own error is
@@ -29,9 +29,11 @@ struct FxDig {
     static constexpr int F = 32 * (W - 1);
     static constexpr int DX = (32 * W + kDigitBits - 1) / kDigitBits;   // |X| < 2^(32W-1)
     static constexpr int DT = (F + 1 + kDigitBits - 1) / kDigitBits;    // T < 2^(F+1)
-    // columns below K0 are skipped: their sum is below 2^(29*K0 + 58 + 6)
-    // <= 2^(F-8), i.e. under 1/256 of the output's last unit
-    static constexpr int K0 = (F - 72) >= 0 ? (F - 72) / kDigitBits : 0;
+    // columns below K0 are skipped.  Column c < K0 sums at most 64 digit
+    // products below 2^58 (DX, DT <= 63) at weight 2^(29 c), so the dropped
+    // part is below 2^64 * 2^(29 (K0 - 1)) * (1 + 2^-28) < 2^(29 K0 + 35.01)
+    // <= 2^(F - 4.99): under 1/32 of the output's last unit
+    static constexpr int K0 = (F - 40) >= 0 ? (F - 40) / kDigitBits : 0;
     static_assert(DX <= 63 && DT <= 63, "column sums would overflow 64 bits");
 };
 
