@@ -144,3 +144,29 @@ def test_fixed_impulse(form, batch, n, P):
     ys = run(n, P, [x1] * batch, form)
     check_against_oracle(n, P, x1, ys[0])
     check_against_oracle(n, P, x1, ys[-1])
+
+
+def test_panel_large_n():
+    # N = 2^20 (plan: a radix-4 pass, then six radix-8 passes; buffers of
+    # 64 x 2^21 reals), P = 128: every transform against the per-lane kernels,
+    # two against the 2P-bit oracle
+    n, P = 1 << 20, 128
+    # 16 random transforms, each also scaled by 2^-1, 2^-2, 2^-3 (an exponent
+    # shift: exact, and a transform that took another's data would differ
+    # by a power of two)
+    base = inputs(n, P, 16, seed=2020)
+    xs = []
+    for sh in range(4):
+        for x in base:
+            h, lm = I.split(x.copy(), n, O.limbs(P))
+            nz = (h & ~1) != np.int32(-2**31)
+            h[nz] -= np.int32(2 * sh)
+            xs.append(I.join(h, lm))
+    plan = M.Plan(n, P, M.FORM_FIXED, 64)
+    assert plan.stages() == 1 + 1 + 6 + 1
+    plan.close()
+    yp = run(n, P, xs, M.FORM_FIXED)
+    yl = run(n, P, xs, M.FORM_FIXED_LANE)
+    check_against_lane(n, P, yp, yl)
+    for i in (0, 63):
+        check_against_oracle(n, P, xs[i], yp[i])
