@@ -101,7 +101,8 @@ def measure(M, I, torch, n, P, batch, steps, warmup, seed, form):
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     S = plan.stages()
-    panel = form == M.FORM_FIXED and batch % 64 == 0 and n >= 16
+    pad = -batch % 64
+    panel = n >= 16 and (form == M.FORM_FIXED_PANEL or (form == M.FORM_FIXED and 8 * pad <= batch))
     cuts = [0, S] if form in M.FIXED_FORMS and not panel else ([0, 1, S - 1, S] if panel else [0, 1, S])
     for _ in range(warmup):
         plan.execute_range(x.data_ptr(), y.data_ptr(), 0, S, sp)

@@ -94,14 +94,21 @@ typedef struct mpfft_plan_s *mpfft_plan;
                               same SoA-limb I/O as MPFFT_FORM_MP, results at
                               the plain form's accuracy or better instead of
                               the reference's bit pattern; |input| < 2^(29 -
-                              log2 n), checked by mpfft_plan_check.  With
-                              batch a multiple of 64 and n >= 16 the
-                              twiddle products run on the int8 matrix cores
-                              (transforms interleaved in panels); otherwise
-                              as MPFFT_FORM_FIXED_LANE                     */
+                              log2 n), checked by mpfft_plan_check.  At
+                              n >= 16 and a batch that fills its panels of
+                              64 transforms to within an eighth (any
+                              multiple of 64, or 56 of 64, ...) as
+                              MPFFT_FORM_FIXED_PANEL; otherwise as
+                              MPFFT_FORM_FIXED_LANE                        */
 #define MPFFT_FORM_FIXED_LANE 9 /* the fixed form with per-lane twiddle
                               products (29-bit digit multiply-adds) at any
                               batch; same format and error bound          */
+#define MPFFT_FORM_FIXED_PANEL 10 /* the fixed form with transforms
+                              interleaved in panels of 64 and the twiddle
+                              products on the int8 matrix cores; any batch
+                              (a last, partial panel is padded), n >= 16
+                              (MPFFT_ERR_SIZE below); same format and error
+                              bound                                        */
 #define MPFFT_FORM_D5  5   /* double, CODE 5 op order (4m2a, 3 butterflies) */
 #define MPFFT_FORM_D6  6   /* double, CODE 6/7 op order (1 butterfly)      */
 
@@ -177,6 +184,54 @@ int mpfft_plan_sync(mpfft_plan plan);
  * fixed form since the last check (MPFFT_ERR_VALUE); MPFFT_OK otherwise.
  * The host and mpfr_t entry points check by themselves. */
 int mpfft_plan_check(mpfft_plan plan);
+
+/* ---- batched form --------------------------------------------------------- */
+
+/* Many transforms of uneven size and precision in one graph launch.
+ * Transforms of equal (n, prec) form a group that runs as one plan of that
+ * group's batch; the groups run on up to four streams inside one HIP graph,
+ * captured once per (d_in, d_out) pair and replayed by every execute.  With
+ * MPFFT_FORM_FIXED a group runs on the panel kernels where its count fills
+ * its panels to within an eighth, else on the per-lane kernels.  The
+ * autotuner times every group alone, deals the groups over 1, 2 and 4
+ * streams by those times and keeps the fastest graph.
+ * (No counterpart in the reference, which runs one transform per call.) */
+typedef struct mpfft_batch_s *mpfft_batch;
+
+/* `count` transforms; transform i has size n[i] (a power of two) at prec[i]
+ * bits.  form: MPFFT_FORM_MP, _MP_C1, _MP_C2 (the reference's op orders;
+ * every transform bit-identical to a plan of that form), MPFFT_FORM_FIXED
+ * (autotuned fixed form), MPFFT_FORM_FIXED_LANE or _FIXED_PANEL (one kernel
+ * family; groups with n < 16 run on the per-lane kernels).  Builds every
+ * group's plans (twiddle tables) before it returns. */
+int mpfft_batch_create(mpfft_batch *batch, unsigned int count, const unsigned long *n,
+                       const long *prec, int form);
+int mpfft_batch_destroy(mpfft_batch batch);
+
+/* Packed device layout, the same for input and output: transform i is the
+ * SoA-limb transform of n[i] points at mpfft_limbs_for_precision(prec[i])
+ * limbs, at word `offset` of buffers of `words` words in all.  `form`: the
+ * form that runs it (MPFFT_FORM_FIXED_LANE or MPFFT_FORM_FIXED_PANEL for
+ * a group of an MPFFT_FORM_FIXED batch).  Null outputs are skipped. */
+int mpfft_batch_words(mpfft_batch batch, size_t *words);
+int mpfft_batch_info(mpfft_batch batch, unsigned int i, size_t *offset, long *limbs, int *form);
+
+/* Time the schedules on these buffers and keep the fastest; d_out receives
+ * the transforms of d_in.  Waits for the device. */
+int mpfft_batch_tune(mpfft_batch batch, const uint32_t *d_in, uint32_t *d_out);
+
+/* The schedule in use: streams of the graph and its measured time (ms) */
+int mpfft_batch_schedule(mpfft_batch batch, int *streams, double *ms);
+
+/* Every transform, d_in -> d_out (packed layout, no overlap), as one graph
+ * launch on `stream` (a hipStream_t, NULL = the batch's own); tunes first on
+ * the first call.  Returns without waiting. */
+int mpfft_batch_execute_limbs(mpfft_batch batch, const uint32_t *d_in, uint32_t *d_out,
+                              void *stream);
+
+/* Wait for the stream of the last execute and report input-range violations
+ * of fixed-form groups since the last check (MPFFT_ERR_VALUE). */
+int mpfft_batch_check(mpfft_batch batch);
 
 /* ---- conversion ----------------------------------------------------------- */
 

@@ -17,8 +17,9 @@ FORM_MP, FORM_D5, FORM_D6 = 0, 5, 6
 FORM_MP_C1, FORM_MP_C2 = 1, 2          # the reference's CODE 1 / CODE 2 op orders
 FORM_FIXED = 8                          # fixed-point radix-4 form (same limb I/O)
 FORM_FIXED_LANE = 9                     # the fixed form on its per-lane kernels only
-FIXED_FORMS = (FORM_FIXED, FORM_FIXED_LANE)
-MP_FORMS = (FORM_MP, FORM_MP_C1, FORM_MP_C2, FORM_FIXED, FORM_FIXED_LANE)
+FORM_FIXED_PANEL = 10                   # the fixed form on panels (any batch, n >= 16)
+FIXED_FORMS = (FORM_FIXED, FORM_FIXED_LANE, FORM_FIXED_PANEL)
+MP_FORMS = (FORM_MP, FORM_MP_C1, FORM_MP_C2) + FIXED_FORMS
 OK = 0
 ERRORS = {
     -1: "MPFFT_ERR_SIZE", -2: "MPFFT_ERR_PREC", -3: "MPFFT_ERR_VALUE",
@@ -37,7 +38,9 @@ EXPORTS = [
     "mpfft_mp_binop_host", "mpfft_limbs_for_precision",
     "mpfft_execute_inverse_limbs", "mpfft_roundtrip_stats", "mpfft_roundtrip_host",
     "mpfft_plan_last_timing", "mpfft_limbs_to_hex", "mpfft_hex_to_limbs",
-    "mpfft_plan_check",
+    "mpfft_plan_check", "mpfft_batch_create", "mpfft_batch_destroy", "mpfft_batch_words",
+    "mpfft_batch_info", "mpfft_batch_tune", "mpfft_batch_schedule",
+    "mpfft_batch_execute_limbs", "mpfft_batch_check",
 ]
 
 _lib = None
@@ -85,6 +88,15 @@ def lib():
                                      ctypes.POINTER(ctypes.c_size_t)]
     L.mpfft_hex_to_limbs.argtypes = [U, L_, L_, ctypes.c_char_p, P]
     L.mpfft_mp_binop_host.argtypes = [I, U, L_, P, P, P]
+    L.mpfft_batch_create.argtypes = [ctypes.POINTER(VP), ctypes.c_uint, P, P, I]
+    L.mpfft_batch_destroy.argtypes = [VP]
+    L.mpfft_batch_words.argtypes = [VP, ctypes.POINTER(ctypes.c_size_t)]
+    L.mpfft_batch_info.argtypes = [VP, ctypes.c_uint, ctypes.POINTER(ctypes.c_size_t),
+                                   ctypes.POINTER(L_), ctypes.POINTER(I)]
+    L.mpfft_batch_tune.argtypes = [VP, P, P]
+    L.mpfft_batch_schedule.argtypes = [VP, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_double)]
+    L.mpfft_batch_execute_limbs.argtypes = [VP, P, P, P]
+    L.mpfft_batch_check.argtypes = [VP]
     L.mpfft_limbs_for_precision.argtypes = [L_]
     L.mpfft_limbs_for_precision.restype = L_
     L.mpfft_strerror.restype = ctypes.c_char_p
@@ -194,6 +206,89 @@ class Plan:
         out = np.zeros(8 * self.batch)
         check(lib().mpfft_roundtrip_host(self.h, _ptr(x), _ptr(out)), "roundtrip_host")
         return out.reshape(self.batch, 8)
+
+
+class Batch:
+    """Transforms of uneven size n[i] and precision P[i] in one graph launch
+    (mpfft_batch_*).  Device buffers use the packed layout: transform i at
+    word offsets[i], limbs[i] limbs; `words` words in all."""
+
+    def __init__(self, ns, Ps, form=FORM_FIXED):
+        ns = np.ascontiguousarray(ns, dtype=np.uint64)
+        Ps = np.ascontiguousarray(Ps, dtype=np.int64)
+        assert ns.size == Ps.size and ns.size > 0
+        self.ns, self.Ps, self.form, self.count = ns, Ps, form, int(ns.size)
+        h = ctypes.c_void_p()
+        check(lib().mpfft_batch_create(ctypes.byref(h), self.count, _ptr(ns), _ptr(Ps), form),
+              "batch_create")
+        self.h = h
+        w = ctypes.c_size_t()
+        check(lib().mpfft_batch_words(h, ctypes.byref(w)), "batch_words")
+        self.words = w.value
+        self.offsets, self.limbs = self._layout()
+
+    def _layout(self):
+        off, L = np.empty(self.count, dtype=np.int64), np.empty(self.count, dtype=np.int64)
+        o, l_ = ctypes.c_size_t(), ctypes.c_long()
+        for i in range(self.count):
+            check(lib().mpfft_batch_info(self.h, i, ctypes.byref(o), ctypes.byref(l_), None),
+                  "batch_info")
+            off[i], L[i] = o.value, l_.value
+        return off, L
+
+    def forms(self):
+        """the form that runs each transform (after tuning: lane or panel)"""
+        f, out = ctypes.c_int(), np.empty(self.count, dtype=np.int64)
+        for i in range(self.count):
+            check(lib().mpfft_batch_info(self.h, i, None, None, ctypes.byref(f)), "batch_info")
+            out[i] = f.value
+        return out
+
+    def close(self):
+        if self.h:
+            lib().mpfft_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pack(self, xs):
+        """host packed buffer from per-transform SoA-limb arrays"""
+        buf = np.zeros(self.words, dtype=np.uint32)
+        for i, x in enumerate(xs):
+            w = 2 * int(self.ns[i]) * (int(self.limbs[i]) + 1)
+            assert x.size == w
+            buf[self.offsets[i]:self.offsets[i] + w] = x
+        return buf
+
+    def unpack(self, buf):
+        out = []
+        for i in range(self.count):
+            w = 2 * int(self.ns[i]) * (int(self.limbs[i]) + 1)
+            out.append(np.array(buf[self.offsets[i]:self.offsets[i] + w], dtype=np.uint32))
+        return out
+
+    def tune(self, d_in, d_out):
+        check(lib().mpfft_batch_tune(self.h, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out)),
+              "batch_tune")
+
+    def schedule(self):
+        """(streams, measured ms of the graph)"""
+        s, ms = ctypes.c_int(), ctypes.c_double()
+        check(lib().mpfft_batch_schedule(self.h, ctypes.byref(s), ctypes.byref(ms)),
+              "batch_schedule")
+        return s.value, ms.value
+
+    def execute_device(self, d_in, d_out, stream=None):
+        s = ctypes.c_void_p(stream) if stream else None
+        check(lib().mpfft_batch_execute_limbs(self.h, ctypes.c_void_p(d_in),
+                                              ctypes.c_void_p(d_out), s), "batch_execute")
+
+    def check(self):
+        check(lib().mpfft_batch_check(self.h), "batch_check")
 
 
 def to_hex(x, count, L):

@@ -307,15 +307,21 @@ fxp_in_kernel(FxArgs a, uint32_t *dst)
 {
     __shared__ uint32_t tile[W * 16 * 17];
     const uint32_t N = a.N, M = 2 * N, t = threadIdx.x;
-    const uint32_t lb = xcd_block(), ng = a.batch / 16;
+    const uint32_t lb = xcd_block(), ng = 4 * fxp_groups(a.batch);
     const uint32_t g16 = lb % ng, eb = lb / ng;
     {
-        // lane = (element t % 16, transform t / 16): 64-byte runs of the input
-        const uint32_t el = t & 15u, bl = t >> 4;
-        Mp<L> x;
-        mp_load(x, a.in + a.in_stride * (g16 * 16 + bl), M, eb * 16 + el);
+        // lane = (element t % 16, transform t / 16): 64-byte runs of the input;
+        // the padding transforms of a last, partial panel are zeros
+        const uint32_t el = t & 15u, bl = t >> 4, b = g16 * 16 + bl;
         Fx<W> z;
-        if (fx_from_mp<L, W>(z, x, a.lim)) atomicOr(a.flag, 1u);
+        if (b < a.batch) {
+            Mp<L> x;
+            mp_load(x, a.in + a.in_stride * b, M, eb * 16 + el);
+            if (fx_from_mp<L, W>(z, x, a.lim)) atomicOr(a.flag, 1u);
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; w++) z.w[w] = 0u;
+        }
 #pragma unroll
         for (int w = 0; w < W; w++) tile[(w * 16 + el) * 17 + bl] = z.w[w];
     }
@@ -590,7 +596,7 @@ fxp_out_kernel(FxArgs a, const uint32_t *src)
 {
     __shared__ uint32_t tile[(L + 1) * 16 * 17];
     const uint32_t N = a.N, M = 2 * N, t = threadIdx.x;
-    const uint32_t lb = xcd_block(), ng = a.batch / 16;
+    const uint32_t lb = xcd_block(), ng = (a.batch + 15) / 16;
     const uint32_t g16 = lb % ng, eb = lb / ng;
     {
         // lane = (transform t % 16, element t / 16): 64-byte runs of a panel row
@@ -608,6 +614,7 @@ fxp_out_kernel(FxArgs a, const uint32_t *src)
     __syncthreads();
     // lane = (element t % 16, transform t / 16): 64-byte runs of the output
     const uint32_t el = t & 15u, bl = t >> 4;
+    if (g16 * 16 + bl >= a.batch) return;           // padding transform
     uint32_t *o = a.out + a.out_stride * (g16 * 16 + bl) + eb * 16 + el;
 #pragma unroll
     for (int j = 0; j <= L; j++) o[(size_t)j * M] = tile[(j * 16 + el) * 17 + bl];
@@ -621,10 +628,11 @@ static hipError_t launch_fxp(const FxArgs &a, int s0, int s1, hipStream_t st)
     const uint32_t N = a.N;
     auto in_range = [&](int s) { return s >= s0 && s < s1; };
     uint32_t *src = a.buf1, *dst = a.buf0;
-    const uint32_t blocks16 = (2 * N / 16) * (a.batch / 16);
-    // stage 0: input conversion into buf1
+    const uint32_t NG = fxp_groups(a.batch);          // panels, the last maybe partial
+    // stage 0: input conversion into buf1 (every panel row, padding included)
     if (in_range(0))
-        hipLaunchKernelGGL((fxp_in_kernel<L, W>), dim3(blocks16), dim3(256), 0, st, a, src);
+        hipLaunchKernelGGL((fxp_in_kernel<L, W>), dim3((2 * N / 16) * 4 * NG), dim3(256), 0, st,
+                           a, src);
     FxpPass ps[32];
     uint32_t nent = 0;
     const int np = fxp_plan(N, W, ps, &nent);
@@ -634,17 +642,17 @@ static hipError_t launch_fxp(const FxArgs &a, int s0, int s1, hipStream_t st)
         const FxpPass &P = ps[i];
         if (in_range(stage)) {
             if (P.R == 2) {
-                hipLaunchKernelGGL((fxp_r2_kernel<W>), dim3((N / 2) * (a.batch / 64) / 4),
+                hipLaunchKernelGGL((fxp_r2_kernel<W>), dim3((N / 2) * NG / 4),
                                    dim3(256), 0, st, a, (const uint32_t *)src, dst);
             } else if (P.R == 4) {
                 const size_t sh = sizeof(uint32_t) * FXP_SETS * 4 * 2 * W * 64;
-                const uint32_t nitems = (N / 4) * (a.batch / 64);
+                const uint32_t nitems = (N / 4) * NG;
                 const uint32_t nblk = nitems < FXP_OCC * a.ncu ? nitems : FXP_OCC * a.ncu;
                 hipLaunchKernelGGL((fxp_r4_kernel<W>), dim3(nblk), dim3(256), sh, st, a,
                                    (const uint32_t *)src, dst, P.Ns, P.toff, nitems);
             } else if constexpr (W <= kPanelR8MaxW) {
                 const size_t sh = sizeof(uint32_t) * (8 * 2 * W * 64 + G.RW);
-                const uint32_t nitems = (N / 8) * (a.batch / 64);
+                const uint32_t nitems = (N / 8) * NG;
                 const uint32_t nblk = nitems < a.ncu ? nitems : a.ncu;
                 hipLaunchKernelGGL((fxp_r8_kernel<W>), dim3(nblk), dim3(512), sh, st, a,
                                    (const uint32_t *)src, dst, P.Ns, P.toff, nitems, nent);
@@ -656,8 +664,8 @@ static hipError_t launch_fxp(const FxArgs &a, int s0, int s1, hipStream_t st)
     }
     // the last stage: output conversion
     if (in_range(stage))
-        hipLaunchKernelGGL((fxp_out_kernel<W, L, FULL>), dim3(blocks16), dim3(256), 0, st, a,
-                           (const uint32_t *)src);
+        hipLaunchKernelGGL((fxp_out_kernel<W, L, FULL>), dim3((2 * N / 16) * ((a.batch + 15) / 16)),
+                           dim3(256), 0, st, a, (const uint32_t *)src);
     return hipGetLastError();
 }
 

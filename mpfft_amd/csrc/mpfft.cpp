@@ -45,7 +45,11 @@ bool is_mp(int form)
     return form == MPFFT_FORM_MP || form == MPFFT_FORM_MP_C1 || form == MPFFT_FORM_MP_C2;
 }
 
-bool is_fixed(int form) { return form == MPFFT_FORM_FIXED || form == MPFFT_FORM_FIXED_LANE; }
+bool is_fixed(int form)
+{
+    return form == MPFFT_FORM_FIXED || form == MPFFT_FORM_FIXED_LANE ||
+           form == MPFFT_FORM_FIXED_PANEL;
+}
 
 // forms whose device data use the SoA-limb layout of include/mpfft.h
 bool is_limb_form(int form) { return is_mp(form) || is_fixed(form); }
@@ -345,7 +349,10 @@ int build_fx_plan(mpfft_plan_s *p)
     const long DT = (F + 1 + 28) / 29;
     int lg0 = 0;
     while ((1UL << lg0) < N) lg0++;
-    p->fx_panel = p->form == MPFFT_FORM_FIXED && fxp_eligible((uint32_t)N, p->batch);
+    if (p->form == MPFFT_FORM_FIXED_PANEL && !fxp_eligible((uint32_t)N, p->batch))
+        return MPFFT_ERR_SIZE;
+    p->fx_panel = p->form == MPFFT_FORM_FIXED_PANEL ||
+                  (p->form == MPFFT_FORM_FIXED && fxp_preferred((uint32_t)N, p->batch));
     int ncu = 0;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device));
     p->ncu = ncu > 0 ? (uint32_t)ncu : 1u;
@@ -457,7 +464,7 @@ int build_fx_plan(mpfft_plan_s *p)
     for (auto &x : th) x.join();
     int rc;
     if ((rc = p->fx_tw.upload(tab))) return rc;
-    const size_t fxw = 2 * N * (size_t)W * p->batch;
+    const size_t fxw = 2 * N * (size_t)W * (p->fx_panel ? fxp_groups(p->batch) * kPanel : p->batch);
     if ((rc = p->fx_buf0.alloc(sizeof(uint32_t) * fxw)) ||
         (rc = p->fx_buf1.alloc(sizeof(uint32_t) * fxw)) ||
         (rc = p->fx_flag.upload(std::vector<uint32_t>{0u})))

@@ -80,6 +80,25 @@ def mp_random(n, P, seed=DEFAULT_SEED, L=None, array_ids=(0, 1)):
     return np.concatenate([hdr.view(np.uint32), limb.reshape(-1)])
 
 
+BATCH_PRECISIONS = (128, 256, 512, 1024)
+
+
+def batch_draw(count=4096, seed=DEFAULT_SEED, lg_lo=6, lg_hi=14, precs=BATCH_PRECISIONS):
+    """The batched workload: `count` transforms with log2 n uniform over
+    lg_lo..lg_hi and P uniform over precs (stream 1000 of the seed).
+    Returns (n, P) as int64 arrays."""
+    d = draws(stream_key(seed, 1000), 0, 2 * count)
+    lg = lg_lo + (d[0::2] % np.uint64(lg_hi - lg_lo + 1)).astype(np.int64)
+    P = np.asarray(precs, dtype=np.int64)[(d[1::2] % np.uint64(len(precs))).astype(np.int64)]
+    return np.left_shift(1, lg).astype(np.int64), P
+
+
+def batch_inputs(ns, Ps, seed=DEFAULT_SEED):
+    """inputs of the batched workload: transform i from arrays 2 + 2i, 3 + 2i"""
+    return [mp_random(int(n), int(P), seed=seed, array_ids=(2 + 2 * i, 3 + 2 * i))
+            for i, (n, P) in enumerate(zip(ns, Ps))]
+
+
 def d_random(n, seed=DEFAULT_SEED, array_ids=(0, 1)):
     """n complex points uniform[-1,1) binary64: array of 2n (re then im)."""
     out = np.empty(2 * n, dtype=np.float64)
