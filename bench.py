@@ -9,6 +9,13 @@ against a 2P-bit oracle is no worse than the plain form's, componentwise
 baseline and the drop-in call from mpfr_t arrays.
 
     python bench.py [--steps K] [--warmup W] [--n N] [--prec P] [--batch B]
+                    [--batched-count C]
+
+Beside the headline line: the double-word FP64 form at 106 bits against
+the plain and fixed forms at 128 bits ("precision_form"), and the batched
+workload of C transforms of uneven size and precision ("batched"), one
+graph launch of the batched forms against the plain form run one transform
+at a time.
 
 A step is one pass of a form over one batch of B synthetic transforms
 whose inputs are already resident in HBM.  Prints ONE JSON line.
@@ -128,6 +135,28 @@ def measure(M, I, torch, n, P, batch, steps, warmup, seed, form):
                 S=S, out0=out0, x0=x0, panel=panel, L=plan.L)
 
 
+def measure_precision_form(M, I, torch, O, n, P, Pcmp, batch, steps, seed):
+    """The double-word form (P <= 106 bits) beside the plain and the fixed
+    form at Pcmp bits, all at N = n on `batch` device-resident transforms;
+    each form's error against its own 2P-bit oracle on transform 0."""
+    out = {"workload": f"N={n}, batch {batch}: double-word form at {P} bits; plain and fixed "
+                       f"forms at {Pcmp} bits"}
+    for name, form, prec in (("double_word", M.FORM_DD, P), ("plain", M.FORM_MP, Pcmp),
+                             ("fixed", M.FORM_FIXED, Pcmp)):
+        r = measure(M, I, torch, n, prec, batch, steps, 1, seed, form)
+        star = O.mp_fft(n, prec, r["x0"], Pw=2 * prec)
+        mu, nw, zb = O.err_stats(n, prec, r["out0"], I.limbs_for(prec), star,
+                                 I.limbs_for(2 * prec))
+        out[name] = {"precision_bits": prec, "fft_per_s": round(r["fft_s"], 1),
+                     "error_vs_2P_oracle": {"max_ulp": mu, "normwise_2^-P": nw,
+                                            "zero_mismatch": zb}}
+    out["double_word_vs_plain"] = round(out["double_word"]["fft_per_s"] /
+                                        out["plain"]["fft_per_s"], 2)
+    out["double_word_vs_fixed"] = round(out["double_word"]["fft_per_s"] /
+                                        out["fixed"]["fft_per_s"], 2)
+    return out
+
+
 def measure_mpfr(M, I, n, P, seed, reps=3):
     """the drop-in path: one transform from and to mpfr_t arrays
     (marshalling, PCIe transfers and the GPU transform), per call"""
@@ -157,6 +186,83 @@ def measure_mpfr(M, I, n, P, seed, reps=3):
             "note": "one transform per call from mpfr_t arrays, PCIe transfers included"}
 
 
+def measure_batched(M, I, torch, seed, count, steps):
+    """The batched workload: `count` transforms, log2 n uniform over 6..14, P
+    over {128, 256, 512, 1024}, inputs resident in HBM.  Times one graph
+    launch of the batched fixed form and of the batched plain form, and the
+    plain form run one transform at a time (a batch-1 plan per (n, P), one
+    launch sequence per transform, as a caller of the reference's one-call-
+    per-transform interface would); checks the two plain runs agree bit for
+    bit."""
+    ns, Ps = I.batch_draw(count, seed=seed)
+    xs = I.batch_inputs(ns, Ps, seed=seed)
+    t0 = time.time()
+    bf = M.Batch(ns, Ps, M.FORM_FIXED)
+    create_s = time.time() - t0
+    bm = M.Batch(ns, Ps, M.FORM_MP)
+    assert np.array_equal(bf.offsets, bm.offsets) and bf.words == bm.words
+    host = bf.pack([I.relimb(x, int(n), I.limbs_for(int(P)), int(L))
+                    for x, n, P, L in zip(xs, ns, Ps, bf.limbs)])
+    del xs
+    dx = torch.from_numpy(host.view(np.int32)).cuda()
+    del host
+    dy, dy2 = torch.empty_like(dx), torch.empty_like(dx)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    def timed(fn, k):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(k):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    out = {"workload": f"{count} transforms, n = 2^6..2^14 log-uniform, P in {{128, 256, 512, "
+                       f"1024}} bits ({len(set(zip(ns.tolist(), Ps.tolist())))} (n, P) groups), "
+                       "inputs resident in HBM",
+           "words": int(bf.words)}
+    t0 = time.time()
+    bf.tune(dx.data_ptr(), dy.data_ptr())
+    tune_s = time.time() - t0
+    ms_f = timed(lambda: bf.execute_device(dx.data_ptr(), dy.data_ptr(), sp), steps)
+    bf.check()
+    forms = bf.forms()
+    out["fixed_graph"] = {"fft_per_s": round(count / ms_f * 1e3, 1), "ms": round(ms_f, 3),
+                          "streams": bf.schedule()[0],
+                          "groups_on_panels": int(len(set(
+                              (int(n), int(P)) for n, P, f in zip(ns, Ps, forms)
+                              if f == M.FORM_FIXED_PANEL))),
+                          "create_s": round(create_s, 2), "tune_s": round(tune_s, 2)}
+    bf.close()
+    bm.tune(dx.data_ptr(), dy.data_ptr())
+    ms_m = timed(lambda: bm.execute_device(dx.data_ptr(), dy.data_ptr(), sp), steps)
+    out["plain_graph"] = {"fft_per_s": round(count / ms_m * 1e3, 1), "ms": round(ms_m, 3),
+                          "streams": bm.schedule()[0]}
+    plans = {}
+    for n, P in set(zip(ns.tolist(), Ps.tolist())):
+        plans[(n, P)] = M.Plan(n, P, M.FORM_MP, 1)
+    keys = [(int(n), int(P)) for n, P in zip(ns, Ps)]
+    offs = [int(o) * 4 for o in bm.offsets]
+    xb, yb = dx.data_ptr(), dy2.data_ptr()
+
+    def one_by_one():
+        for k, o in zip(keys, offs):
+            plans[k].execute_device(xb + o, yb + o, sp)
+
+    ms_1 = timed(one_by_one, 1)
+    out["plain_per_transform"] = {"fft_per_s": round(count / ms_1 * 1e3, 1), "ms": round(ms_1, 3)}
+    out["plain_graph_bit_identical_to_per_transform"] = bool(torch.equal(dy, dy2))
+    out["fixed_graph_vs_plain_per_transform"] = round(ms_1 / ms_f, 2)
+    out["plain_graph_vs_plain_per_transform"] = round(ms_1 / ms_m, 2)
+    for p in plans.values():
+        p.close()
+    bm.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +277,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-err", action="store_true", help="skip the 2P-bit oracle error")
     ap.add_argument("--no-mpfr", action="store_true", help="skip the mpfr_t-array call timing")
+    ap.add_argument("--dd-prec", type=int, default=106,
+                    help="precision of the double-word form leg (0 = skip); compared with "
+                         "the plain and fixed forms at 128 bits")
+    ap.add_argument("--batched-count", type=int, default=4096,
+                    help="transforms of the batched uneven-(n, P) workload (0 = skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -320,6 +431,12 @@ def main():
         line["error_seconds"] = err_s
     if not a.no_mpfr:
         line["mpfr_path"] = measure_mpfr(M, I, n, P, a.seed)
+    if a.dd_prec:
+        line["precision_form"] = measure_precision_form(M, I, torch, O, n, a.dd_prec, 128, B,
+                                                        max(2, a.steps // 2), a.seed)
+    if a.batched_count:
+        line["batched"] = measure_batched(M, I, torch, a.seed, a.batched_count,
+                                          max(2, a.steps // 2))
     if not a.no_cpu:
         # bounded sample: the reference's OpenMP CODE 2 (4 threads, hard-coded
         # at parallel_mp_fft2.c:262) when its build is present, and the oracle
@@ -336,11 +453,16 @@ def main():
         port = {"value": round(1.0 / t, 4), "unit": "FFT/s", "cores": nthr, "kind": "port",
                 "sample": f"2 FFTs N={n} P={P}, oracle restatement, {nthr} OpenMP threads, "
                           "LUT and conversion excluded"}
+        t4 = O.mp_fft_time(n, P, xs, 2, 4)
+        port4 = dict(port, value=round(1.0 / t4, 4), cores=4,
+                     sample=f"2 FFTs N={n} P={P}, oracle restatement, 4 OpenMP threads, "
+                            "LUT and conversion excluded")
         if cpu:
             line["cpu_baseline"] = cpu
             line["cpu_port"] = port
         else:
             line["cpu_baseline"] = port
+        line["cpu_port_4_threads"] = port4
     print(json.dumps(line))
 
 

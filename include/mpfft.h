@@ -109,6 +109,15 @@ typedef struct mpfft_plan_s *mpfft_plan;
                               (a last, partial panel is padded), n >= 16
                               (MPFFT_ERR_SIZE below); same format and error
                               bound                                        */
+#define MPFFT_FORM_DD 11  /* double-word form, 53 <= prec <= 106: every
+                              real is hi + lo (two binary64) and every
+                              operation an error-free transformation on the
+                              FP64 units (TwoSum, FMA residual products);
+                              same SoA-limb I/O, the result rounded once to
+                              prec bits; |exponent| of the input <= 900,
+                              checked by mpfft_plan_check (MPFFT_ERR_VALUE);
+                              normwise error against the exact transform
+                              below 2^-prec + (2 + 3 log2 n) 2^-104       */
 #define MPFFT_FORM_D5  5   /* double, CODE 5 op order (4m2a, 3 butterflies) */
 #define MPFFT_FORM_D6  6   /* double, CODE 6/7 op order (1 butterfly)      */
 

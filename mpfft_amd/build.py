@@ -29,7 +29,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           f"--offload-arch={ARCH}", f"-I{MPFR_INC}", f"-I{os.path.join(ROOT, 'include')}"]
 
-SOURCES = ["mp_fft.hip", "fx_fft.hip", "fx_panel.hip", "d_fft.hip", "mpfft.cpp", "batch.cpp",
+SOURCES = ["mp_fft.hip", "fx_fft.hip", "fx_panel.hip", "dd_fft.hip", "d_fft.hip", "mpfft.cpp",
+           "batch.cpp",
            "hexio.cpp"]
 HEADERS = ["mp_arith.h", "fx_arith.h", "fx_lane.h", "kernels.h"]
 

@@ -32,6 +32,8 @@ thread_local int g_last_error = MPFFT_OK;
 constexpr long kMaxPrec = 1024;          // largest compiled limb count * 32
 constexpr long kExpLimit = 1L << 29;     // accepted input exponent range
 constexpr unsigned long kMaxN = 1UL << 24;
+constexpr long kDdMinPrec = 53, kDdMaxPrec = 106;   // precisions of the double-word form
+constexpr int kDdExpLimit = 900;         // |exponent| of double-word inputs
 
 #define HIP_TRY(expr)                                          \
     do {                                                       \
@@ -51,8 +53,11 @@ bool is_fixed(int form)
            form == MPFFT_FORM_FIXED_PANEL;
 }
 
+// forms that read their input through a range check (MPFFT_ERR_VALUE)
+bool has_range_flag(int form) { return is_fixed(form) || form == MPFFT_FORM_DD; }
+
 // forms whose device data use the SoA-limb layout of include/mpfft.h
-bool is_limb_form(int form) { return is_mp(form) || is_fixed(form); }
+bool is_limb_form(int form) { return is_mp(form) || has_range_flag(form); }
 
 // the reference's operation order of a multiprecision form: CODE 3
 // (serial_mp_fft3.c), CODE 1 (serial_mp_fft1.c) or CODE 2 (serial_mp_fft2.c)
@@ -474,6 +479,58 @@ int build_fx_plan(mpfft_plan_s *p)
     return MPFFT_OK;
 }
 
+// Double-word form: twiddles of the radix-4 passes (the per-lane fixed
+// form's entry order: pass after pass, entry 3k + r - 1 = w^(r k N / (4 Ns)))
+// as {re.hi, re.lo, im.hi, im.lo}, each part the binary64 nearest to the
+// value, resp. to what the first part leaves (MPFR at 256 bits), and the
+// double2 ping-pong buffers.
+int build_dd_plan(mpfft_plan_s *p)
+{
+    const unsigned long N = p->N;
+    int lg0 = 0;
+    while ((1UL << lg0) < N) lg0++;
+    std::vector<unsigned long> ent;
+    for (unsigned long Ns = (lg0 & 1) ? 2 : 1; Ns < N; Ns *= 4)
+        for (unsigned long k = 0; k < Ns; k++)
+            for (unsigned long r = 1; r <= 3; r++) ent.push_back(r * k * (N / (4 * Ns)));
+    std::vector<double> tab(4 * (ent.empty() ? 1 : ent.size()), 0.0);
+    unsigned nt = std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
+    if (ent.size() < 1024) nt = 1;
+    auto work = [&](size_t e0, size_t e1) {
+        mpfr_t th, c, d, t;
+        mpfr_inits2(256, th, c, d, t, (mpfr_ptr)0);
+        for (size_t e = e0; e < e1; e++) {
+            mpfr_const_pi(th, MPFR_RNDN);
+            mpfr_mul_ui(th, th, 2 * ent[e], MPFR_RNDN);
+            mpfr_div_ui(th, th, N, MPFR_RNDN);
+            mpfr_sin_cos(d, c, th, MPFR_RNDN);
+            mpfr_neg(d, d, MPFR_RNDN);                    // w = cos - i sin
+            for (int u = 0; u < 2; u++) {
+                mpfr_ptr v = u ? d : c;
+                const double hi = mpfr_get_d(v, MPFR_RNDN);
+                mpfr_sub_d(t, v, hi, MPFR_RNDN);          // exact at 256 bits
+                tab[4 * e + 2 * u] = hi;
+                tab[4 * e + 2 * u + 1] = mpfr_get_d(t, MPFR_RNDN);
+            }
+        }
+        mpfr_clears(th, c, d, t, (mpfr_ptr)0);
+        mpfr_free_cache();
+    };
+    std::vector<std::thread> th;
+    for (unsigned i = 0; i < nt; i++)
+        th.emplace_back(work, ent.size() * i / nt, ent.size() * (i + 1) / nt);
+    for (auto &x : th) x.join();
+    int rc;
+    if ((rc = p->fx_tw.upload(tab))) return rc;
+    const size_t bytes = 2 * sizeof(double) * 2 * N * (size_t)p->batch;
+    if ((rc = p->fx_buf0.alloc(bytes)) || (rc = p->fx_buf1.alloc(bytes)) ||
+        (rc = p->fx_flag.upload(std::vector<uint32_t>{0u})))
+        return rc;
+    p->fx_ntw = (uint32_t)ent.size();
+    p->fx_lim = kDdExpLimit;
+    return MPFFT_OK;
+}
+
 int run_fx(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t st,
            int s0 = 0, int s1 = 1 << 30)
 {
@@ -489,15 +546,17 @@ int run_fx(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t s
     a.N = (uint32_t)p->N; a.batch = p->batch;
     a.lim = p->fx_lim; a.P = (int)p->P;
     a.ncu = p->ncu;
-    hipError_t e = p->fx_panel ? fxp_fft_launch((int)p->L, (int)p->P, a, s0, s1, st)
-                               : fx_fft_launch((int)p->L, (int)p->P, a, s0, s1, st);
+    if (p->form == MPFFT_FORM_DD) a.stride = 2 * p->N;          // double2 per real
+    hipError_t e = p->form == MPFFT_FORM_DD ? dd_fft_launch((int)p->L, (int)p->P, a, s0, s1, st)
+                   : p->fx_panel            ? fxp_fft_launch((int)p->L, (int)p->P, a, s0, s1, st)
+                                            : fx_fft_launch((int)p->L, (int)p->P, a, s0, s1, st);
     return e == hipSuccess ? MPFFT_OK : MPFFT_ERR_HIP;
 }
 
 // read and clear the fixed form's range flag (waits for the plan's stream)
 int fx_check(mpfft_plan_s *p)
 {
-    if (!is_fixed(p->form)) return MPFFT_OK;
+    if (!has_range_flag(p->form)) return MPFFT_OK;
     uint32_t f = 0;
     HIP_TRY(hipStreamSynchronize(p->stream));
     HIP_TRY(hipMemcpy(&f, p->fx_flag.p, sizeof f, hipMemcpyDeviceToHost));
@@ -587,7 +646,7 @@ int run_mp(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t s
 // one whole transform of a limb-layout form
 int run_limbs(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t st)
 {
-    return is_fixed(p->form) ? run_fx(p, d_in, d_out, st) : run_mp(p, d_in, d_out, st);
+    return has_range_flag(p->form) ? run_fx(p, d_in, d_out, st) : run_mp(p, d_in, d_out, st);
 }
 
 // Combine the per-block partials of mp_err_kernel into, per transform,
@@ -738,10 +797,14 @@ int mpfft_plan_create(mpfft_plan *plan, unsigned long n, long prec, int form, un
     int rc;
     if (is_limb_form(form)) {
         if (prec < 2 || prec > kMaxPrec) return MPFFT_ERR_PREC;
+        if (form == MPFFT_FORM_DD && (prec < kDdMinPrec || prec > kDdMaxPrec))
+            return MPFFT_ERR_PREC;
         p->P = prec;
         p->L = limbs_for_precision(prec);
         p->words = 2 * n * (size_t)(p->L + 1);
-        rc = is_fixed(form) ? build_fx_plan(p.get()) : build_mp_plan(p.get());
+        rc = form == MPFFT_FORM_DD ? build_dd_plan(p.get())
+             : is_fixed(form)      ? build_fx_plan(p.get())
+                                   : build_mp_plan(p.get());
     } else {
         p->P = 53;
         p->words = 2 * n;
@@ -851,13 +914,14 @@ int mpfft_execute_limbs_range(mpfft_plan p, const uint32_t *d_in, uint32_t *d_ou
     if (!is_limb_form(p->form)) return MPFFT_ERR_FORM;
     HIP_TRY(hipSetDevice(p->device));
     hipStream_t st = stream ? (hipStream_t)stream : p->stream;
-    if (is_fixed(p->form)) return run_fx(p, d_in, d_out, st, stage_begin, stage_end);
+    if (has_range_flag(p->form)) return run_fx(p, d_in, d_out, st, stage_begin, stage_end);
     return run_mp(p, d_in, d_out, st, stage_begin, stage_end);
 }
 
 int mpfft_plan_stages(mpfft_plan p)
 {
     if (!p) return MPFFT_ERR_ARG;
+    if (p->form == MPFFT_FORM_DD) return p->N == 1 ? 1 : fx_passes((uint32_t)p->N);
     if (is_fixed(p->form))
         return p->fx_panel ? p->fx_npass + 2 : fx_stages((uint32_t)p->N);
     return is_mp(p->form) ? 1 + (int)p->levels.size() : MPFFT_ERR_FORM;
@@ -1028,7 +1092,7 @@ int mpfft_plan_last_timing(mpfft_plan p, double *ms)
 int mpfft_execute_f64(mpfft_plan p, const double *d_in, double *d_out, void *stream)
 {
     if (!p || !d_in || !d_out) return MPFFT_ERR_ARG;
-    if (is_mp(p->form)) return MPFFT_ERR_FORM;
+    if (is_limb_form(p->form)) return MPFFT_ERR_FORM;
     HIP_TRY(hipSetDevice(p->device));
     hipError_t e = d_fft_launch(p->dt, d_in, d_out, p->words, p->batch,
                                 stream ? (hipStream_t)stream : p->stream);
@@ -1038,7 +1102,7 @@ int mpfft_execute_f64(mpfft_plan p, const double *d_in, double *d_out, void *str
 int mpfft_execute_f64_host(mpfft_plan p, const double *h_in, double *h_out)
 {
     if (!p || !h_in || !h_out) return MPFFT_ERR_ARG;
-    if (is_mp(p->form)) return MPFFT_ERR_FORM;
+    if (is_limb_form(p->form)) return MPFFT_ERR_FORM;
     std::lock_guard<std::mutex> g(p->mu);
     HIP_TRY(hipSetDevice(p->device));
     int rc = ensure_stage(p);
