@@ -34,7 +34,6 @@ struct FxDig {
     // part is below 2^64 * 2^(29 (K0 - 1)) * (1 + 2^-28) < 2^(29 K0 + 35.01)
     // <= 2^(F - 4.99): under 1/32 of the output's last unit
     static constexpr int K0 = (F - 40) >= 0 ? (F - 40) / kDigitBits : 0;
-    static_assert(DX <= 63 && DT <= 63, "column sums would overflow 64 bits");
 };
 
 template <int W>
@@ -100,6 +99,9 @@ template <int W>
 MPFFT_FN void fx_mul(Fx<W> &z, Fx<W> &x, const uint32_t (&T)[FxDig<W>::DT], uint32_t tneg)
 {
     constexpr int F = FxDig<W>::F, DX = FxDig<W>::DX, DT = FxDig<W>::DT, K0 = FxDig<W>::K0;
+    // the per-lane product exists up to W = 34 (the panel kernels, whose
+    // product runs on the matrix cores, also take wider reals)
+    static_assert(DX <= 63 && DT <= 63, "column sums would overflow 64 bits");
     const uint32_t xneg = fx_sign(x);
     fx_abs(x, x, xneg);
     uint32_t A[DX];

@@ -50,6 +50,10 @@ namespace mpfft {
 
 namespace {
 
+// blocks per CU of the radix-4 pass kernel: reals wider than 34 words (up to
+// 66, P = 2048) need the whole register file and 4 LDS slots of 34 KB
+constexpr int fxp_occ(int W) { return W > 34 ? 1 : FXP_OCC; }
+
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
@@ -238,9 +242,14 @@ __device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *R
             A2[dd] = REAL ? A1[dd] : mf_atile(R2, off0 - 32 * dd);
         }
 #endif
+        // |S| <= 2 (4W - 3) 2^14 (two products of balanced digits): the
+        // 32-bit form of the carry step below holds up to W = 34; wider reals
+        // take 64-bit sums
+        constexpr bool kNarrow = 257LL * 2 * (4 * W - 3) * 16384 + (1LL << 16) < (1LL << 31);
         int32_t carry = 0;
         uint32_t *out = slot + (uint32_t)h * W * 64 + lane;
-#pragma unroll
+        // wide reals: the output-tile loop stays rolled (register pressure)
+#pragma unroll (kNarrow ? MT : 1)
         for (int p = 0; p < MT; p++) {
             v16i D0 = {}, D1 = {};
 #pragma unroll
@@ -284,6 +293,13 @@ __device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *R
                 const int t = 8 * p + u - 1;
                 if (t >= W) break;
                 const int32_t *S = (u & 1) ? &S1[4 * (u >> 1)] : &S0[4 * (u >> 1)];
+                if constexpr (!kNarrow) {
+                    const int64_t V = (int64_t)S[0] + ((int64_t)S[1] << 8) +
+                                      ((int64_t)S[2] << 16) + ((int64_t)S[3] << 24) + carry;
+                    if (t >= 0) out[(uint32_t)t * 64] = (uint32_t)V;
+                    carry = (int32_t)(V >> 32);
+                    continue;
+                }
                 // V = S0 + 2^8 S1 + 2^16 S2 + 2^24 S3 + carry = P0 + 2^16 P1
                 const int32_t P0 = (int32_t)((uint32_t)S[0] + ((uint32_t)S[1] << 8)) + carry;
                 const int32_t P1 = (int32_t)((uint32_t)S[2] + ((uint32_t)S[3] << 8));
@@ -361,7 +377,7 @@ fxp_r2_kernel(FxArgs a, const uint32_t *src, uint32_t *dst)
 // two sets (alternating items) of four slots of 2 W words x 64 lanes; one
 // barrier per item.
 template <int W>
-__global__ void __launch_bounds__(256, FXP_OCC)
+__global__ void __launch_bounds__(256, fxp_occ(W))
 fxp_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t toff,
               uint32_t nitems)
 {
@@ -398,7 +414,8 @@ fxp_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
         uint32_t *set = lds + (FXP_SETS == 2 ? (n & 1u) : 0u) * 4 * SLOT;
         if (FXP_SETS == 1 && n > 0) __syncthreads();   // the slots of the last item are read
         uint32_t *slot = set + wv * SLOT;
-        uint32_t *R = slot + SLOT - TWN;          // twiddle tables at the end of the slot
+        // twiddle tables at the end of the slot, or (wide reals) after the slots
+        uint32_t *R = mf_tail_fits(W) ? slot + SLOT - TWN : lds + FXP_SETS * 4 * SLOT + wv * TWN;
         const bool prod = r != 0 && k != 0;
         // consume this item's operands before the next item's loads are
         // issued: loads and stores share one in-order counter (at most 63
@@ -416,13 +433,16 @@ fxp_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
 #pragma unroll
             for (int i = 0; i < W; i++) slot[(W + i) * 64 + lane] = im.w[i];
         }
-        if (nx < nitems) fetch(nx, re, im, tw);
+        // wide reals (W > 34) hold no prefetched operands across the product
+        constexpr bool kPrefetch = W <= 34;
+        if (kPrefetch && nx < nitems) fetch(nx, re, im, tw);
         if (prod) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             mf_cmul<W>(B, R, R + G.RW, R + 2 * G.RW, lane, slot);
         }
+        if (!kPrefetch && nx < nitems) fetch(nx, re, im, tw);
         __syncthreads();
         // DFT4, output y_r of this wave; v_s sits in the slot of wave
         // (s - rot) mod 4
@@ -452,7 +472,9 @@ fxp_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
 //                                                     by 1/sqrt(2), matrix cores)
 //   y_(q0 + 2 q1) = sum_r1 (-i)^(r1 q1) u'_(r1, q0)  (DFT4 of four slots)
 // LDS: eight slots of 2 W words x 64 lanes (139 KB at W = 34, one block per
-// CU, two waves per SIMD) and the table of 1/sqrt(2).
+// CU, two waves per SIMD), the table of 1/sqrt(2) and, where a product's
+// twiddle tables do not fit at the end of a slot (W = 8), one table area per
+// wave.
 template <int W>
 __global__ void __launch_bounds__(512, 1)
 fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t toff,
@@ -493,7 +515,8 @@ fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
         // slot of the wave holding input s: wave (s - rot) mod 8
         auto sl = [&](uint32_t s_) { return lds + ((s_ - rot) & 7u) * SLOT + lane; };
         uint32_t *slot = lds + wv * SLOT;
-        uint32_t *R = slot + SLOT - TWN;
+        // twiddle tables at the end of the slot, or after the slots and c8t
+        uint32_t *R = mf_tail_fits(W) ? slot + SLOT - TWN : lds + 8 * SLOT + G.RW + wv * TWN;
         __syncthreads();   // the slots of the last item are read (first item: c8t staged)
         const bool prod = r != 0 && k != 0;
         MfOperand<W> B;
@@ -645,13 +668,16 @@ static hipError_t launch_fxp(const FxArgs &a, int s0, int s1, hipStream_t st)
                 hipLaunchKernelGGL((fxp_r2_kernel<W>), dim3((N / 2) * NG / 4),
                                    dim3(256), 0, st, a, (const uint32_t *)src, dst);
             } else if (P.R == 4) {
-                const size_t sh = sizeof(uint32_t) * FXP_SETS * 4 * 2 * W * 64;
+                const size_t sh = sizeof(uint32_t) * (FXP_SETS * 4 * 2 * W * 64 +
+                                                      4 * mf_table_words(W));
                 const uint32_t nitems = (N / 4) * NG;
-                const uint32_t nblk = nitems < FXP_OCC * a.ncu ? nitems : FXP_OCC * a.ncu;
+                const uint32_t occ = fxp_occ(W);
+                const uint32_t nblk = nitems < occ * a.ncu ? nitems : occ * a.ncu;
                 hipLaunchKernelGGL((fxp_r4_kernel<W>), dim3(nblk), dim3(256), sh, st, a,
                                    (const uint32_t *)src, dst, P.Ns, P.toff, nitems);
             } else if constexpr (W <= kPanelR8MaxW) {
-                const size_t sh = sizeof(uint32_t) * (8 * 2 * W * 64 + G.RW);
+                const size_t sh =
+                    sizeof(uint32_t) * (8 * 2 * W * 64 + G.RW + 8 * mf_table_words(W));
                 const uint32_t nitems = (N / 8) * NG;
                 const uint32_t nblk = nitems < a.ncu ? nitems : a.ncu;
                 hipLaunchKernelGGL((fxp_r8_kernel<W>), dim3(nblk), dim3(512), sh, st, a,
@@ -686,6 +712,7 @@ hipError_t fxp_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStre
     case 16: return CALLP(16);
     case 24: return CALLP(24);
     case 32: return CALLP(32);
+    case 64: return CALLP(64);
     default: return hipErrorInvalidValue;
     }
 #undef CALLP

@@ -161,6 +161,18 @@ constexpr MfGeom mf_geom(int W)
     return g;
 }
 
+// A product's three twiddle digit tables (c, d, -d: 3 RW words) may sit at
+// the end of the wave's LDS slot when they overlap only im' words that the
+// last product tile writes (im' word t at slot words (W + t) 64 ..; tile p
+// writes words t = 8p - 1 .. 8p + 6), i.e. up to W = 34; wider reals keep
+// them in a per-wave area after the slots.
+constexpr bool mf_tail_fits(int W)
+{
+    return 3 * mf_geom(W).RW <= (W - 8 * (mf_geom(W).MT - 1) + 1) * 64;
+}
+
+constexpr int mf_table_words(int W) { return mf_tail_fits(W) ? 0 : 3 * mf_geom(W).RW; }
+
 // the panel kernels run any batch at N >= 16
 inline bool fxp_eligible(uint32_t N, uint32_t batch)
 {

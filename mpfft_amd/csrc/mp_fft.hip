@@ -395,7 +395,8 @@ static hipError_t launch_binop(int op, uint32_t count, const uint32_t *x,
 
 // limb counts with compiled kernels; a precision uses the smallest one that
 // holds it
-static const int kLimbs[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32};
+// (64 limbs: the fixed form on panels only; every other form stops at 32)
+static const int kLimbs[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 64};
 
 int limbs_for_precision(long P)
 {

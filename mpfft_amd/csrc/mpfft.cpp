@@ -29,7 +29,8 @@ namespace {
 
 thread_local int g_last_error = MPFFT_OK;
 
-constexpr long kMaxPrec = 1024;          // largest compiled limb count * 32
+constexpr long kMaxPrec = 2048;          // largest compiled limb count * 32 (panels)
+constexpr long kMaxLimbsAll = 32;        // limb counts every kernel family compiles
 constexpr long kExpLimit = 1L << 29;     // accepted input exponent range
 constexpr unsigned long kMaxN = 1UL << 24;
 constexpr long kDdMinPrec = 53, kDdMaxPrec = 106;   // precisions of the double-word form
@@ -357,7 +358,8 @@ int build_fx_plan(mpfft_plan_s *p)
     if (p->form == MPFFT_FORM_FIXED_PANEL && !fxp_eligible((uint32_t)N, p->batch))
         return MPFFT_ERR_SIZE;
     p->fx_panel = p->form == MPFFT_FORM_FIXED_PANEL ||
-                  (p->form == MPFFT_FORM_FIXED && fxp_preferred((uint32_t)N, p->batch));
+                  (p->form == MPFFT_FORM_FIXED &&
+                   (fxp_preferred((uint32_t)N, p->batch) || p->L > kMaxLimbsAll));
     int ncu = 0;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device));
     p->ncu = ncu > 0 ? (uint32_t)ncu : 1u;
@@ -802,6 +804,11 @@ int mpfft_plan_create(mpfft_plan *plan, unsigned long n, long prec, int form, un
         p->P = prec;
         p->L = limbs_for_precision(prec);
         p->words = 2 * n * (size_t)(p->L + 1);
+        // beyond 32 limbs only the panel kernels of the fixed form exist
+        if (p->L > kMaxLimbsAll &&
+            !((form == MPFFT_FORM_FIXED || form == MPFFT_FORM_FIXED_PANEL) &&
+              fxp_eligible((uint32_t)n, batch)))
+            return MPFFT_ERR_PREC;
         rc = form == MPFFT_FORM_DD ? build_dd_plan(p.get())
              : is_fixed(form)      ? build_fx_plan(p.get())
                                    : build_mp_plan(p.get());
@@ -1122,7 +1129,7 @@ int mpfft_mp_binop_host(int op, unsigned long count, long prec, const uint32_t *
     if (!x || !y || !z || op < 0 || op > 2 || count == 0 || count > (1UL << 26))
         return MPFFT_ERR_ARG;
     const long L = mpfft_limbs_for_precision(prec);
-    if (L < 1) return MPFFT_ERR_PREC;
+    if (L < 1 || L > kMaxLimbsAll) return MPFFT_ERR_PREC;
     const size_t bytes = sizeof(uint32_t) * count * (L + 1);
     DevBuf dx, dy, dz;
     int rc;

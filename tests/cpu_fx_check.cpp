@@ -194,6 +194,8 @@ int main(int argc, char **argv)
     check_conv<8, true>(trials, 256);
     check_conv<16, false>(trials, 500);
     check_conv<32, true>(trials, 1024);
+    check_conv<64, true>(trials, 2048);          // the panel kernels' 66-word reals
+    check_conv<64, false>(trials, 1500);
     printf(fails ? "FAIL\n" : "PASS\n");
     return fails ? 1 : 0;
 }

@@ -37,7 +37,9 @@ def test_limbs_for_precision():
     assert M.limbs_for_precision(1024) == 32
     assert M.limbs_for_precision(160) == 6      # padded to a compiled count
     assert M.limbs_for_precision(1) == -1
-    assert M.limbs_for_precision(1025) == -1
+    assert M.limbs_for_precision(1025) == 64     # the panel kernels' 66-word reals
+    assert M.limbs_for_precision(2048) == 64
+    assert M.limbs_for_precision(2049) == -1
 
 
 def test_strerror():

@@ -55,7 +55,11 @@ def check_against_oracle(n, P, x, y):
 
 @pytest.mark.parametrize("n,P", [(16, 1024), (32, 256), (64, 53), (128, 32), (256, 64),
                                  (512, 100), (1024, 512), (2048, 1024), (4096, 128),
-                                 (8192, 256)])
+                                 (8192, 256),
+                                 # every other compiled width: W = 5, 8, 14, 26 (W = 8:
+                                 # the twiddle tables live outside the slots)
+                                 (1024, 96), (2048, 192), (512, 160), (1024, 384),
+                                 (256, 768)])
 def test_panel_matches_lane_and_oracle(n, P):
     xs = inputs(n, P, 64, seed=9000 + n + P)
     yp = run(n, P, xs, M.FORM_FIXED)
@@ -170,3 +174,43 @@ def test_panel_large_n():
     check_against_lane(n, P, yp, yl)
     for i in (0, 63):
         check_against_oracle(n, P, xs[i], yp[i])
+
+
+@pytest.mark.parametrize("n,P,b", [(16, 2048, 64), (256, 1500, 64), (4096, 2048, 3)])
+def test_panel_2048_bits(n, P, b):
+    # reals of 66 words (P up to 2048, 64 limbs): the panel kernels only;
+    # against the 2P-bit oracle, no worse than the plain order (the oracle's
+    # restatement of the reference at P bits; the GPU plain form stops at
+    # 1024 bits)
+    xs = inputs(n, P, b, seed=2048 + n)
+    plan = M.Plan(n, P, M.FORM_FIXED, b)
+    assert plan.L == 64
+    plan.close()
+    yp = run(n, P, xs, M.FORM_FIXED)
+    for i in sorted({0, b - 1}):
+        check_against_oracle(n, P, xs[i], yp[i])
+
+
+def test_panel_2048_bits_forms_checked():
+    # beyond 1024 bits every other form (and the panels below n = 16) refuses
+    for form in (M.FORM_MP, M.FORM_MP_C1, M.FORM_FIXED_LANE, M.FORM_DD):
+        with pytest.raises(M.MpfftError) as e:
+            M.Plan(64, 2048, form, 1)
+        assert e.value.code == -2
+    with pytest.raises(M.MpfftError) as e:
+        M.Plan(8, 2048, M.FORM_FIXED, 64)
+    assert e.value.code == -2
+    with pytest.raises(M.MpfftError) as e:
+        M.Plan(64, 2049, M.FORM_FIXED, 64)
+    assert e.value.code == -2
+
+
+def test_panel_2048_bits_large_n_impulse():
+    # N = 2^20 at 2048 bits, one transform (a padded panel): x = delta_0
+    # gives exactly 1 everywhere (ten passes of zeros added and multiplied;
+    # the random-input error contract is checked against the oracle above)
+    n, P = 1 << 20, 2048
+    y = run(n, P, [impulse(n, P, 0)], M.FORM_FIXED)[0]
+    h, lm = I.split(y, n, O.limbs(P))
+    assert (h[:n] == 2).all() and (lm[0, :n] == 0x80000000).all() and (lm[1:, :n] == 0).all()
+    assert ((h[n:] & ~1) == np.int32(-2**31)).all() and (lm[:, n:] == 0).all()
