@@ -12,7 +12,8 @@ baseline and the drop-in call from mpfr_t arrays.
                     [--batched-count C]
 
 Beside the headline line: the double-word FP64 form at 106 bits against
-the plain and fixed forms at 128 bits ("precision_form"), and the batched
+the plain and fixed forms at 128 bits ("precision_form"), N = 2^20 at 512
+and 2048 bits ("wide"), and the batched
 workload of C transforms of uneven size and precision ("batched"), one
 graph launch of the batched forms against the plain form run one transform
 at a time.
@@ -157,6 +158,44 @@ def measure_precision_form(M, I, torch, O, n, P, Pcmp, batch, steps, seed):
     return out
 
 
+def measure_wide(M, I, torch, lgn, steps, seed):
+    """N = 2^lgn at 512 bits (fixed form on panels and plain form, batch 64)
+    and at 2048 bits (fixed form on panels, the only form there; batch 1 in
+    a padded panel).  One generated transform is copied into every slot of
+    the batch on the device (the kernels' time does not depend on the
+    values)."""
+    n = 1 << lgn
+    out = {"workload": f"N=2^{lgn}: 512 bits batch 64 (fixed, plain), 2048 bits batch 1 (fixed)"}
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    for P, B, forms in ((512, 64, (("fixed", M.FORM_FIXED), ("plain", M.FORM_MP))),
+                        (2048, 1, (("fixed", M.FORM_FIXED),))):
+        x1 = I.mp_random(n, P, seed=seed)
+        for name, form in forms:
+            plan = M.Plan(n, P, form, B)
+            one = torch.from_numpy(I.relimb(x1, n, I.limbs_for(P), plan.L).view(np.int32)).cuda()
+            x = one.repeat(B)
+            del one
+            y = torch.empty_like(x)
+            plan.execute_device(x.data_ptr(), y.data_ptr(), sp)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(steps):
+                plan.execute_device(x.data_ptr(), y.data_ptr(), sp)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            plan.check()
+            ms = e0.elapsed_time(e1) / steps
+            out[f"{name}_P{P}"] = {"batch": B, "ms_per_step": round(ms, 3),
+                                   "fft_per_s": round(B / ms * 1e3, 2)}
+            plan.close()
+            del x, y
+    out["fixed_vs_plain_P512"] = round(out["fixed_P512"]["fft_per_s"] /
+                                       out["plain_P512"]["fft_per_s"], 3)
+    return out
+
+
 def measure_mpfr(M, I, n, P, seed, reps=3):
     """the drop-in path: one transform from and to mpfr_t arrays
     (marshalling, PCIe transfers and the GPU transform), per call"""
@@ -280,6 +319,8 @@ def main():
     ap.add_argument("--dd-prec", type=int, default=106,
                     help="precision of the double-word form leg (0 = skip); compared with "
                          "the plain and fixed forms at 128 bits")
+    ap.add_argument("--wide-lgn", type=int, default=20,
+                    help="log2 N of the wide-precision leg (512 and 2048 bits; 0 = skip)")
     ap.add_argument("--batched-count", type=int, default=4096,
                     help="transforms of the batched uneven-(n, P) workload (0 = skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -434,6 +475,8 @@ def main():
     if a.dd_prec:
         line["precision_form"] = measure_precision_form(M, I, torch, O, n, a.dd_prec, 128, B,
                                                         max(2, a.steps // 2), a.seed)
+    if a.wide_lgn:
+        line["wide"] = measure_wide(M, I, torch, a.wide_lgn, 2, a.seed)
     if a.batched_count:
         line["batched"] = measure_batched(M, I, torch, a.seed, a.batched_count,
                                           max(2, a.steps // 2))
