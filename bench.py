@@ -110,7 +110,8 @@ def measure(M, I, torch, n, P, batch, steps, warmup, seed, form):
     sp = stream.cuda_stream
     S = plan.stages()
     pad = -batch % 64
-    panel = n >= 16 and (form == M.FORM_FIXED_PANEL or (form == M.FORM_FIXED and 8 * pad <= batch))
+    panel = n >= 16 and (form == M.FORM_FIXED_PANEL or (form == M.FORM_FIXED and (
+        4 * pad <= batch or (plan.L + 2 >= 34 and n <= 1024))))
     cuts = [0, S] if form in M.FIXED_FORMS and not panel else ([0, 1, S - 1, S] if panel else [0, 1, S])
     for _ in range(warmup):
         plan.execute_range(x.data_ptr(), y.data_ptr(), 0, S, sp)

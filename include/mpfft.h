@@ -96,8 +96,9 @@ typedef struct mpfft_plan_s *mpfft_plan;
                               the reference's bit pattern; |input| < 2^(29 -
                               log2 n), checked by mpfft_plan_check.  At
                               n >= 16 and a batch that fills its panels of
-                              64 transforms to within an eighth (any
-                              multiple of 64, or 56 of 64, ...) as
+                              64 transforms to within a quarter (any
+                              multiple of 64, or 48 of 64, ...; above 768
+                              bits with n <= 1024: any batch) as
                               MPFFT_FORM_FIXED_PANEL; otherwise as
                               MPFFT_FORM_FIXED_LANE                        */
 #define MPFFT_FORM_FIXED_LANE 9 /* the fixed form with per-lane twiddle
@@ -203,8 +204,8 @@ int mpfft_plan_check(mpfft_plan plan);
  * Transforms of equal (n, prec) form a group that runs as one plan of that
  * group's batch; the groups run on up to four streams inside one HIP graph,
  * captured once per (d_in, d_out) pair and replayed by every execute.  With
- * MPFFT_FORM_FIXED a group runs on the panel kernels where its count fills
- * its panels to within an eighth, else on the per-lane kernels.  The
+ * MPFFT_FORM_FIXED a group runs on the panel or the per-lane kernels as an
+ * MPFFT_FORM_FIXED plan of its size and count would.  The
  * autotuner times every group alone, deals the groups over 1, 2 and 4
  * streams by those times and keeps the fastest graph.
  * (No counterpart in the reference, which runs one transform per call.) */

@@ -9,9 +9,9 @@
 // in each launch).  The groups are dealt over up to kMaxStreams streams,
 // captured into one graph (fork from the batch's stream, join back) and
 // replayed: one launch per execute, independent groups overlapping on the
-// device.  A fixed-form group takes the panel kernels where its count fills
-// its panels to within an eighth (as MPFFT_FORM_FIXED plans do), else the
-// per-lane kernels; the autotuner times every group on the caller's
+// device.  A fixed-form group takes the panel or the per-lane kernels by the
+// rule MPFFT_FORM_FIXED plans use (mpfft::fxp_preferred, from measured
+// crossovers); the autotuner times every group on the caller's
 // buffers, then the graph with the groups dealt over 1, 2 and 4 streams, and
 // keeps the fastest.  Built only on the public plan API.
 #include "../../include/mpfft.h"
@@ -242,7 +242,7 @@ int mpfft_batch_create(mpfft_batch *batch, unsigned int count, const unsigned lo
         off += g.words * g.idx.size();
         const uint32_t cnt = (uint32_t)g.idx.size();
         if (form == MPFFT_FORM_FIXED)      // (reals beyond 32 limbs: panels only)
-            g.form = mpfft::fxp_preferred((uint32_t)g.N, cnt) || g.L > 32
+            g.form = mpfft::fxp_preferred((uint32_t)g.N, cnt, (int)g.L + 2) || g.L > 32
                          ? MPFFT_FORM_FIXED_PANEL
                          : MPFFT_FORM_FIXED_LANE;
         else if (form == MPFFT_FORM_FIXED_PANEL && !mpfft::fxp_eligible((uint32_t)g.N, cnt))

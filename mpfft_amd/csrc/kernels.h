@@ -179,12 +179,18 @@ inline bool fxp_eligible(uint32_t N, uint32_t batch)
     return N >= 16 && batch > 0;
 }
 
-// MPFFT_FORM_FIXED takes the panels when the padding of the last panel is at
-// most an eighth of the batch (the panels are ~1.15-1.2x faster than the
-// per-lane kernels at full panels)
-inline bool fxp_preferred(uint32_t N, uint32_t batch)
+// MPFFT_FORM_FIXED takes the panels (reals of W words) when the padding of
+// the last panel is at most a quarter of the batch, and at W >= 34 for N <=
+// 1024 at any batch.  Measured on MI355X, time per transform of the per-lane
+// kernels over the panels at N = 2^6 .. 2^14, P = 128 .. 1024, batches 16 ..
+// 128: 1.1-1.3 at full panels, falling with the padding to ~1 near a quarter
+// of the batch for N >= 2^12; at W = 34 and N <= 2^10 above 1 at every batch
+// (the per-lane kernels run few waves of very wide reals there).
+inline bool fxp_preferred(uint32_t N, uint32_t batch, int W)
 {
-    return fxp_eligible(N, batch) && 8 * (fxp_groups(batch) * kPanel - batch) <= batch;
+    if (!fxp_eligible(N, batch)) return false;
+    if (W >= 34 && N <= 1024) return true;
+    return 4 * (fxp_groups(batch) * kPanel - batch) <= batch;
 }
 
 // Passes of the fixed form on panels.  Up to W = kPanelR8MaxW words per real:

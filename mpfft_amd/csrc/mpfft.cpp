@@ -359,7 +359,7 @@ int build_fx_plan(mpfft_plan_s *p)
         return MPFFT_ERR_SIZE;
     p->fx_panel = p->form == MPFFT_FORM_FIXED_PANEL ||
                   (p->form == MPFFT_FORM_FIXED &&
-                   (fxp_preferred((uint32_t)N, p->batch) || p->L > kMaxLimbsAll));
+                   (fxp_preferred((uint32_t)N, p->batch, (int)W) || p->L > kMaxLimbsAll));
     int ncu = 0;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device));
     p->ncu = ncu > 0 ? (uint32_t)ncu : 1u;

@@ -5,8 +5,8 @@ Contracts:
   - plain op orders (MPFFT_FORM_MP): every transform bit-identical to the
     oracle's restatement of the reference (serial_mp_fft3.c:121-253), as a
     plan of that form is;
-  - fixed form (a group on the panel kernels where its count fills its
-    panels to within an eighth, else per lane): every transform within 2 ulp
+  - fixed form (a group on the panel or the per-lane kernels as a fixed-form
+    plan of its size and count would run): every transform within 2 ulp
     of the per-lane kernels (the two differ only where the exact value lies
     next to a rounding boundary) and no worse than the plain form against
     the 2P-bit oracle;
