@@ -141,10 +141,13 @@ def measure_precision_form(M, I, torch, O, n, P, Pcmp, batch, steps, seed):
     """The double-word form (P <= 106 bits) beside the plain and the fixed
     form at Pcmp bits, all at N = n on `batch` device-resident transforms;
     each form's error against its own 2P-bit oracle on transform 0."""
-    out = {"workload": f"N={n}, batch {batch}: double-word form at {P} bits; plain and fixed "
-                       f"forms at {Pcmp} bits"}
+    out = {"workload": f"N={n}, batch {batch}: double-word form at {P} bits, plain and fixed "
+                       f"forms at {Pcmp} bits; quad-word form at {2 * P} bits, plain and "
+                       f"fixed forms at {2 * Pcmp} bits"}
     for name, form, prec in (("double_word", M.FORM_DD, P), ("plain", M.FORM_MP, Pcmp),
-                             ("fixed", M.FORM_FIXED, Pcmp)):
+                             ("fixed", M.FORM_FIXED, Pcmp), ("quad_word", M.FORM_QD, 2 * P),
+                             ("plain_2x", M.FORM_MP, 2 * Pcmp),
+                             ("fixed_2x", M.FORM_FIXED, 2 * Pcmp)):
         r = measure(M, I, torch, n, prec, batch, steps, 1, seed, form)
         star = O.mp_fft(n, prec, r["x0"], Pw=2 * prec)
         mu, nw, zb = O.err_stats(n, prec, r["out0"], I.limbs_for(prec), star,
@@ -156,6 +159,10 @@ def measure_precision_form(M, I, torch, O, n, P, Pcmp, batch, steps, seed):
                                         out["plain"]["fft_per_s"], 2)
     out["double_word_vs_fixed"] = round(out["double_word"]["fft_per_s"] /
                                         out["fixed"]["fft_per_s"], 2)
+    out["quad_word_vs_plain_2x"] = round(out["quad_word"]["fft_per_s"] /
+                                         out["plain_2x"]["fft_per_s"], 2)
+    out["quad_word_vs_fixed_2x"] = round(out["quad_word"]["fft_per_s"] /
+                                         out["fixed_2x"]["fft_per_s"], 2)
     return out
 
 
@@ -318,8 +325,9 @@ def main():
     ap.add_argument("--no-err", action="store_true", help="skip the 2P-bit oracle error")
     ap.add_argument("--no-mpfr", action="store_true", help="skip the mpfr_t-array call timing")
     ap.add_argument("--dd-prec", type=int, default=106,
-                    help="precision of the double-word form leg (0 = skip); compared with "
-                         "the plain and fixed forms at 128 bits")
+                    help="precision of the double-word form leg (0 = skip; the quad-word "
+                         "form runs at twice it); compared with the plain and fixed forms "
+                         "at 128 (256) bits")
     ap.add_argument("--wide-lgn", type=int, default=20,
                     help="log2 N of the wide-precision leg (512 and 2048 bits; 0 = skip)")
     ap.add_argument("--batched-count", type=int, default=4096,

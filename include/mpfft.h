@@ -122,6 +122,14 @@ typedef struct mpfft_plan_s *mpfft_plan;
                               checked by mpfft_plan_check (MPFFT_ERR_VALUE);
                               normwise error against the exact transform
                               below 2^-prec + (2 + 3 log2 n) 2^-104       */
+#define MPFFT_FORM_QD 12  /* quad-word form, 53 <= prec <= 212: every real
+                              c0 + c1 + c2 + c3 (four binary64), sums and
+                              products by error-free transformations with
+                              the order-3 terms rounded and renormalised;
+                              same SoA-limb I/O, result rounded to prec
+                              bits; |exponent| of the input <= 800
+                              (MPFFT_ERR_VALUE); normwise error below
+                              2^-prec + (8 + 32 log2 n) 2^-212            */
 #define MPFFT_FORM_D5  5   /* double, CODE 5 op order (4m2a, 3 butterflies) */
 #define MPFFT_FORM_D6  6   /* double, CODE 6/7 op order (1 butterfly)      */
 

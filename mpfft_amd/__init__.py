@@ -19,8 +19,9 @@ FORM_FIXED = 8                          # fixed-point radix-4 form (same limb I/
 FORM_FIXED_LANE = 9                     # the fixed form on its per-lane kernels only
 FORM_FIXED_PANEL = 10                   # the fixed form on panels (any batch, n >= 16)
 FORM_DD = 11                            # double-word FP64 form, 53 <= P <= 106
+FORM_QD = 12                            # quad-word FP64 form, 53 <= P <= 212
 FIXED_FORMS = (FORM_FIXED, FORM_FIXED_LANE, FORM_FIXED_PANEL)
-MP_FORMS = (FORM_MP, FORM_MP_C1, FORM_MP_C2, FORM_DD) + FIXED_FORMS
+MP_FORMS = (FORM_MP, FORM_MP_C1, FORM_MP_C2, FORM_DD, FORM_QD) + FIXED_FORMS
 OK = 0
 ERRORS = {
     -1: "MPFFT_ERR_SIZE", -2: "MPFFT_ERR_PREC", -3: "MPFFT_ERR_VALUE",

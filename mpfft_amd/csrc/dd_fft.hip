@@ -1,6 +1,6 @@
-// Double-word form (MPFFT_FORM_DD): the FFT on unevaluated sums of two
-// binary64 numbers, hi + lo with |lo| <= ulp(hi) / 2, for precisions of 53 to
-// 106 bits.
+// Double-word and quad-word forms (MPFFT_FORM_DD, MPFFT_FORM_QD): the FFT on
+// unevaluated sums of K = 2 or 4 binary64 numbers, for precisions of 53 to
+// 106, resp. 212 bits.
 //
 // The reference has no such form; it computes every operation with MPFR at
 // P bits (lab/serial/mp/serial_mp_fft3.c:121-253).  Here each real is a
@@ -13,13 +13,15 @@
 // 44(2), 2017): DWPlusDW (their Algorithm 6, relative error below 3u^2) and
 // DWTimesDW with two FMA corrections (Algorithm 11, below 5u^2), u = 2^-53.
 //
+// Quad-word numbers c0 + c1 + c2 + c3 follow the same principles (see
+// below).
+//
 // Structure: radix-4 Stockham passes, a radix-2 pass first when log2 N is
 // odd, as the fixed form (fx_fft.hip).  The first pass reads the SoA-limb
-// input and splits every real into hi + lo exactly (P <= 106); the last pass
-// rounds hi + lo to P bits (correctly rounded: both parts are exact P-bit
-// values, and their sum is rounded once by the limb adder of mp_arith.h)
-// and writes SoA limbs.  In between, a real is one double2 {hi, lo}; element
-// e of transform b (re: e < N, im: N + e) is at b * stride + e.
+// input and splits every real into its words exactly; the last pass rounds
+// the words' sum to P bits with the limb adder of mp_arith.h (double-word:
+// both parts are exact P-bit values, so one correctly rounded addition) and
+// writes SoA limbs.  In between, a real is K/2 double2.
 #include "kernels.h"
 #include "mp_arith.h"
 
@@ -71,34 +73,6 @@ __device__ __forceinline__ DD dd_mul(const DD &x, const DD &y)
     e = __builtin_fma(x.hi, y.lo, e);
     e = __builtin_fma(x.lo, y.hi, e);
     return fast_two_sum(p, e);
-}
-
-__device__ __forceinline__ CDD c_add(const CDD &a, const CDD &b)
-{
-    return {dd_add(a.re, b.re), dd_add(a.im, b.im)};
-}
-
-__device__ __forceinline__ CDD c_sub(const CDD &a, const CDD &b)
-{
-    return {dd_sub(a.re, b.re), dd_sub(a.im, b.im)};
-}
-
-// a * w
-__device__ __forceinline__ CDD c_mul(const CDD &a, const CDD &w)
-{
-    return {dd_sub(dd_mul(a.re, w.re), dd_mul(a.im, w.im)),
-            dd_add(dd_mul(a.re, w.im), dd_mul(a.im, w.re))};
-}
-
-// a - i b and a + i b
-__device__ __forceinline__ CDD c_sub_ib(const CDD &a, const CDD &b)
-{
-    return {dd_add(a.re, b.im), dd_sub(a.im, b.re)};
-}
-
-__device__ __forceinline__ CDD c_add_ib(const CDD &a, const CDD &b)
-{
-    return {dd_sub(a.re, b.im), dd_add(a.im, b.re)};
 }
 
 // P-bit limb value (53 <= P <= 106) -> hi + lo, exact.  The significand's
@@ -165,124 +139,347 @@ __device__ __forceinline__ void dd_to_mp(Mp<L> &z, const DD &x, int P)
     mp_add<L, FULL>(z, a, 0u, b, 0u, P);
 }
 
-// element e of transform b: from the limb input (IN) or a double2 buffer
-template <int L, bool IN>
-__device__ __forceinline__ CDD dd_load(const FxArgs &a, const double2 *src, uint32_t b,
-                                       uint32_t e)
+// ---- quad-word numbers (K = 4) ---------------------------------------------
+//
+// c0 + c1 + c2 + c3, each word at most about half an ulp of the one before.
+// Products and sums collect their partial terms by order (a term of order k
+// is ~u^k times the largest), carry every rounding error of orders 0 .. 2
+// exactly to the next order with TwoSum / the FMA residual, round the order-3
+// sum, and renormalise: relative error of order u^4 ~ 2^-212 per operation
+// (a few units of it), as for the usual quad-word algorithms.
+
+struct QD {
+    double c[4];
+};
+
+struct CQD {
+    QD re, im;
+};
+
+// five overlapping words (roughly decreasing) -> four: a bottom-up TwoSum
+// pass leaves the rounded total in c0 and each step's error below it, a
+// top-down pass pushes what each word holds beyond half an ulp of the one
+// above into the next
+__device__ __forceinline__ QD q_renorm(double c0, double c1, double c2, double c3, double c4)
 {
-    CDD v;
+    DD s = two_sum(c3, c4);
+    c3 = s.hi; c4 = s.lo;
+    s = two_sum(c2, c3);
+    c2 = s.hi; c3 = s.lo;
+    s = two_sum(c1, c2);
+    c1 = s.hi; c2 = s.lo;
+    s = two_sum(c0, c1);
+    c0 = s.hi; c1 = s.lo;
+    s = two_sum(c1, c2);
+    c1 = s.hi; c2 = s.lo;
+    s = two_sum(c2, c3);
+    c2 = s.hi; c3 = s.lo;
+    return {{c0, c1, c2, c3 + c4}};
+}
+
+__device__ __forceinline__ QD q_add(const QD &a, const QD &b)
+{
+    const DD s0 = two_sum(a.c[0], b.c[0]);            // order 0 (+ its error, order 1)
+    const DD s1 = two_sum(a.c[1], b.c[1]);            // order 1
+    const DD s2 = two_sum(a.c[2], b.c[2]);            // order 2
+    const DD t1 = two_sum(s1.hi, s0.lo);              // order 1 total
+    const DD t2 = two_sum(s2.hi, s1.lo);              // order 2 ...
+    const DD u2 = two_sum(t2.hi, t1.lo);              // ... total
+    const double o3 = ((a.c[3] + b.c[3]) + s2.lo) + (t2.lo + u2.lo);
+    return q_renorm(s0.hi, t1.hi, u2.hi, o3, 0.0);
+}
+
+__device__ __forceinline__ QD q_neg(const QD &x) { return {{-x.c[0], -x.c[1], -x.c[2], -x.c[3]}}; }
+
+__device__ __forceinline__ QD q_sub(const QD &a, const QD &b) { return q_add(a, q_neg(b)); }
+
+__device__ __forceinline__ DD two_prod(double a, double b)
+{
+    const double p = a * b;
+    return {p, __builtin_fma(a, b, -p)};
+}
+
+__device__ __forceinline__ QD q_mul(const QD &a, const QD &b)
+{
+    const DD p00 = two_prod(a.c[0], b.c[0]);
+    const DD p01 = two_prod(a.c[0], b.c[1]), p10 = two_prod(a.c[1], b.c[0]);
+    const DD p02 = two_prod(a.c[0], b.c[2]), p11 = two_prod(a.c[1], b.c[1]),
+             p20 = two_prod(a.c[2], b.c[0]);
+    // order 1: p00.lo + p01 + p10
+    const DD s1 = two_sum(p01.hi, p10.hi);
+    const DD s1b = two_sum(s1.hi, p00.lo);
+    // order 2: p02 + p11 + p20 + the order-1 residuals, every addition exact
+    // (a rounded sum of order-2 terms would lose an order-3 error)
+    const DD s2 = two_sum(p02.hi, p11.hi);
+    const DD s2b = two_sum(s2.hi, p20.hi);
+    const DD s2c = two_sum(s2b.hi, p01.lo);
+    const DD s2d = two_sum(s2c.hi, p10.lo);
+    const DD s2e = two_sum(s2d.hi, s1.lo);
+    const DD s2f = two_sum(s2e.hi, s1b.lo);
+    // order 3: the remaining products and residuals, rounded
+    double o3 = __builtin_fma(a.c[0], b.c[3], a.c[1] * b.c[2]);
+    o3 = __builtin_fma(a.c[2], b.c[1], o3);
+    o3 = __builtin_fma(a.c[3], b.c[0], o3);
+    o3 += ((p02.lo + p11.lo) + (p20.lo + s2.lo)) + ((s2b.lo + s2c.lo) + (s2d.lo + s2e.lo)) +
+          s2f.lo;
+    return q_renorm(p00.hi, s1b.hi, s2f.hi, o3, 0.0);
+}
+
+// ---- word-type traits: one real of K binary64 words ---------------------------
+
+template <int K> struct Word;
+template <> struct Word<2> {
+    using R = DD;
+    using C = CDD;
+    static __device__ __forceinline__ R add(const R &x, const R &y) { return dd_add(x, y); }
+    static __device__ __forceinline__ R sub(const R &x, const R &y) { return dd_sub(x, y); }
+    static __device__ __forceinline__ R mul(const R &x, const R &y) { return dd_mul(x, y); }
+    static __device__ __forceinline__ R neg(const R &x) { return dd_neg(x); }
+    static __device__ __forceinline__ void get(R &x, const double2 *p) { x = {p[0].x, p[0].y}; }
+    static __device__ __forceinline__ void put(double2 *p, const R &x)
+    {
+        p[0] = make_double2(x.hi, x.lo);
+    }
+};
+template <> struct Word<4> {
+    using R = QD;
+    using C = CQD;
+    static __device__ __forceinline__ R add(const R &x, const R &y) { return q_add(x, y); }
+    static __device__ __forceinline__ R sub(const R &x, const R &y) { return q_sub(x, y); }
+    static __device__ __forceinline__ R mul(const R &x, const R &y) { return q_mul(x, y); }
+    static __device__ __forceinline__ R neg(const R &x) { return q_neg(x); }
+    static __device__ __forceinline__ void get(R &x, const double2 *p)
+    {
+        const double2 u = p[0], v = p[1];
+        x = {{u.x, u.y, v.x, v.y}};
+    }
+    static __device__ __forceinline__ void put(double2 *p, const R &x)
+    {
+        p[0] = make_double2(x.c[0], x.c[1]);
+        p[1] = make_double2(x.c[2], x.c[3]);
+    }
+};
+
+template <int K> using RW_ = typename Word<K>::R;
+template <int K> using CW_ = typename Word<K>::C;
+
+template <int K>
+__device__ __forceinline__ CW_<K> w_add(const CW_<K> &a, const CW_<K> &b)
+{
+    return {Word<K>::add(a.re, b.re), Word<K>::add(a.im, b.im)};
+}
+
+template <int K>
+__device__ __forceinline__ CW_<K> w_sub(const CW_<K> &a, const CW_<K> &b)
+{
+    return {Word<K>::sub(a.re, b.re), Word<K>::sub(a.im, b.im)};
+}
+
+template <int K>
+__device__ __forceinline__ CW_<K> w_mul(const CW_<K> &a, const CW_<K> &w)
+{
+    return {Word<K>::sub(Word<K>::mul(a.re, w.re), Word<K>::mul(a.im, w.im)),
+            Word<K>::add(Word<K>::mul(a.re, w.im), Word<K>::mul(a.im, w.re))};
+}
+
+template <int K>
+__device__ __forceinline__ CW_<K> w_sub_ib(const CW_<K> &a, const CW_<K> &b)   // a - i b
+{
+    return {Word<K>::add(a.re, b.im), Word<K>::sub(a.im, b.re)};
+}
+
+template <int K>
+__device__ __forceinline__ CW_<K> w_add_ib(const CW_<K> &a, const CW_<K> &b)   // a + i b
+{
+    return {Word<K>::sub(a.re, b.im), Word<K>::add(a.im, b.re)};
+}
+
+// P-bit limb value -> quad-word, exact for P <= 212: the significand cut
+// into four 53-bit pieces (all of the value's sign), then renormalised.
+// Returns 1 when |exponent| > lim (the last piece would leave the normal
+// binary64 range).
+template <int L>
+__device__ __forceinline__ uint32_t qd_from_mp(QD &z, const Mp<L> &x, int lim)
+{
+    z = {{0.0, 0.0, 0.0, 0.0}};
+    if (x.zero) {
+        z.c[0] = x.neg ? -0.0 : 0.0;
+        return 0;
+    }
+    if (x.e > lim || x.e < -lim) return 1;
+    double c[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        // 53 bits from significand bit 53 i on (bit 0 = the leading one)
+        const int s = 53 * i, li = s >> 5, off = s & 31;
+        const uint64_t w0 = li < L ? x.m[li] : 0u, w1 = li + 1 < L ? x.m[li + 1] : 0u,
+                       w2 = li + 2 < L ? x.m[li + 2] : 0u;
+        uint64_t top = (w0 << 32) | w1;
+        if (off) top = (top << off) | (w2 >> (32 - off));
+        const double v = __builtin_ldexp((double)(top >> 11), x.e - 53 * (i + 1));
+        c[i] = x.neg ? -v : v;
+    }
+    z = q_renorm(c[0], c[1], c[2], c[3], 0.0);
+    return 0;
+}
+
+// c0 + c1 + c2 + c3 rounded to P bits: the lower words summed at the full
+// 32 L bits, then one rounding to P (P <= 212 < 32 L - 40 for L = 8; below,
+// the partial sums lie ~2^-53 under the result, so their rounding cannot
+// move the final one but at an exact tie)
+template <int L, bool FULL>
+__device__ __forceinline__ void qd_to_mp(Mp<L> &z, const QD &x, int P)
+{
+    Mp<L> a, b;
+    mp_from_double(a, x.c[2]);
+    mp_from_double(b, x.c[3]);
+    mp_add<L, true>(b, a, 0u, b, 0u, 32 * L);
+    mp_from_double(a, x.c[1]);
+    mp_add<L, true>(b, a, 0u, b, 0u, 32 * L);
+    mp_from_double(a, x.c[0]);
+    mp_add<L, FULL>(z, a, 0u, b, 0u, P);
+}
+
+template <int K, int L>
+__device__ __forceinline__ uint32_t w_from_mp(RW_<K> &z, const Mp<L> &x, int lim)
+{
+    if constexpr (K == 2) return dd_from_mp(z, x, lim);
+    else return qd_from_mp(z, x, lim);
+}
+
+template <int K, int L, bool FULL>
+__device__ __forceinline__ void w_to_mp(Mp<L> &z, const RW_<K> &x, int P)
+{
+    if constexpr (K == 2) dd_to_mp<L, FULL>(z, x, P);
+    else qd_to_mp<L, FULL>(z, x, P);
+}
+
+// ---- passes ------------------------------------------------------------------
+//
+// Element e of transform b (re: e < N, im: N + e) is K/2 double2 at
+// b * stride + (K/2) e.
+
+// element e of transform b: from the limb input (IN) or a word buffer
+template <int K, int L, bool IN>
+__device__ __forceinline__ CW_<K> w_load(const FxArgs &a, const double2 *src, uint32_t b,
+                                         uint32_t e)
+{
+    CW_<K> v;
     if constexpr (IN) {
         const uint32_t *in = a.in + a.in_stride * b;
         Mp<L> x;
         uint32_t bad = 0;
         mp_load(x, in, 2 * a.N, e);
-        bad |= dd_from_mp(v.re, x, a.lim);
+        bad |= w_from_mp<K, L>(v.re, x, a.lim);
         mp_load(x, in, 2 * a.N, a.N + e);
-        bad |= dd_from_mp(v.im, x, a.lim);
+        bad |= w_from_mp<K, L>(v.im, x, a.lim);
         if (bad) atomicOr(a.flag, 1u);
     } else {
         const double2 *p = src + a.stride * b;
-        const double2 r = p[e], i = p[a.N + e];
-        v.re = {r.x, r.y};
-        v.im = {i.x, i.y};
+        Word<K>::get(v.re, p + (K / 2) * e);
+        Word<K>::get(v.im, p + (K / 2) * (a.N + e));
     }
     return v;
 }
 
-template <int L, bool FULL, bool OUT>
-__device__ __forceinline__ void dd_store(const FxArgs &a, double2 *dst, uint32_t b, uint32_t e,
-                                         const CDD &v)
+template <int K, int L, bool FULL, bool OUT>
+__device__ __forceinline__ void w_store(const FxArgs &a, double2 *dst, uint32_t b, uint32_t e,
+                                        const CW_<K> &v)
 {
     if constexpr (OUT) {
         uint32_t *out = a.out + a.out_stride * b;
         Mp<L> z;
-        dd_to_mp<L, FULL>(z, v.re, a.P);
+        w_to_mp<K, L, FULL>(z, v.re, a.P);
         mp_store(z, out, 2 * a.N, e);
-        dd_to_mp<L, FULL>(z, v.im, a.P);
+        w_to_mp<K, L, FULL>(z, v.im, a.P);
         mp_store(z, out, 2 * a.N, a.N + e);
     } else {
         double2 *p = dst + a.stride * b;
-        p[e] = make_double2(v.re.hi, v.re.lo);
-        p[a.N + e] = make_double2(v.im.hi, v.im.lo);
+        Word<K>::put(p + (K / 2) * e, v.re);
+        Word<K>::put(p + (K / 2) * (a.N + e), v.im);
     }
 }
 
 // radix-2 pass without twiddles (the first pass when log2 N is odd)
-template <int L, bool FULL, bool IN, bool OUT>
+template <int K, int L, bool FULL, bool IN, bool OUT>
 __global__ void __launch_bounds__(256)
-dd_r2_kernel(FxArgs a, const double2 *src, double2 *dst)
+mw_r2_kernel(FxArgs a, const double2 *src, double2 *dst)
 {
     const uint32_t H = a.N >> 1, j = blockIdx.x * 256 + threadIdx.x, b = blockIdx.y;
     if (j >= H) return;
-    const CDD v0 = dd_load<L, IN>(a, src, b, j), v1 = dd_load<L, IN>(a, src, b, j + H);
-    dd_store<L, FULL, OUT>(a, dst, b, 2 * j, c_add(v0, v1));
-    dd_store<L, FULL, OUT>(a, dst, b, 2 * j + 1, c_sub(v0, v1));
+    const CW_<K> v0 = w_load<K, L, IN>(a, src, b, j), v1 = w_load<K, L, IN>(a, src, b, j + H);
+    w_store<K, L, FULL, OUT>(a, dst, b, 2 * j, w_add<K>(v0, v1));
+    w_store<K, L, FULL, OUT>(a, dst, b, 2 * j + 1, w_sub<K>(v0, v1));
 }
 
 // Radix-4 Stockham pass of sub-transform size Ns: thread j (< N/4) reads
 // v_r = x[j + r N/4], multiplies v_r by w^(r k N / (4 Ns)), k = j mod Ns
-// (table entry toff + 3k + r - 1: re then im, each a double2 {hi, lo}), and
-// writes y_q = sum_r v_r (-i)^(r q) to (j - k) 4 + k + q Ns.
-template <int L, bool FULL, bool IN, bool OUT>
+// (table entry t = toff + 3k + r - 1: K double2, the real part's K/2 then
+// the imaginary part's), and writes y_q = sum_r v_r (-i)^(r q) to
+// (j - k) 4 + k + q Ns.
+template <int K, int L, bool FULL, bool IN, bool OUT>
 __global__ void __launch_bounds__(256)
-dd_r4_kernel(FxArgs a, const double2 *src, double2 *dst, uint32_t Ns, uint32_t toff)
+mw_r4_kernel(FxArgs a, const double2 *src, double2 *dst, uint32_t Ns, uint32_t toff)
 {
     const uint32_t Q = a.N >> 2, j = blockIdx.x * 256 + threadIdx.x, b = blockIdx.y;
     if (j >= Q) return;
     const uint32_t k = j & (Ns - 1);
-    CDD v[4];
+    CW_<K> v[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) v[r] = dd_load<L, IN>(a, src, b, j + r * Q);
+    for (int r = 0; r < 4; r++) v[r] = w_load<K, L, IN>(a, src, b, j + r * Q);
     if (k) {
         const double2 *tw = reinterpret_cast<const double2 *>(a.tw);
 #pragma unroll
         for (int r = 1; r < 4; r++) {
-            const uint32_t t = toff + 3 * k + r - 1;
-            const double2 wr = tw[2 * t], wi = tw[2 * t + 1];
-            v[r] = c_mul(v[r], CDD{{wr.x, wr.y}, {wi.x, wi.y}});
+            const double2 *t = tw + (size_t)K * (toff + 3 * k + r - 1);
+            CW_<K> w;
+            Word<K>::get(w.re, t);
+            Word<K>::get(w.im, t + K / 2);
+            v[r] = w_mul<K>(v[r], w);
         }
     }
-    const CDD t0 = c_add(v[0], v[2]), t1 = c_sub(v[0], v[2]);
-    const CDD t2 = c_add(v[1], v[3]), t3 = c_sub(v[1], v[3]);
+    const CW_<K> t0 = w_add<K>(v[0], v[2]), t1 = w_sub<K>(v[0], v[2]);
+    const CW_<K> t2 = w_add<K>(v[1], v[3]), t3 = w_sub<K>(v[1], v[3]);
     const uint32_t o = (j - k) * 4 + k;
-    dd_store<L, FULL, OUT>(a, dst, b, o, c_add(t0, t2));
-    dd_store<L, FULL, OUT>(a, dst, b, o + Ns, c_sub_ib(t1, t3));
-    dd_store<L, FULL, OUT>(a, dst, b, o + 2 * Ns, c_sub(t0, t2));
-    dd_store<L, FULL, OUT>(a, dst, b, o + 3 * Ns, c_add_ib(t1, t3));
+    w_store<K, L, FULL, OUT>(a, dst, b, o, w_add<K>(t0, t2));
+    w_store<K, L, FULL, OUT>(a, dst, b, o + Ns, w_sub_ib<K>(t1, t3));
+    w_store<K, L, FULL, OUT>(a, dst, b, o + 2 * Ns, w_sub<K>(t0, t2));
+    w_store<K, L, FULL, OUT>(a, dst, b, o + 3 * Ns, w_add_ib<K>(t1, t3));
 }
 
 // N = 1: the transform is the identity; the input split and rounded back
-// (exact for P <= 106), with its range check
-template <int L, bool FULL>
+// (exact), with its range check
+template <int K, int L, bool FULL>
 __global__ void __launch_bounds__(256)
-dd_copy_kernel(FxArgs a)
+mw_copy_kernel(FxArgs a)
 {
     const uint32_t b = blockIdx.x * 256 + threadIdx.x;
     if (b >= a.batch) return;
-    dd_store<L, FULL, true>(a, nullptr, b, 0, dd_load<L, true>(a, nullptr, b, 0));
+    w_store<K, L, FULL, true>(a, nullptr, b, 0, w_load<K, L, true>(a, nullptr, b, 0));
 }
 
-template <int L, bool FULL, bool IN, bool OUT>
+template <int K, int L, bool FULL, bool IN, bool OUT>
 void launch_pass(const FxArgs &a, uint32_t R, const double2 *src, double2 *dst, uint32_t Ns,
                  uint32_t toff, hipStream_t st)
 {
     const uint32_t threads = a.N / R;
     const dim3 grid((threads + 255) / 256, a.batch);
     if (R == 2)
-        hipLaunchKernelGGL((dd_r2_kernel<L, FULL, IN, OUT>), grid, dim3(256), 0, st, a, src, dst);
+        hipLaunchKernelGGL((mw_r2_kernel<K, L, FULL, IN, OUT>), grid, dim3(256), 0, st, a, src,
+                           dst);
     else
-        hipLaunchKernelGGL((dd_r4_kernel<L, FULL, IN, OUT>), grid, dim3(256), 0, st, a, src, dst,
-                           Ns, toff);
+        hipLaunchKernelGGL((mw_r4_kernel<K, L, FULL, IN, OUT>), grid, dim3(256), 0, st, a, src,
+                           dst, Ns, toff);
 }
 
-template <int L, bool FULL>
-hipError_t launch_dd(const FxArgs &a, int s0, int s1, hipStream_t st)
+template <int K, int L, bool FULL>
+hipError_t launch_mw(const FxArgs &a, int s0, int s1, hipStream_t st)
 {
     const uint32_t N = a.N;
     if (N == 1) {
         if (s0 <= 0 && s1 > 0)
-            hipLaunchKernelGGL((dd_copy_kernel<L, FULL>), dim3((a.batch + 255) / 256), dim3(256),
-                               0, st, a);
+            hipLaunchKernelGGL((mw_copy_kernel<K, L, FULL>), dim3((a.batch + 255) / 256),
+                               dim3(256), 0, st, a);
         return hipGetLastError();
     }
     uint32_t lg = 0;
@@ -294,10 +491,10 @@ hipError_t launch_dd(const FxArgs &a, int s0, int s1, hipStream_t st)
         const uint32_t R = (s == 0 && (lg & 1)) ? 2u : 4u;
         if (s >= s0 && s < s1) {
             const bool first = s == 0, last = s == npass - 1;
-            if (first && last) launch_pass<L, FULL, true, true>(a, R, src, dst, Ns, toff, st);
-            else if (first) launch_pass<L, FULL, true, false>(a, R, src, dst, Ns, toff, st);
-            else if (last) launch_pass<L, FULL, false, true>(a, R, src, dst, Ns, toff, st);
-            else launch_pass<L, FULL, false, false>(a, R, src, dst, Ns, toff, st);
+            if (first && last) launch_pass<K, L, FULL, true, true>(a, R, src, dst, Ns, toff, st);
+            else if (first) launch_pass<K, L, FULL, true, false>(a, R, src, dst, Ns, toff, st);
+            else if (last) launch_pass<K, L, FULL, false, true>(a, R, src, dst, Ns, toff, st);
+            else launch_pass<K, L, FULL, false, false>(a, R, src, dst, Ns, toff, st);
         }
         if (R == 4) toff += 3 * Ns;
         Ns *= R;
@@ -308,15 +505,31 @@ hipError_t launch_dd(const FxArgs &a, int s0, int s1, hipStream_t st)
 
 }  // namespace
 
-hipError_t dd_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st)
+hipError_t dd_fft_launch(int K, int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st)
 {
     const bool full = (P == 32 * L);
-    switch (L) {
-    case 2: return full ? launch_dd<2, true>(a, s0, s1, st) : launch_dd<2, false>(a, s0, s1, st);
-    case 3: return full ? launch_dd<3, true>(a, s0, s1, st) : launch_dd<3, false>(a, s0, s1, st);
-    case 4: return full ? launch_dd<4, true>(a, s0, s1, st) : launch_dd<4, false>(a, s0, s1, st);
-    default: return hipErrorInvalidValue;
+#define CALLW(Kv, Lv) (full ? launch_mw<Kv, Lv, true>(a, s0, s1, st) \
+                            : launch_mw<Kv, Lv, false>(a, s0, s1, st))
+    if (K == 2) {
+        switch (L) {
+        case 2: return CALLW(2, 2);
+        case 3: return CALLW(2, 3);
+        case 4: return CALLW(2, 4);
+        default: return hipErrorInvalidValue;
+        }
     }
+    if (K == 4) {
+        switch (L) {
+        case 2: return CALLW(4, 2);
+        case 3: return CALLW(4, 3);
+        case 4: return CALLW(4, 4);
+        case 6: return CALLW(4, 6);
+        case 8: return CALLW(4, 8);
+        default: return hipErrorInvalidValue;
+        }
+    }
+#undef CALLW
+    return hipErrorInvalidValue;
 }
 
 }  // namespace mpfft

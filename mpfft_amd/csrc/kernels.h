@@ -233,14 +233,15 @@ inline int fxp_plan(uint32_t N, int W, FxpPass (&p)[32], uint32_t *nent)
 // stage per pass, the output conversion (rounding back to floats)
 hipError_t fxp_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st);
 
-// ---- double-word form (dd_fft.hip) ----
+// ---- double- and quad-word forms (dd_fft.hip) ----
 //
-// Same arguments as the fixed form: buf0 / buf1 hold one double2 {hi, lo}
-// per real (stride = 2N double2 per transform), tw holds ntw entries of
-// four doubles {re.hi, re.lo, im.hi, im.lo} in the per-lane fixed form's
-// order, lim bounds |exponent| of the inputs.  Stages: one per pass (the
-// first splits the limb input, the last rounds to P bits); N = 1: one.
-hipError_t dd_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st);
+// Same arguments as the fixed form, for K = 2 or 4 binary64 words per real:
+// buf0 / buf1 hold K/2 double2 per real (stride = N K double2 per
+// transform), tw holds ntw entries of 2K doubles (the real part's K words,
+// then the imaginary part's) in the per-lane fixed form's order, lim bounds
+// |exponent| of the inputs.  Stages: one per pass (the first splits the limb
+// input, the last rounds to P bits); N = 1: one.
+hipError_t dd_fft_launch(int K, int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st);
 
 // ---- double path (d_fft.hip) ----
 

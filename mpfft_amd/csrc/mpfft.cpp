@@ -33,8 +33,10 @@ constexpr long kMaxPrec = 2048;          // largest compiled limb count * 32 (pa
 constexpr long kMaxLimbsAll = 32;        // limb counts every kernel family compiles
 constexpr long kExpLimit = 1L << 29;     // accepted input exponent range
 constexpr unsigned long kMaxN = 1UL << 24;
-constexpr long kDdMinPrec = 53, kDdMaxPrec = 106;   // precisions of the double-word form
-constexpr int kDdExpLimit = 900;         // |exponent| of double-word inputs
+constexpr long kDdMinPrec = 53;          // precisions of the double- and quad-word forms
+constexpr long kDdMaxPrec = 106, kQdMaxPrec = 212;
+constexpr int kDdExpLimit = 900;         // |exponent| of their inputs (the lowest word
+constexpr int kQdExpLimit = 800;         // stays a normal binary64)
 
 #define HIP_TRY(expr)                                          \
     do {                                                       \
@@ -54,8 +56,14 @@ bool is_fixed(int form)
            form == MPFFT_FORM_FIXED_PANEL;
 }
 
+// the FP64 word forms: binary64 words per real (0: not one of them)
+int word_count(int form)
+{
+    return form == MPFFT_FORM_DD ? 2 : form == MPFFT_FORM_QD ? 4 : 0;
+}
+
 // forms that read their input through a range check (MPFFT_ERR_VALUE)
-bool has_range_flag(int form) { return is_fixed(form) || form == MPFFT_FORM_DD; }
+bool has_range_flag(int form) { return is_fixed(form) || word_count(form) > 0; }
 
 // forms whose device data use the SoA-limb layout of include/mpfft.h
 bool is_limb_form(int form) { return is_mp(form) || has_range_flag(form); }
@@ -481,13 +489,14 @@ int build_fx_plan(mpfft_plan_s *p)
     return MPFFT_OK;
 }
 
-// Double-word form: twiddles of the radix-4 passes (the per-lane fixed
-// form's entry order: pass after pass, entry 3k + r - 1 = w^(r k N / (4 Ns)))
-// as {re.hi, re.lo, im.hi, im.lo}, each part the binary64 nearest to the
-// value, resp. to what the first part leaves (MPFR at 256 bits), and the
-// double2 ping-pong buffers.
+// Double- and quad-word forms: twiddles of the radix-4 passes (the per-lane
+// fixed form's entry order: pass after pass, entry 3k + r - 1 =
+// w^(r k N / (4 Ns))) as K words of the real part, then K of the imaginary
+// part, each word the binary64 nearest to what the words before it leave of
+// the value (MPFR at 512 bits), and the word ping-pong buffers.
 int build_dd_plan(mpfft_plan_s *p)
 {
+    const int K = word_count(p->form);
     const unsigned long N = p->N;
     int lg0 = 0;
     while ((1UL << lg0) < N) lg0++;
@@ -495,12 +504,12 @@ int build_dd_plan(mpfft_plan_s *p)
     for (unsigned long Ns = (lg0 & 1) ? 2 : 1; Ns < N; Ns *= 4)
         for (unsigned long k = 0; k < Ns; k++)
             for (unsigned long r = 1; r <= 3; r++) ent.push_back(r * k * (N / (4 * Ns)));
-    std::vector<double> tab(4 * (ent.empty() ? 1 : ent.size()), 0.0);
+    std::vector<double> tab(2 * K * (ent.empty() ? 1 : ent.size()), 0.0);
     unsigned nt = std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
     if (ent.size() < 1024) nt = 1;
     auto work = [&](size_t e0, size_t e1) {
         mpfr_t th, c, d, t;
-        mpfr_inits2(256, th, c, d, t, (mpfr_ptr)0);
+        mpfr_inits2(512, th, c, d, t, (mpfr_ptr)0);
         for (size_t e = e0; e < e1; e++) {
             mpfr_const_pi(th, MPFR_RNDN);
             mpfr_mul_ui(th, th, 2 * ent[e], MPFR_RNDN);
@@ -508,11 +517,12 @@ int build_dd_plan(mpfft_plan_s *p)
             mpfr_sin_cos(d, c, th, MPFR_RNDN);
             mpfr_neg(d, d, MPFR_RNDN);                    // w = cos - i sin
             for (int u = 0; u < 2; u++) {
-                mpfr_ptr v = u ? d : c;
-                const double hi = mpfr_get_d(v, MPFR_RNDN);
-                mpfr_sub_d(t, v, hi, MPFR_RNDN);          // exact at 256 bits
-                tab[4 * e + 2 * u] = hi;
-                tab[4 * e + 2 * u + 1] = mpfr_get_d(t, MPFR_RNDN);
+                mpfr_set(t, u ? d : c, MPFR_RNDN);
+                for (int w = 0; w < K; w++) {
+                    const double v = mpfr_get_d(t, MPFR_RNDN);
+                    tab[2 * K * e + K * u + w] = v;
+                    mpfr_sub_d(t, t, v, MPFR_RNDN);       // exact at 512 bits
+                }
             }
         }
         mpfr_clears(th, c, d, t, (mpfr_ptr)0);
@@ -524,12 +534,12 @@ int build_dd_plan(mpfft_plan_s *p)
     for (auto &x : th) x.join();
     int rc;
     if ((rc = p->fx_tw.upload(tab))) return rc;
-    const size_t bytes = 2 * sizeof(double) * 2 * N * (size_t)p->batch;
+    const size_t bytes = K * sizeof(double) * 2 * N * (size_t)p->batch;
     if ((rc = p->fx_buf0.alloc(bytes)) || (rc = p->fx_buf1.alloc(bytes)) ||
         (rc = p->fx_flag.upload(std::vector<uint32_t>{0u})))
         return rc;
     p->fx_ntw = (uint32_t)ent.size();
-    p->fx_lim = kDdExpLimit;
+    p->fx_lim = K == 2 ? kDdExpLimit : kQdExpLimit;
     return MPFFT_OK;
 }
 
@@ -548,10 +558,11 @@ int run_fx(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t s
     a.N = (uint32_t)p->N; a.batch = p->batch;
     a.lim = p->fx_lim; a.P = (int)p->P;
     a.ncu = p->ncu;
-    if (p->form == MPFFT_FORM_DD) a.stride = 2 * p->N;          // double2 per real
-    hipError_t e = p->form == MPFFT_FORM_DD ? dd_fft_launch((int)p->L, (int)p->P, a, s0, s1, st)
-                   : p->fx_panel            ? fxp_fft_launch((int)p->L, (int)p->P, a, s0, s1, st)
-                                            : fx_fft_launch((int)p->L, (int)p->P, a, s0, s1, st);
+    const int K = word_count(p->form);
+    if (K) a.stride = (uint64_t)p->N * K;                      // K/2 double2 per real
+    hipError_t e = K            ? dd_fft_launch(K, (int)p->L, (int)p->P, a, s0, s1, st)
+                   : p->fx_panel ? fxp_fft_launch((int)p->L, (int)p->P, a, s0, s1, st)
+                                 : fx_fft_launch((int)p->L, (int)p->P, a, s0, s1, st);
     return e == hipSuccess ? MPFFT_OK : MPFFT_ERR_HIP;
 }
 
@@ -801,6 +812,8 @@ int mpfft_plan_create(mpfft_plan *plan, unsigned long n, long prec, int form, un
         if (prec < 2 || prec > kMaxPrec) return MPFFT_ERR_PREC;
         if (form == MPFFT_FORM_DD && (prec < kDdMinPrec || prec > kDdMaxPrec))
             return MPFFT_ERR_PREC;
+        if (form == MPFFT_FORM_QD && (prec < kDdMinPrec || prec > kQdMaxPrec))
+            return MPFFT_ERR_PREC;
         p->P = prec;
         p->L = limbs_for_precision(prec);
         p->words = 2 * n * (size_t)(p->L + 1);
@@ -809,7 +822,7 @@ int mpfft_plan_create(mpfft_plan *plan, unsigned long n, long prec, int form, un
             !((form == MPFFT_FORM_FIXED || form == MPFFT_FORM_FIXED_PANEL) &&
               fxp_eligible((uint32_t)n, batch)))
             return MPFFT_ERR_PREC;
-        rc = form == MPFFT_FORM_DD ? build_dd_plan(p.get())
+        rc = word_count(form)       ? build_dd_plan(p.get())
              : is_fixed(form)      ? build_fx_plan(p.get())
                                    : build_mp_plan(p.get());
     } else {
@@ -928,7 +941,7 @@ int mpfft_execute_limbs_range(mpfft_plan p, const uint32_t *d_in, uint32_t *d_ou
 int mpfft_plan_stages(mpfft_plan p)
 {
     if (!p) return MPFFT_ERR_ARG;
-    if (p->form == MPFFT_FORM_DD) return p->N == 1 ? 1 : fx_passes((uint32_t)p->N);
+    if (word_count(p->form)) return p->N == 1 ? 1 : fx_passes((uint32_t)p->N);
     if (is_fixed(p->form))
         return p->fx_panel ? p->fx_npass + 2 : fx_stages((uint32_t)p->N);
     return is_mp(p->form) ? 1 + (int)p->levels.size() : MPFFT_ERR_FORM;

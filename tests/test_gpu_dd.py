@@ -134,3 +134,85 @@ def test_dd_precision_and_range_checked():
         run(n, P, [x])
     assert e.value.code == -3
     assert len(run(n, P, [I.mp_random(n, P, seed=9)])) == 1
+
+
+# ---- quad-word form (MPFFT_FORM_QD, 53 <= P <= 212) --------------------------
+#
+# Contract: normwise error (units of 2^-P) at most 1 + (8 + 32 log2 n)
+# 2^(P-212): the final rounding, plus the word arithmetic, whose order-3
+# terms are rounded (a product ~10 u^4, a sum ~4 u^4, u^4 = 2^-212; per
+# radix-4 pass one complex product and two levels of sums, ~64 u^4 at
+# most, log4 n passes).  Measured at P = 212: 0.5-0.7 at n = 16 .. 4096,
+# 0.76 at 2^16 (the plain form: 2.9).
+
+
+def qbound(n, P):
+    lg = max(1, n.bit_length() - 1)
+    return 1.0 + (8 + 32 * lg) * 2.0 ** (P - 212)
+
+
+def qcheck(n, P, x, y):
+    Lo, L2 = O.limbs(P), O.limbs(2 * P)
+    star = O.mp_fft(n, P, x, Pw=2 * P)
+    mu, nw, zb = O.err_stats(n, P, y, Lo, star, L2)
+    assert zb == 0, (n, P)
+    assert nw <= qbound(n, P), (n, P, nw, qbound(n, P))
+    return mu, nw
+
+
+@pytest.mark.parametrize("n", [1, 2, 4, 16, 64, 1024, 2048, 4096])
+@pytest.mark.parametrize("P", [53, 106, 160, 200, 212])
+def test_qd_error_bound(n, P):
+    xs = [I.mp_random(n, P, seed=2300 + n + P + 7 * i) for i in range(3)]
+    for x, y in zip(xs, run(n, P, xs, M.FORM_QD)):
+        mu, nw = qcheck(n, P, x, y)
+    print(f"qd n={n} P={P}: max {mu:.3g} ulp, normwise {nw:.3g} 2^-P (bound {qbound(n, P):.3g})")
+
+
+def test_qd_full_size():
+    n, P = 1 << 16, 212
+    x = I.mp_random(n, P, seed=98)
+    mu, nw = qcheck(n, P, x, run(n, P, [x], M.FORM_QD)[0])
+    Lo, L2 = O.limbs(P), O.limbs(2 * P)
+    star = O.mp_fft(n, P, x, Pw=2 * P)
+    mu_p, nw_p, _ = O.err_stats(n, P, O.mp_fft(n, P, x), Lo, star, L2)
+    print(f"qd n={n} P={P}: max {mu:.3g} ulp, normwise {nw:.3g} 2^-P; "
+          f"plain form {mu_p:.3g} ulp, {nw_p:.3g} 2^-P")
+
+
+@pytest.mark.parametrize("n", [1, 16, 1024])
+def test_qd_impulse(n):
+    P = 212
+    for y in run(n, P, [impulse(n, P)] * 2, M.FORM_QD):
+        h, lm = I.split(y, n, O.limbs(P))
+        assert (h[:n] == 2).all() and (lm[0, :n] == 0x80000000).all() and (lm[1:, :n] == 0).all()
+        assert ((h[n:] & ~1) == np.int32(-2**31)).all()
+
+
+def test_qd_batch_stages_and_checks():
+    n, P, b = 512, 180, 3
+    xs = [I.mp_random(n, P, seed=600 + i) for i in range(b)]
+    for x, y in zip(xs, run(n, P, xs, M.FORM_QD)):
+        qcheck(n, P, x, y)
+    plan = M.Plan(n, P, M.FORM_QD, b)
+    S = plan.stages()
+    assert S == 1 + 4                    # log2 n = 9: radix 2, then four radix-4 passes
+    x = np.concatenate([I.relimb(t, n, O.limbs(P), plan.L) for t in xs])
+    dx = torch.from_numpy(x.view(np.int32)).cuda()
+    d1, d2 = torch.empty_like(dx), torch.empty_like(dx)
+    plan.execute_device(dx.data_ptr(), d1.data_ptr())
+    for s in range(S):
+        plan.execute_range(dx.data_ptr(), d2.data_ptr(), s, s + 1)
+    plan.check()
+    assert torch.equal(d1, d2)
+    plan.close()
+    for Pb in (52, 213):
+        with pytest.raises(M.MpfftError) as e:
+            M.Plan(64, Pb, M.FORM_QD, 1)
+        assert e.value.code == -2
+    xb = I.mp_random(256, 212, seed=8)
+    h, _ = I.split(xb, 256, O.limbs(212))
+    h[3] = np.int32((801 << 1) | (h[3] & 1))
+    with pytest.raises(M.MpfftError) as e:
+        run(256, 212, [xb], M.FORM_QD)
+    assert e.value.code == -3
