@@ -347,7 +347,9 @@ def main():
 
     n, P, B = a.n, a.prec, a.batch
     Lc = I.limbs_for(P)
-    form_ids = {"plain": M.FORM_MP, "fixed": M.FORM_FIXED, "fixed_lane": M.FORM_FIXED_LANE}
+    form_ids = {"plain": M.FORM_MP, "fixed": M.FORM_FIXED, "fixed_lane": M.FORM_FIXED_LANE,
+                "fixed_panel": M.FORM_FIXED_PANEL, "double_word": M.FORM_DD,
+                "quad_word": M.FORM_QD}
     forms = [f for f in a.forms.split(",") if f]
     if "plain" not in forms:
         forms.insert(0, "plain")
@@ -368,6 +370,10 @@ def main():
             out[f]["kernel_ms_per_step"] = {"leaf": round(r["parts_ms"][0], 4),
                                             "levels": round(r["parts_ms"][1], 4),
                                             "level_launches": r["S"] - 1}
+        elif form_ids[f] in (M.FORM_DD, M.FORM_QD):
+            out[f]["kernel_ms_per_step"] = {"first_pass": round(r["parts_ms"][0], 4),
+                                            "other_passes": round(r["parts_ms"][1], 4),
+                                            "pass_launches": r["S"]}
         elif r["panel"]:
             out[f]["kernel_ms_per_step"] = {"in": round(r["parts_ms"][0], 4),
                                             "passes": round(r["parts_ms"][1], 4),
