@@ -16,8 +16,8 @@
 // Quad-word numbers c0 + c1 + c2 + c3 follow the same principles (see
 // below).
 //
-// Structure: radix-4 Stockham passes, a radix-2 pass first when log2 N is
-// odd, as the fixed form (fx_fft.hip).  The first pass reads the SoA-limb
+// Structure: Stockham passes (mw_plan in kernels.h): radix 16 for the
+// double-word form, radix 4 for the quad-word form.  The first pass reads the SoA-limb
 // input and splits every real into its words exactly; the last pass rounds
 // the words' sum to P bits with the limb adder of mp_arith.h (double-word:
 // both parts are exact P-bit values, so one correctly rounded addition) and
@@ -447,6 +447,74 @@ mw_r4_kernel(FxArgs a, const double2 *src, double2 *dst, uint32_t Ns, uint32_t t
     w_store<K, L, FULL, OUT>(a, dst, b, o + 3 * Ns, w_add_ib<K>(t1, t3));
 }
 
+template <int K>
+__device__ __forceinline__ CW_<K> w_neg_i(const CW_<K> &a)        // -i a
+{
+    return {a.im, Word<K>::neg(a.re)};
+}
+
+// forward DFT4 in place: x_q <- sum_r x_r (-i)^(r q)
+template <int K>
+__device__ __forceinline__ void w_dft4(CW_<K> &x0, CW_<K> &x1, CW_<K> &x2, CW_<K> &x3)
+{
+    const CW_<K> t0 = w_add<K>(x0, x2), t1 = w_sub<K>(x0, x2);
+    const CW_<K> t2 = w_add<K>(x1, x3), t3 = w_sub<K>(x1, x3);
+    x0 = w_add<K>(t0, t2);
+    x1 = w_sub_ib<K>(t1, t3);
+    x2 = w_sub<K>(t0, t2);
+    x3 = w_add_ib<K>(t1, t3);
+}
+
+// Radix-16 Stockham pass of sub-transform size Ns: thread j (< N/16) reads
+// v_r = x[j + r N/16], multiplies v_r by w^(r k N / (16 Ns)) (entry
+// toff + 15k + r - 1), then the DFT16 as 4 x 4: a DFT4 over r2 for each r1
+// (inputs v_(r1 + 4 r2)), the internal twiddles w16^(r1 q1) (entries cent +
+// r1 q1; w16^4 = -i exactly), a DFT4 over r1 for each q1, output q1 + 4 q2
+// written to (j - k) 16 + k + (q1 + 4 q2) Ns.
+template <int K, int L, bool FULL, bool IN, bool OUT>
+__global__ void __launch_bounds__(256)
+mw_r16_kernel(FxArgs a, const double2 *src, double2 *dst, uint32_t Ns, uint32_t toff,
+              uint32_t cent)
+{
+    const uint32_t S = a.N >> 4, j = blockIdx.x * 256 + threadIdx.x, b = blockIdx.y;
+    if (j >= S) return;
+    const uint32_t k = j & (Ns - 1);
+    const double2 *tw = reinterpret_cast<const double2 *>(a.tw);
+    auto entry = [&](uint32_t t) {
+        CW_<K> w;
+        Word<K>::get(w.re, tw + (size_t)K * t);
+        Word<K>::get(w.im, tw + (size_t)K * t + K / 2);
+        return w;
+    };
+    CW_<K> v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = w_load<K, L, IN>(a, src, b, j + r * S);
+    if (k) {
+#pragma unroll
+        for (int r = 1; r < 16; r++) v[r] = w_mul<K>(v[r], entry(toff + 15 * k + r - 1));
+    }
+#pragma unroll
+    for (int r1 = 0; r1 < 4; r1++) w_dft4<K>(v[r1], v[r1 + 4], v[r1 + 8], v[r1 + 12]);
+    // v[r1 + 4 q1] now holds U_r1(q1)
+#pragma unroll
+    for (int r1 = 1; r1 < 4; r1++)
+#pragma unroll
+        for (int q1 = 1; q1 < 4; q1++) {
+            const int m = r1 * q1;
+            CW_<K> &u = v[r1 + 4 * q1];
+            u = m == 4 ? w_neg_i<K>(u) : w_mul<K>(u, entry(cent + m));
+        }
+#pragma unroll
+    for (int q1 = 0; q1 < 4; q1++) w_dft4<K>(v[4 * q1], v[4 * q1 + 1], v[4 * q1 + 2], v[4 * q1 + 3]);
+    // v[4 q1 + q2] = y_(q1 + 4 q2)
+    const uint32_t o = (j - k) * 16 + k;
+#pragma unroll
+    for (int q1 = 0; q1 < 4; q1++)
+#pragma unroll
+        for (int q2 = 0; q2 < 4; q2++)
+            w_store<K, L, FULL, OUT>(a, dst, b, o + (q1 + 4 * q2) * Ns, v[4 * q1 + q2]);
+}
+
 // N = 1: the transform is the identity; the input split and rounded back
 // (exact), with its range check
 template <int K, int L, bool FULL>
@@ -459,17 +527,20 @@ mw_copy_kernel(FxArgs a)
 }
 
 template <int K, int L, bool FULL, bool IN, bool OUT>
-void launch_pass(const FxArgs &a, uint32_t R, const double2 *src, double2 *dst, uint32_t Ns,
-                 uint32_t toff, hipStream_t st)
+void launch_pass(const FxArgs &a, const MwPass &ps, uint32_t cent, const double2 *src,
+                 double2 *dst, hipStream_t st)
 {
-    const uint32_t threads = a.N / R;
+    const uint32_t threads = a.N / ps.R;
     const dim3 grid((threads + 255) / 256, a.batch);
-    if (R == 2)
+    if (ps.R == 2)
         hipLaunchKernelGGL((mw_r2_kernel<K, L, FULL, IN, OUT>), grid, dim3(256), 0, st, a, src,
                            dst);
-    else
+    else if (ps.R == 4)
         hipLaunchKernelGGL((mw_r4_kernel<K, L, FULL, IN, OUT>), grid, dim3(256), 0, st, a, src,
-                           dst, Ns, toff);
+                           dst, ps.Ns, ps.toff);
+    else if constexpr (K == 2 && !IN && !OUT)       // (the plan keeps radix 16 inside)
+        hipLaunchKernelGGL((mw_r16_kernel<K, L, FULL, IN, OUT>), grid, dim3(256), 0, st, a, src,
+                           dst, ps.Ns, ps.toff, cent);
 }
 
 template <int K, int L, bool FULL>
@@ -482,22 +553,19 @@ hipError_t launch_mw(const FxArgs &a, int s0, int s1, hipStream_t st)
                                dim3(256), 0, st, a);
         return hipGetLastError();
     }
-    uint32_t lg = 0;
-    while ((1u << lg) < N) lg++;
-    const int npass = fx_passes(N);
+    MwPass ps[32];
+    uint32_t cent = 0;
+    const int npass = mw_plan(N, K, ps, &cent);
     double2 *src = reinterpret_cast<double2 *>(a.buf1), *dst = reinterpret_cast<double2 *>(a.buf0);
-    uint32_t Ns = 1, toff = 0;
     for (int s = 0; s < npass; s++) {
-        const uint32_t R = (s == 0 && (lg & 1)) ? 2u : 4u;
+        if (ps[s].R == 16 && (K != 2 || s == 0 || s == npass - 1)) return hipErrorInvalidValue;
         if (s >= s0 && s < s1) {
             const bool first = s == 0, last = s == npass - 1;
-            if (first && last) launch_pass<K, L, FULL, true, true>(a, R, src, dst, Ns, toff, st);
-            else if (first) launch_pass<K, L, FULL, true, false>(a, R, src, dst, Ns, toff, st);
-            else if (last) launch_pass<K, L, FULL, false, true>(a, R, src, dst, Ns, toff, st);
-            else launch_pass<K, L, FULL, false, false>(a, R, src, dst, Ns, toff, st);
+            if (first && last) launch_pass<K, L, FULL, true, true>(a, ps[s], cent, src, dst, st);
+            else if (first) launch_pass<K, L, FULL, true, false>(a, ps[s], cent, src, dst, st);
+            else if (last) launch_pass<K, L, FULL, false, true>(a, ps[s], cent, src, dst, st);
+            else launch_pass<K, L, FULL, false, false>(a, ps[s], cent, src, dst, st);
         }
-        if (R == 4) toff += 3 * Ns;
-        Ns *= R;
         std::swap(src, dst);
     }
     return hipGetLastError();

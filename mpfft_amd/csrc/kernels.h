@@ -243,6 +243,47 @@ hipError_t fxp_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStre
 // input, the last rounds to P bits); N = 1: one.
 hipError_t dd_fft_launch(int K, int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st);
 
+// Passes of the word forms.  Double-word (K = 2): the first pass (which
+// also splits the limb input) radix 2 or 4, the last (which also rounds to
+// limbs) radix 4, and radix 16 in between (one thread holds the 16 inputs
+// of a butterfly), with one more radix-4 pass where log2 N needs it (a
+// converting radix-16 pass ran at one wave per SIMD and took 2.4x the time
+// of a converting radix-4 pass; an inner one 0.82x that of the two radix-4
+// passes it replaces); quad-word (K = 4, twice the registers per value):
+// radix 4 after a radix-2 pass when log2 N is odd.
+// Pass i has radix R and sub-transform size Ns; its twiddles w^(r k N / (R
+// Ns)) (k < Ns, r = 1 .. R-1) are table entries toff + (R - 1) k + r - 1.
+// With a radix-16 pass, ten entries from *nent on hold w16^m = w^(m N / 16),
+// m = 0 .. 9, the internal twiddles of the 4 x 4 butterfly.
+struct MwPass {
+    uint32_t R, Ns, toff;
+};
+
+inline int mw_plan(uint32_t N, int K, MwPass (&p)[32], uint32_t *nent)
+{
+    int lg = 0;
+    while ((1u << lg) < N) lg++;
+    int n = 0;
+    uint32_t Ns = 1, toff = 0;
+    auto add = [&](uint32_t R) {
+        p[n++] = {R, Ns, toff};
+        toff += (R - 1) * Ns;
+        Ns *= R;
+    };
+    if (K == 2 && lg >= 5) {
+        add((lg & 1) ? 2 : 4);
+        const int m = lg - ((lg & 1) ? 1 : 2) - 2;   // log2 of the middle passes' product
+        if (m % 4) add(4);
+        for (int i = 0; i < m / 4; i++) add(16);
+        add(4);
+    } else {
+        if (lg & 1) add(2);
+        while (Ns < N) add(4);
+    }
+    *nent = toff;
+    return n;
+}
+
 // ---- double path (d_fft.hip) ----
 
 struct DTables {

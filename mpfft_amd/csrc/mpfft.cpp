@@ -500,10 +500,22 @@ int build_dd_plan(mpfft_plan_s *p)
     const unsigned long N = p->N;
     int lg0 = 0;
     while ((1UL << lg0) < N) lg0++;
+    (void)lg0;
+    MwPass ps[32];
+    uint32_t nent = 0;
+    const int np = N > 1 ? mw_plan((uint32_t)N, K, ps, &nent) : 0;
     std::vector<unsigned long> ent;
-    for (unsigned long Ns = (lg0 & 1) ? 2 : 1; Ns < N; Ns *= 4)
-        for (unsigned long k = 0; k < Ns; k++)
-            for (unsigned long r = 1; r <= 3; r++) ent.push_back(r * k * (N / (4 * Ns)));
+    bool r16 = false;
+    for (int i = 0; i < np; i++) {
+        for (unsigned long k = 0; k < ps[i].Ns; k++)
+            for (unsigned long r = 1; r < ps[i].R; r++)
+                ent.push_back(r * k * (N / ((unsigned long)ps[i].R * ps[i].Ns)));
+        r16 |= ps[i].R == 16;
+    }
+    if (ent.size() != nent) return MPFFT_ERR_ARG;
+    if (r16)
+        for (unsigned long m = 0; m < 10; m++) ent.push_back(m * (N / 16));   // w16^m
+    p->fx_npass = np;
     std::vector<double> tab(2 * K * (ent.empty() ? 1 : ent.size()), 0.0);
     unsigned nt = std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
     if (ent.size() < 1024) nt = 1;
@@ -941,7 +953,7 @@ int mpfft_execute_limbs_range(mpfft_plan p, const uint32_t *d_in, uint32_t *d_ou
 int mpfft_plan_stages(mpfft_plan p)
 {
     if (!p) return MPFFT_ERR_ARG;
-    if (word_count(p->form)) return p->N == 1 ? 1 : fx_passes((uint32_t)p->N);
+    if (word_count(p->form)) return p->N == 1 ? 1 : p->fx_npass;
     if (is_fixed(p->form))
         return p->fx_panel ? p->fx_npass + 2 : fx_stages((uint32_t)p->N);
     return is_mp(p->form) ? 1 + (int)p->levels.size() : MPFFT_ERR_FORM;

@@ -109,7 +109,7 @@ def test_dd_batch_and_stages():
         check(n, P, x, y)
     plan = M.Plan(n, P, M.FORM_DD, b)
     S = plan.stages()
-    assert S == 1 + 5                    # log2 n = 11: radix 2, then five radix-4 passes
+    assert S == 1 + 2 + 1                # log2 n = 11: radix 2, two radix 16, radix 4
     x = np.concatenate([I.relimb(t, n, O.limbs(P), plan.L) for t in xs])
     dx = torch.from_numpy(x.view(np.int32)).cuda()
     d1, d2 = torch.empty_like(dx), torch.empty_like(dx)
