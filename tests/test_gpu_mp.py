@@ -202,6 +202,30 @@ def test_mpfr_plan_timing():
     assert np.array_equal(soa_from_mpfr_pair(n, P, outr, outi), O.mp_fft(n, P, x))
 
 
+@pytest.mark.parametrize("form,P", [(M.FORM_FIXED, 256), (M.FORM_DD, 106), (M.FORM_QD, 212),
+                                    (M.FORM_FIXED, 2048)])
+def test_mpfr_entry_point_other_forms(form, P):
+    # the drop-in mpfr_t path runs every limb-layout form: each result equals
+    # the same plan's device execution bit for bit (same kernels), and
+    # stays within the plain form's error against the 2P-bit oracle
+    n = 1024
+    plan = M.Plan(n, P, form, 1)
+    x = I.mp_random(n, P, seed=21)
+    inr, ini = mpfr_pair_from_soa(n, P, x)
+    outr, outi = MA.MpfrArray(n, P), MA.MpfrArray(n, P)
+    M.check(M.lib().mpfft_execute_mpfr(plan.h, inr.ptr(), ini.ptr(), outr.ptr(), outi.ptr()),
+            "execute_mpfr")
+    y = soa_from_mpfr_pair(n, P, outr, outi)
+    yh = I.relimb(plan.execute_host(I.relimb(x, n, O.limbs(P), plan.L)), n, plan.L, O.limbs(P))
+    plan.close()
+    assert np.array_equal(y, yh)
+    Lo, L2 = O.limbs(P), O.limbs(2 * P)
+    star = O.mp_fft(n, P, x, Pw=2 * P)
+    _, nw, zb = O.err_stats(n, P, y, Lo, star, L2)
+    _, nw_p, _ = O.err_stats(n, P, O.mp_fft(n, P, x), Lo, star, L2)
+    assert zb == 0 and nw <= max(nw_p, 1.0), (nw, nw_p)
+
+
 def test_mpfr_entry_point_errors():
     lib = M.lib()
     n, P = 16, 128
