@@ -1,0 +1,140 @@
+// Host-logic check of libmpfft without a GPU: the host library (plans,
+// twiddle tables, batched form) built against a stand-in HIP runtime whose
+// kernel launchers touch every buffer range a launch would read or write
+// (stub.cpp), run under AddressSanitizer; plus the pass-plan invariants the
+// kernels rely on, for every size and compiled width.
+#include "../../include/mpfft.h"
+#include "../../mpfft_amd/csrc/kernels.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+static int fails = 0;
+#define CHECK(c, ...)                                   \
+    do {                                                \
+        if (!(c)) {                                     \
+            fails++;                                    \
+            printf("FAIL %s:%d ", __FILE__, __LINE__);  \
+            printf(__VA_ARGS__);                        \
+            printf("\n");                               \
+        }                                               \
+    } while (0)
+
+static void plans()
+{
+    using namespace mpfft;
+    const int widths[] = {3, 4, 5, 6, 8, 10, 14, 18, 26, 34, 66};
+    for (int W : widths) {
+        // the twiddle tables fit at the slot end, or get their own area
+        const MfGeom G = mf_geom(W);
+        CHECK(mf_tail_fits(W) || mf_table_words(W) == 3 * G.RW, "W=%d table area", W);
+        CHECK(G.KT * 32 >= G.NXB && G.MT * 32 >= G.NCOL, "W=%d tiles", W);
+        for (int lg = 4; lg <= 24; lg++) {
+            const uint32_t N = 1u << lg;
+            FxpPass p[32];
+            uint32_t nent = 0, prod = 1, ent = 0;
+            const int np = fxp_plan(N, W, p, &nent);
+            for (int i = 0; i < np; i++) {
+                CHECK(p[i].Ns == prod && p[i].toff == ent, "fxp W=%d lg=%d pass %d", W, lg, i);
+                CHECK(p[i].R != 8 || W <= kPanelR8MaxW, "fxp radix 8 at W=%d", W);
+                ent += (p[i].R - 1) * p[i].Ns;
+                prod *= p[i].R;
+            }
+            CHECK(prod == N && ent == nent, "fxp W=%d lg=%d product", W, lg);
+        }
+    }
+    for (int K : {2, 4})
+        for (int lg = 1; lg <= 24; lg++) {
+            const uint32_t N = 1u << lg;
+            MwPass p[32];
+            uint32_t nent = 0, prod = 1, ent = 0;
+            const int np = mw_plan(N, K, p, &nent);
+            for (int i = 0; i < np; i++) {
+                CHECK(p[i].Ns == prod && p[i].toff == ent, "mw K=%d lg=%d pass %d", K, lg, i);
+                // radix 16 only between the converting first and last passes
+                CHECK(p[i].R != 16 || (K == 2 && i > 0 && i < np - 1), "mw K=%d lg=%d r16", K, lg);
+                ent += (p[i].R - 1) * p[i].Ns;
+                prod *= p[i].R;
+            }
+            CHECK(prod == N && ent == nent, "mw K=%d lg=%d product", K, lg);
+        }
+    // panels: a partial last panel is padded; the preferred rule
+    CHECK(fxp_groups(1) == 1 && fxp_groups(64) == 1 && fxp_groups(65) == 2, "groups");
+    CHECK(fxp_preferred(1024, 64, 10) && fxp_preferred(1024, 120, 10) &&
+              !fxp_preferred(1 << 16, 80, 10) && fxp_preferred(256, 3, 34) &&
+              !fxp_preferred(8, 64, 10),
+          "preferred rule");
+}
+
+static void batch(int form, std::vector<unsigned long> n, std::vector<long> P, int expect_check)
+{
+    mpfft_batch b;
+    int rc = mpfft_batch_create(&b, (unsigned)n.size(), n.data(), P.data(), form);
+    CHECK(rc == 0, "create form %d rc %d", form, rc);
+    if (rc) return;
+    size_t words = 0;
+    mpfft_batch_words(b, &words);
+    // every transform owns a disjoint, tight range of the packed buffers
+    std::vector<char> used(words, 0);
+    for (unsigned i = 0; i < n.size(); i++) {
+        size_t off;
+        long L;
+        int f;
+        mpfft_batch_info(b, i, &off, &L, &f);
+        CHECK(L == mpfft_limbs_for_precision(P[i]), "limbs");
+        const size_t w = 2 * n[i] * (L + 1);
+        for (size_t k = off; k < off + w; k++) {
+            CHECK(k < words && !used[k], "overlap at transform %u", i);
+            if (k < words) used[k] = 1;
+        }
+    }
+    for (size_t k = 0; k < words; k++) CHECK(used[k], "gap at word %zu", k);
+    std::vector<uint32_t> in(words, 0), out(words, 0), in2(words, 0), out2(words, 0);
+    CHECK(mpfft_batch_tune(b, in.data(), out.data()) == 0, "tune");
+    CHECK(mpfft_batch_execute_limbs(b, in.data(), out.data(), nullptr) == 0, "execute");
+    CHECK(mpfft_batch_execute_limbs(b, in2.data(), out2.data(), nullptr) == 0, "execute again");
+    // the stand-in launchers set the range flag of the fixed and word forms
+    CHECK(mpfft_batch_check(b) == expect_check, "check");
+    int S = 0;
+    double ms = 0;
+    mpfft_batch_schedule(b, &S, &ms);
+    CHECK(S == 1 || S == 2 || S == 4, "streams %d", S);
+    mpfft_batch_destroy(b);
+}
+
+int main()
+{
+    plans();
+    batch(MPFFT_FORM_MP, {1, 2, 4, 8, 16, 64, 512, 512, 64, 32, 256, 128, 8, 512},
+          {64, 100, 128, 256, 53, 128, 256, 256, 1024, 200, 64, 512, 128, 100}, 0);
+    srand(11);
+    std::vector<unsigned long> n;
+    std::vector<long> P;
+    const long ps[] = {128, 256, 512, 1024, 2048};
+    for (int i = 0; i < 160; i++) {
+        n.push_back(1UL << (4 + rand() % 7));
+        P.push_back(ps[rand() % 5]);
+    }
+    batch(MPFFT_FORM_FIXED, n, P, MPFFT_ERR_VALUE);
+    batch(MPFFT_FORM_FIXED_PANEL, {256, 256, 256, 16}, {256, 256, 256, 2048}, MPFFT_ERR_VALUE);
+    // a plan of every form, created and run through the host path
+    const int forms[] = {MPFFT_FORM_MP, MPFFT_FORM_MP_C1, MPFFT_FORM_MP_C2, MPFFT_FORM_FIXED,
+                         MPFFT_FORM_FIXED_LANE, MPFFT_FORM_DD, MPFFT_FORM_QD};
+    for (int f : forms)
+        for (unsigned long N : {1UL, 2UL, 16UL, 128UL, 2048UL}) {
+            const long prec = f == MPFFT_FORM_DD ? 106 : f == MPFFT_FORM_QD ? 212 : 256;
+            mpfft_plan p;
+            int rc = mpfft_plan_create(&p, N, prec, f, 3);
+            CHECK(rc == 0, "plan form %d N %lu rc %d", f, N, rc);
+            if (rc) continue;
+            long L;
+            size_t w;
+            mpfft_plan_info(p, &L, &w);
+            std::vector<uint32_t> x(3 * w, 0), y(3 * w, 0);
+            mpfft_execute_limbs_host(p, x.data(), y.data());
+            mpfft_plan_destroy(p);
+        }
+    printf(fails ? "FAIL\n" : "PASS\n");
+    return fails ? 1 : 0;
+}

@@ -1,0 +1,95 @@
+// Stand-in HIP runtime for the host-logic check (tests/test_host_logic.py):
+// allocations are host memory, streams and events are tokens, graph capture
+// records the launches and a graph launch replays them, and the kernel
+// launchers touch every buffer range a launch would read or write, so a
+// wrong size or offset in the host library shows under AddressSanitizer.
+#include <hip/hip_runtime.h>
+#include "../../mpfft_amd/csrc/kernels.h"
+#include <cstdlib>
+#include <cstring>
+#include <cstdio>
+#include <mutex>
+#include <vector>
+#include <functional>
+#include <set>
+struct ihipStream_t { int id; bool capturing = false; std::vector<std::function<void()>> *cap = nullptr; };
+struct ihipEvent_t { int x; };
+struct ihipGraph { std::vector<std::function<void()>> ops; };
+struct hipGraphExec { std::vector<std::function<void()>> ops; };
+static thread_local hipStream_t g_capture = nullptr;
+static std::vector<std::function<void()>> *cur_cap() { return g_capture ? g_capture->cap : nullptr; }
+static void run_or_record(std::function<void()> f) { if (auto c = cur_cap()) c->push_back(f); else f(); }
+hipError_t hipMalloc(void **p, size_t b) { *p = malloc(b ? b : 1); return hipSuccess; }
+hipError_t hipFree(void *p) { free(p); return hipSuccess; }
+hipError_t hipHostMalloc(void **p, size_t b, unsigned) { *p = malloc(b ? b : 1); return hipSuccess; }
+hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
+hipError_t hipMemcpy(void *d, const void *s, size_t b, hipMemcpyKind) { memcpy(d, s, b); return hipSuccess; }
+hipError_t hipMemcpyAsync(void *d, const void *s, size_t b, hipMemcpyKind, hipStream_t) { run_or_record([=] { memcpy(d, s, b); }); return hipSuccess; }
+hipError_t hipMemcpy2DAsync(void *d, size_t dp, const void *s, size_t sp, size_t w, size_t h, hipMemcpyKind, hipStream_t) {
+    for (size_t i = 0; i < h; i++) memcpy((char *)d + i * dp, (const char *)s + i * sp, w); return hipSuccess; }
+hipError_t hipStreamCreate(hipStream_t *s) { *s = new ihipStream_t{1}; return hipSuccess; }
+hipError_t hipStreamDestroy(hipStream_t s) { delete s; return hipSuccess; }
+hipError_t hipStreamSynchronize(hipStream_t s) { (void)s->id; return hipSuccess; }
+hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned) { (void)s->id; (void)e->x; return hipSuccess; }
+hipError_t hipEventCreate(hipEvent_t *e) { *e = new ihipEvent_t{0}; return hipSuccess; }
+hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { *e = new ihipEvent_t{0}; return hipSuccess; }
+hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) { (void)e->x; if (s) (void)s->id; return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t e) { (void)e->x; return hipSuccess; }
+hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) { (void)a->x; (void)b->x; *ms = (float)(rand() % 100 + 1) * 0.01f; return hipSuccess; }
+hipError_t hipSetDevice(int) { return hipSuccess; }
+hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
+hipError_t hipDeviceSynchronize() { return hipSuccess; }
+hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t, int) { *v = 256; return hipSuccess; }
+hipError_t hipGetLastError() { return hipSuccess; }
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) { s->cap = new std::vector<std::function<void()>>; g_capture = s; return hipSuccess; }
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t *g) { *g = new ihipGraph{*s->cap}; delete s->cap; s->cap = nullptr; g_capture = nullptr; return hipSuccess; }
+hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, hipGraphNode_t *, char *, size_t) { *e = new hipGraphExec{g->ops}; return hipSuccess; }
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t s) { (void)s->id; for (auto &f : e->ops) f(); return hipSuccess; }
+hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }
+hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+
+namespace mpfft {
+static const int kL[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 64};   // as mp_fft.hip
+int limbs_for_precision(long P) { long need = (P + 31) / 32; for (int L : kL) if (L >= need) return L; return -1; }
+static volatile uint32_t sink;
+static void touch_r(const uint32_t *p, size_t n) { uint32_t s = 0; for (size_t i = 0; i < n; i++) s += p[i]; sink = s; }
+static void touch_w(uint32_t *p, size_t n) { for (size_t i = 0; i < n; i++) p[i] = 0x5a5a5a5au; }
+hipError_t mp_fft_launch(int L, int, const LeafArgs &la, const LevelArgs *lv, int nlev, uint32_t batch, hipStream_t) {
+    const size_t w = 2 * (size_t)la.N * (L + 1);
+    LeafArgs a = la; std::vector<LevelArgs> v(lv, lv + nlev);
+    run_or_record([=] {
+        for (uint32_t b = 0; b < batch; b++) {
+            if (a.npos) { touch_r(a.in + a.in_stride * b, w); touch_w(a.data + a.stride * b, w); touch_w(a.aos + a.aos_stride * b, (size_t)2 * a.N * a.aos_words); }
+            for (auto &x : v) { touch_w(x.data + x.stride * b, w); touch_r(x.lut, 1); }
+        }
+    });
+    return hipSuccess;
+}
+hipError_t mp_unswap_scale_launch(int L, uint32_t *d, uint64_t stride, uint32_t N, int32_t, uint32_t batch, hipStream_t) {
+    run_or_record([=] { for (uint32_t b = 0; b < batch; b++) touch_w(d + stride * b, 2 * (size_t)N * (L + 1)); }); return hipSuccess; }
+hipError_t mp_err_launch(int, int, const uint32_t *, const uint32_t *, uint64_t, uint32_t, double *, uint32_t, hipStream_t) { return hipSuccess; }
+hipError_t mp_binop_launch(int, int, int, uint32_t, const uint32_t *, const uint32_t *, uint32_t *, hipStream_t) { return hipSuccess; }
+static void fx_touch(int L, const FxArgs &a, bool panel) {
+    const size_t w = 2 * (size_t)a.N * (L + 1);
+    for (uint32_t b = 0; b < a.batch; b++) { touch_r(a.in + a.in_stride * b, w); touch_w(a.out + a.out_stride * b, w); }
+    const size_t fw = 2 * (size_t)a.N * (L + 2) * (panel ? fxp_groups(a.batch) * kPanel : a.batch);
+    touch_w(a.buf0, fw); touch_w(a.buf1, fw); touch_r(a.tw, 1); touch_w(a.flag, 1);
+}
+hipError_t fx_fft_launch(int L, int, const FxArgs &a, int, int, hipStream_t) { FxArgs c = a; run_or_record([=] { fx_touch(L, c, false); }); return hipSuccess; }
+hipError_t fxp_fft_launch(int L, int, const FxArgs &a, int, int, hipStream_t) {
+    if (!fxp_eligible(a.N, a.batch)) return hipErrorInvalidValue;
+    FxArgs c = a; run_or_record([=] { fx_touch(L, c, true); }); return hipSuccess; }
+hipError_t dd_fft_launch(int K, int L, int, const FxArgs &a, int, int, hipStream_t) {
+    FxArgs c = a;
+    run_or_record([=] {
+        const size_t w = 2 * (size_t)c.N * (L + 1);
+        for (uint32_t b = 0; b < c.batch; b++) { touch_r(c.in + c.in_stride * b, w); touch_w(c.out + c.out_stride * b, w); }
+        const size_t fw = (size_t)K * 2 * c.N * c.batch * 2;      // doubles -> 32-bit words
+        touch_w(c.buf0, fw); touch_w(c.buf1, fw); touch_r(c.tw, 1); touch_w(c.flag, 1);
+    });
+    return hipSuccess;
+}
+hipError_t d_fft_launch(const DTables &, const double *, double *, uint64_t, uint32_t, hipStream_t) { return hipSuccess; }
+}
