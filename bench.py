@@ -137,6 +137,44 @@ def measure(M, I, torch, n, P, batch, steps, warmup, seed, form):
                 S=S, out0=out0, x0=x0, panel=panel, L=plan.L)
 
 
+FP64_PEAK_T = 78.6              # MI355X FP64 peak, TFLOP/s
+
+
+def word_fp64_ops(n, K):
+    """FP64 operations per transform of the word forms (an FMA counts 2),
+    counted from dd_fft.hip: TwoSum 6, Fast2Sum 3, TwoProd 3; double-word
+    sum 20, product 10; quad-word sum 78, product 106 (41 + 37 and 69 + 37
+    with the renormalisation); complex product 4 products + 2 sums, complex
+    sum 2 sums, DFT4 8 complex sums.  Radix-r butterflies as the kernels run
+    them (twiddle products skipped at k = 0; the radix-16 DFT's 8 internal
+    complex products), over the pass plan of kernels.h (mw_plan); the limb
+    conversions of the first and last pass are integer work, not counted."""
+    add, mul = (20, 10) if K == 2 else (78, 106)
+    cadd, cmul = 2 * add, 4 * mul + 2 * add
+    dft4 = 8 * cadd
+    lg = n.bit_length() - 1
+    plan = []
+    if K == 2 and lg >= 5:
+        plan.append(2 if lg & 1 else 4)
+        m = lg - (1 if lg & 1 else 2) - 2
+        if m % 4:
+            plan.append(4)
+        plan += [16] * (m // 4)
+        plan.append(4)
+    else:
+        if lg & 1:
+            plan.append(2)
+        while (1 << sum(r.bit_length() - 1 for r in plan)) < n:
+            plan.append(4)
+    ops, Ns = 0, 1
+    for R in plan:
+        base = {2: 2 * cadd, 4: dft4, 16: 8 * dft4 + 8 * cmul}[R]
+        tw = (R - 1) * cmul
+        ops += (n // R) * (base + tw * (Ns - 1) / Ns)
+        Ns *= R
+    return ops
+
+
 def measure_precision_form(M, I, torch, O, n, P, Pcmp, batch, steps, seed):
     """The double-word form (P <= 106 bits) beside the plain and the fixed
     form at Pcmp bits, all at N = n on `batch` device-resident transforms;
@@ -155,6 +193,12 @@ def measure_precision_form(M, I, torch, O, n, P, Pcmp, batch, steps, seed):
         out[name] = {"precision_bits": prec, "fft_per_s": round(r["fft_s"], 1),
                      "error_vs_2P_oracle": {"max_ulp": mu, "normwise_2^-P": nw,
                                             "zero_mismatch": zb}}
+        if form in (M.FORM_DD, M.FORM_QD):
+            ops = word_fp64_ops(n, 2 if form == M.FORM_DD else 4)
+            rate = ops * r["fft_s"] / 1e12
+            out[name]["fp64"] = {"ops_per_fft": int(ops), "achieved": round(rate, 2),
+                                 "peak": FP64_PEAK_T, "unit": "TFLOP/s",
+                                 "frac": round(rate / FP64_PEAK_T, 4)}
     out["double_word_vs_plain"] = round(out["double_word"]["fft_per_s"] /
                                         out["plain"]["fft_per_s"], 2)
     out["double_word_vs_fixed"] = round(out["double_word"]["fft_per_s"] /
