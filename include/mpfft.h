@@ -163,8 +163,11 @@ int mpfft_execute_limbs(mpfft_plan plan, const uint32_t *d_in, uint32_t *d_out,
 
 /* Part of the transform for profiling: for the plain forms stage 0 is the
  * leaf pass, stage i (1 <= i < mpfft_plan_stages) the combine pass of node
- * size 2^(i+2); for the fixed form stage i is the i-th radix-2/4 pass (the
- * first also converts the input, the last rounds the output).
+ * size 2^(i+2); for the fixed form on lanes and the double- and quad-word
+ * forms stage i is the i-th radix pass (the first also converts the input,
+ * the last rounds the output); for the fixed form on panels stage 0
+ * converts the input, the last stage the output, the ones between are the
+ * passes.
  * Running stages [0, mpfft_plan_stages) equals mpfft_execute_limbs. */
 int mpfft_execute_limbs_range(mpfft_plan plan, const uint32_t *d_in, uint32_t *d_out,
                               void *stream, int stage_begin, int stage_end);
