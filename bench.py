@@ -143,13 +143,13 @@ FP64_PEAK_T = 78.6              # MI355X FP64 peak, TFLOP/s
 def word_fp64_ops(n, K):
     """FP64 operations per transform of the word forms (an FMA counts 2),
     counted from dd_fft.hip: TwoSum 6, Fast2Sum 3, TwoProd 3; double-word
-    sum 20, product 10; quad-word sum 78, product 106 (41 + 37 and 69 + 37
+    sum 20, product 10; quad-word sum 71, product 99 (41 + 30 and 69 + 30
     with the renormalisation); complex product 4 products + 2 sums, complex
     sum 2 sums, DFT4 8 complex sums.  Radix-r butterflies as the kernels run
     them (twiddle products skipped at k = 0; the radix-16 DFT's 8 internal
     complex products), over the pass plan of kernels.h (mw_plan); the limb
     conversions of the first and last pass are integer work, not counted."""
-    add, mul = (20, 10) if K == 2 else (78, 106)
+    add, mul = (20, 10) if K == 2 else (71, 99)
     cadd, cmul = 2 * add, 4 * mul + 2 * add
     dft4 = 8 * cadd
     lg = n.bit_length() - 1

@@ -156,15 +156,13 @@ struct CQD {
     QD re, im;
 };
 
-// five overlapping words (roughly decreasing) -> four: a bottom-up TwoSum
+// four overlapping words (roughly decreasing) -> four: a bottom-up TwoSum
 // pass leaves the rounded total in c0 and each step's error below it, a
 // top-down pass pushes what each word holds beyond half an ulp of the one
-// above into the next
-__device__ __forceinline__ QD q_renorm(double c0, double c1, double c2, double c3, double c4)
+// above into the next (exact: the words' sum is unchanged)
+__device__ __forceinline__ QD q_renorm(double c0, double c1, double c2, double c3)
 {
-    DD s = two_sum(c3, c4);
-    c3 = s.hi; c4 = s.lo;
-    s = two_sum(c2, c3);
+    DD s = two_sum(c2, c3);
     c2 = s.hi; c3 = s.lo;
     s = two_sum(c1, c2);
     c1 = s.hi; c2 = s.lo;
@@ -174,7 +172,7 @@ __device__ __forceinline__ QD q_renorm(double c0, double c1, double c2, double c
     c1 = s.hi; c2 = s.lo;
     s = two_sum(c2, c3);
     c2 = s.hi; c3 = s.lo;
-    return {{c0, c1, c2, c3 + c4}};
+    return {{c0, c1, c2, c3}};
 }
 
 __device__ __forceinline__ QD q_add(const QD &a, const QD &b)
@@ -186,7 +184,7 @@ __device__ __forceinline__ QD q_add(const QD &a, const QD &b)
     const DD t2 = two_sum(s2.hi, s1.lo);              // order 2 ...
     const DD u2 = two_sum(t2.hi, t1.lo);              // ... total
     const double o3 = ((a.c[3] + b.c[3]) + s2.lo) + (t2.lo + u2.lo);
-    return q_renorm(s0.hi, t1.hi, u2.hi, o3, 0.0);
+    return q_renorm(s0.hi, t1.hi, u2.hi, o3);
 }
 
 __device__ __forceinline__ QD q_neg(const QD &x) { return {{-x.c[0], -x.c[1], -x.c[2], -x.c[3]}}; }
@@ -222,7 +220,7 @@ __device__ __forceinline__ QD q_mul(const QD &a, const QD &b)
     o3 = __builtin_fma(a.c[3], b.c[0], o3);
     o3 += ((p02.lo + p11.lo) + (p20.lo + s2.lo)) + ((s2b.lo + s2c.lo) + (s2d.lo + s2e.lo)) +
           s2f.lo;
-    return q_renorm(p00.hi, s1b.hi, s2f.hi, o3, 0.0);
+    return q_renorm(p00.hi, s1b.hi, s2f.hi, o3);
 }
 
 // ---- word-type traits: one real of K binary64 words ---------------------------
@@ -319,7 +317,7 @@ __device__ __forceinline__ uint32_t qd_from_mp(QD &z, const Mp<L> &x, int lim)
         const double v = __builtin_ldexp((double)(top >> 11), x.e - 53 * (i + 1));
         c[i] = x.neg ? -v : v;
     }
-    z = q_renorm(c[0], c[1], c[2], c[3], 0.0);
+    z = q_renorm(c[0], c[1], c[2], c[3]);
     return 0;
 }
 
