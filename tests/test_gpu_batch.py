@@ -122,6 +122,22 @@ def test_batch_panels_any_count(count):
     bl.close()
 
 
+def test_batch_fixed_small_sizes():
+    # the fixed form's edge sizes in one batch: n = 1 .. 16 (per-lane kernels
+    # below 16), a 2048-bit group (panels only), precisions not a multiple
+    # of 32 bits; each transform against the 2P-bit oracle
+    ns = np.array([1, 2, 4, 8, 16, 16, 32, 4], dtype=np.int64)
+    Ps = np.array([128, 256, 64, 512, 1024, 100, 2048, 1024], dtype=np.int64)
+    xs = [I.mp_random(int(n), int(P), seed=1700 + i) for i, (n, P) in enumerate(zip(ns, Ps))]
+    b = M.Batch(ns, Ps, M.FORM_FIXED)
+    ys = run_batch(b, xs)
+    forms = b.forms()
+    assert forms[6] == M.FORM_FIXED_PANEL and forms[0] == M.FORM_FIXED_LANE
+    for x, y, n, P in zip(xs, ys, ns, Ps):
+        oracle_check(int(n), int(P), x, y)
+    b.close()
+
+
 def test_fixed_plan_partial_panel():
     # MPFFT_FORM_FIXED takes the panels at 120 of 128 (padding 8 <= 120 / 8)
     n, P, B = 1024, 128, 120
