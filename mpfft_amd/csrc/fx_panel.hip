@@ -471,10 +471,10 @@ fxp_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
 //   u'_(r1, 1) = w8^r1 u_(r1, 1)                     (r1 = 1, 3: two real products
 //                                                     by 1/sqrt(2), matrix cores)
 //   y_(q0 + 2 q1) = sum_r1 (-i)^(r1 q1) u'_(r1, q0)  (DFT4 of four slots)
-// LDS: eight slots of 2 W words x 64 lanes (139 KB at W = 34, one block per
-// CU, two waves per SIMD), the table of 1/sqrt(2) and, where a product's
-// twiddle tables do not fit at the end of a slot (W = 8), one table area per
-// wave.
+// LDS: two sets of eight slots of 2 W words x 64 lanes (148 KB at W = 18,
+// one block per CU, two waves per SIMD), the table of 1/sqrt(2) and, where a
+// product's twiddle tables do not fit at the end of a slot (W = 8), one
+// table area per wave.
 template <int W>
 __global__ void __launch_bounds__(512, 1)
 fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t toff,
@@ -485,9 +485,13 @@ fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
     constexpr uint32_t SLOT = 2 * W * 64, TWN = 3 * G.RW, TWL = (TWN + 63) / 64;
     const uint32_t N = a.N, E8 = N >> 3;
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
-    uint32_t *c8t = lds + 8 * SLOT;
+    // two sets of eight slots: the products and stage 1 use set A, stage 2
+    // writes and stage 3 reads set B, so an item needs two barriers
+    constexpr uint32_t SETB = 8 * SLOT;
+    uint32_t *c8t = lds + 16 * SLOT;
     for (uint32_t i = threadIdx.x; i < (uint32_t)G.RW; i += blockDim.x)
         c8t[i] = a.tw[(size_t)c8e * TWN + i];
+    __syncthreads();                                   // c8t staged
     uint32_t it = blockIdx.x;
     if (it >= nitems) return;
     auto item = [&](uint32_t i, uint32_t &j, uint32_t &grp, uint32_t &rot, uint32_t &r,
@@ -512,12 +516,18 @@ fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
         uint32_t j, grp, rot, r, k;
         item(it, j, grp, rot, r, k);
         const uint32_t nx = it + gridDim.x;
-        // slot of the wave holding input s: wave (s - rot) mod 8
-        auto sl = [&](uint32_t s_) { return lds + ((s_ - rot) & 7u) * SLOT + lane; };
-        uint32_t *slot = lds + wv * SLOT;
+        // slot of the wave holding input s: wave (s - rot) mod 8.  No barrier
+        // is needed before this item's writes: set A was last read in the
+        // previous item's stage 1, and a wave gets here only after that
+        // item's second barrier, which every wave reaches after its stage 1;
+        // set B (read in the previous item's stage 3) is written only after
+        // this item's first barrier, which every wave reaches after its
+        // stage 3.
+        auto sl = [&](uint32_t s_) { return lds + SETB + ((s_ - rot) & 7u) * SLOT + lane; };
+        auto sla = [&](uint32_t s_) { return lds + ((s_ - rot) & 7u) * SLOT + lane; };
+        uint32_t *slot = lds + wv * SLOT, *slotb = lds + SETB + wv * SLOT;
         // twiddle tables at the end of the slot, or after the slots and c8t
-        uint32_t *R = mf_tail_fits(W) ? slot + SLOT - TWN : lds + 8 * SLOT + G.RW + wv * TWN;
-        __syncthreads();   // the slots of the last item are read (first item: c8t staged)
+        uint32_t *R = mf_tail_fits(W) ? slot + SLOT - TWN : lds + 16 * SLOT + G.RW + wv * TWN;
         const bool prod = r != 0 && k != 0;
         MfOperand<W> B;
         if (prod) {
@@ -541,7 +551,7 @@ fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
         // stage 1: u = v_r + v_(r+4) (r < 4) or v_(r-4) - v_r (r >= 4)
         Fx<W> ur, ui;
         {
-            const uint32_t *lo = sl(r & 3u), *hi = sl((r & 3u) + 4);
+            const uint32_t *lo = sla(r & 3u), *hi = sla((r & 3u) + 4);
             uint32_t cr = 0, ci = 0;
             if (r < 4) {
 #pragma unroll
@@ -557,8 +567,7 @@ fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
                 }
             }
         }
-        __syncthreads();
-        // stage 2: u' = w8^r1 u for the odd half (r = 4 + r1)
+        // stage 2 (into set B): u' = w8^r1 u for the odd half (r = 4 + r1)
         if (r == 5 || r == 7) {
             // w8 (a + ib) = ((a + b) + i (b - a)) / sqrt2; w8^3 (a + ib) = ((b - a) - i (a + b)) / sqrt2
             Fx<W> s_, d_;
@@ -574,7 +583,7 @@ fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
             }
             MfOperand<W> Bq;
             mf_operand<W>(Bq, s_, d_);
-            mf_cmul<W, true>(Bq, c8t, c8t, c8t, lane, slot);
+            mf_cmul<W, true>(Bq, c8t, c8t, c8t, lane, slotb);
         } else {
             if (r == 6) {                     // -i (a + ib) = b - ia
                 Fx<W> z, t_;
@@ -585,9 +594,9 @@ fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
                 ui = t_;
             }
 #pragma unroll
-            for (int i = 0; i < W; i++) slot[i * 64 + lane] = ur.w[i];
+            for (int i = 0; i < W; i++) slotb[i * 64 + lane] = ur.w[i];
 #pragma unroll
-            for (int i = 0; i < W; i++) slot[(W + i) * 64 + lane] = ui.w[i];
+            for (int i = 0; i < W; i++) slotb[(W + i) * 64 + lane] = ui.w[i];
         }
         // the next item's operands load during stage 3 and the stores (issued
         // here, after both products, to keep them out of the products'
@@ -677,7 +686,7 @@ static hipError_t launch_fxp(const FxArgs &a, int s0, int s1, hipStream_t st)
                                    (const uint32_t *)src, dst, P.Ns, P.toff, nitems);
             } else if constexpr (W <= kPanelR8MaxW) {
                 const size_t sh =
-                    sizeof(uint32_t) * (8 * 2 * W * 64 + G.RW + 8 * mf_table_words(W));
+                    sizeof(uint32_t) * (16 * 2 * W * 64 + G.RW + 8 * mf_table_words(W));
                 const uint32_t nitems = (N / 8) * NG;
                 const uint32_t nblk = nitems < a.ncu ? nitems : a.ncu;
                 hipLaunchKernelGGL((fxp_r8_kernel<W>), dim3(nblk), dim3(512), sh, st, a,
