@@ -66,62 +66,105 @@ __device__ __forceinline__ uint32_t xcd_block()
     return (nb & 7u) == 0 ? (bid & 7u) * (nb >> 3) + (bid >> 3) : bid;
 }
 
-// word w of transform b at element e in a panel buffer: pan_base + w kPanel
-__device__ __forceinline__ size_t pan_base(uint32_t b, uint32_t e, uint32_t N, int W)
+// Inside the W x 64 words of one element of a panel, word w of the panel's
+// transform l sits at pan_word(w) + pan_lane(l): for even W as word pairs,
+// (w / 2) 128 + 2 l + w % 2, so a lane moves its words 8 bytes at a time
+// and a wave 512 contiguous bytes; for odd W word-major, 64 w + l.
+template <int W> constexpr bool kPairs = W % 2 == 0;
+
+template <int W>
+__device__ __forceinline__ uint32_t pan_word(int w)
 {
-    return ((size_t)(b / kPanel) * (2 * (size_t)N) + e) * (size_t)W * kPanel + (b % kPanel);
+    return kPairs<W> ? (uint32_t)((w >> 1) * 2 * kPanel + (w & 1)) : (uint32_t)(w * kPanel);
+}
+
+template <int W>
+__device__ __forceinline__ uint32_t pan_lane(uint32_t l)
+{
+    return kPairs<W> ? 2 * l : l;
+}
+
+// element e of panel grp: its W x 64 words start here
+__device__ __forceinline__ size_t pan_elem(uint32_t grp, uint32_t e, uint32_t N, int W)
+{
+    return ((size_t)grp * (2 * (size_t)N) + e) * (size_t)W * kPanel;
+}
+
+// word w of transform b at element e in a panel buffer: pan_base + pan_word(w)
+template <int W>
+__device__ __forceinline__ size_t pan_base(uint32_t b, uint32_t e, uint32_t N)
+{
+    return pan_elem(b / kPanel, e, N, W) + pan_lane<W>(b % kPanel);
+}
+
+// the W words of one real (W x 64 words of the element, lane part included)
+template <int W>
+__device__ __forceinline__ void pan_get(Fx<W> &x, const uint32_t *p)
+{
+    if constexpr (kPairs<W>) {
+#pragma unroll
+        for (int j = 0; j < W / 2; j++) {
+            const uint2 v = *reinterpret_cast<const uint2 *>(p + j * 2 * kPanel);
+            x.w[2 * j] = v.x;
+            x.w[2 * j + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < W; j++) x.w[j] = p[j * kPanel];
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void pan_put(uint32_t *p, const Fx<W> &x)
+{
+    if constexpr (kPairs<W>) {
+#pragma unroll
+        for (int j = 0; j < W / 2; j++)
+            *reinterpret_cast<uint2 *>(p + j * 2 * kPanel) = make_uint2(x.w[2 * j], x.w[2 * j + 1]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < W; j++) p[j * kPanel] = x.w[j];
+    }
 }
 
 template <int W>
 __device__ __forceinline__ void pan_load(Fx<W> &re, Fx<W> &im, const uint32_t *buf, uint32_t b,
                                          uint32_t e, uint32_t N)
 {
-    const uint32_t *p = buf + pan_base(b, e, N, W);
-#pragma unroll
-    for (int j = 0; j < W; j++) re.w[j] = p[j * kPanel];
-    p += (size_t)N * W * kPanel;
-#pragma unroll
-    for (int j = 0; j < W; j++) im.w[j] = p[j * kPanel];
+    const uint32_t *p = buf + pan_base<W>(b, e, N);
+    pan_get<W>(re, p);
+    pan_get<W>(im, p + (size_t)N * W * kPanel);
 }
 
 template <int W>
 __device__ __forceinline__ void pan_store(const Fx<W> &re, const Fx<W> &im, uint32_t *buf,
                                           uint32_t b, uint32_t e, uint32_t N)
 {
-    uint32_t *p = buf + pan_base(b, e, N, W);
-#pragma unroll
-    for (int j = 0; j < W; j++) p[j * kPanel] = re.w[j];
-    p += (size_t)N * W * kPanel;
-#pragma unroll
-    for (int j = 0; j < W; j++) p[j * kPanel] = im.w[j];
+    uint32_t *p = buf + pan_base<W>(b, e, N);
+    pan_put<W>(p, re);
+    pan_put<W>(p + (size_t)N * W * kPanel, im);
 }
 
 // The same element e of the 64 transforms of group grp (one panel): lane l
 // holds transform 64 grp + l.  The element's words are one contiguous block
-// of W x 256 bytes; every address is a wave-uniform base plus 4 lane bytes
-// plus an immediate word offset.
+// of W x 256 bytes; every address is a wave-uniform base plus the lane part
+// plus an immediate offset.
 template <int W>
 __device__ __forceinline__ void row_load(Fx<W> &re, Fx<W> &im, const uint32_t *buf, uint32_t grp,
                                          uint32_t e, uint32_t N, uint32_t lane)
 {
-    const uint32_t *p = buf + ((size_t)grp * (2 * (size_t)N) + e) * (size_t)W * kPanel;
-#pragma unroll
-    for (int j = 0; j < W; j++) re.w[j] = (p + j * kPanel)[lane];
-    p += (size_t)N * W * kPanel;
-#pragma unroll
-    for (int j = 0; j < W; j++) im.w[j] = (p + j * kPanel)[lane];
+    const uint32_t *p = buf + pan_elem(grp, e, N, W);
+    pan_get<W>(re, p + pan_lane<W>(lane));
+    pan_get<W>(im, p + (size_t)N * W * kPanel + pan_lane<W>(lane));
 }
 
 template <int W>
 __device__ __forceinline__ void row_store(const Fx<W> &re, const Fx<W> &im, uint32_t *buf,
                                           uint32_t grp, uint32_t e, uint32_t N, uint32_t lane)
 {
-    uint32_t *p = buf + ((size_t)grp * (2 * (size_t)N) + e) * (size_t)W * kPanel;
-#pragma unroll
-    for (int j = 0; j < W; j++) (p + j * kPanel)[lane] = re.w[j];
-    p += (size_t)N * W * kPanel;
-#pragma unroll
-    for (int j = 0; j < W; j++) (p + j * kPanel)[lane] = im.w[j];
+    uint32_t *p = buf + pan_elem(grp, e, N, W);
+    pan_put<W>(p + pan_lane<W>(lane), re);
+    pan_put<W>(p + (size_t)N * W * kPanel + pan_lane<W>(lane), im);
 }
 
 // Output R of the DFT4 over the four slots: with a = v0 +- v2, c = v1 +- v3
@@ -344,9 +387,11 @@ fxp_in_kernel(FxArgs a, uint32_t *dst)
     __syncthreads();
     // lane = (transform t % 16, element t / 16): 64-byte runs of a panel row
     const uint32_t bl = t & 15u, el = t >> 4;
-    uint32_t *p = dst + pan_base(g16 * 16 + bl, eb * 16 + el, N, W);
+    uint32_t *p = dst + pan_base<W>(g16 * 16 + bl, eb * 16 + el, N);
+    Fx<W> z;
 #pragma unroll
-    for (int w = 0; w < W; w++) p[w * kPanel] = tile[(w * 16 + el) * 17 + bl];
+    for (int w = 0; w < W; w++) z.w[w] = tile[(w * 16 + el) * 17 + bl];
+    pan_put<W>(p, z);
 }
 
 // A radix-2 pass without twiddles (the first pass when log2 N is odd): a
@@ -633,10 +678,9 @@ fxp_out_kernel(FxArgs a, const uint32_t *src)
     {
         // lane = (transform t % 16, element t / 16): 64-byte runs of a panel row
         const uint32_t bl = t & 15u, el = t >> 4;
-        const uint32_t *p = src + pan_base(g16 * 16 + bl, eb * 16 + el, N, W);
+        const uint32_t *p = src + pan_base<W>(g16 * 16 + bl, eb * 16 + el, N);
         Fx<W> x;
-#pragma unroll
-        for (int w = 0; w < W; w++) x.w[w] = p[w * kPanel];
+        pan_get<W>(x, p);
         Mp<L> z;
         fx_to_mp<W, L, FULL>(z, x, a.P);
         tile[el * 17 + bl] = z.zero ? ((uint32_t)kHdrZero | z.neg) : (((uint32_t)z.e << 1) | z.neg);

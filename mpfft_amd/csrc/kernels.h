@@ -100,9 +100,10 @@ hipError_t fx_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStrea
 
 // ---- fixed form, panel kernels (fx_panel.hip) ----
 //
-// Transforms are interleaved in panels of kPanel: word w of element e of
-// transform b sits at ((b / 64) 2N + e) W 64 + 64 w + b % 64, so one element
-// of 64 transforms is a contiguous block of W x 256 bytes.  A twiddle
+// Transforms are interleaved in panels of kPanel: element e of the 64
+// transforms of panel g is a contiguous block of W x 256 bytes at
+// (g 2N + e) W 64 words (inside it, word pairs per transform for even W:
+// fx_panel.hip, pan_word).  A twiddle
 // product runs on the int8 matrix cores: the twiddle t (|t| <= 1) and the
 // operand x are written in balanced base-256 digits (each in [-128, 127]),
 // and the columns of x t are a Toeplitz matrix of t's digits times the
