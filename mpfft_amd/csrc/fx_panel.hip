@@ -356,42 +356,50 @@ __device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *R
 }  // namespace
 
 // Input conversion: float SoA -> fixed point in the panel layout, through an
-// LDS transpose so that both sides move 64-byte runs.  A block covers 16
-// elements (of the 2N re/im positions) of 16 transforms; blocks are ordered
-// transform group fastest, so the four blocks that fill one 256-byte panel
-// row run next to each other on one XCD.
+// LDS transpose.  A block covers 32 elements (of the 2N re/im positions) of
+// 16 transforms, a thread converts two reals: the limb side reads 128-byte
+// runs (32 elements), the panel side writes 128-byte runs (16 transforms'
+// word pairs).  Blocks are ordered transform group fastest, so the four
+// blocks that fill one panel row run next to each other on one XCD.
 template <int L, int W>
 __global__ void __launch_bounds__(256)
 fxp_in_kernel(FxArgs a, uint32_t *dst)
 {
-    __shared__ uint32_t tile[W * 16 * 17];
+    __shared__ uint32_t tile[W * 32 * 17];
     const uint32_t N = a.N, M = 2 * N, t = threadIdx.x;
     const uint32_t lb = xcd_block(), ng = 4 * fxp_groups(a.batch);
     const uint32_t g16 = lb % ng, eb = lb / ng;
     {
-        // lane = (element t % 16, transform t / 16): 64-byte runs of the input;
-        // the padding transforms of a last, partial panel are zeros
-        const uint32_t el = t & 15u, bl = t >> 4, b = g16 * 16 + bl;
-        Fx<W> z;
-        if (b < a.batch) {
-            Mp<L> x;
-            mp_load(x, a.in + a.in_stride * b, M, eb * 16 + el);
-            if (fx_from_mp<L, W>(z, x, a.lim)) atomicOr(a.flag, 1u);
-        } else {
+        // lane = (element t % 32, transform t / 32 and that + 8); the padding
+        // transforms of a last, partial panel are zeros
+        const uint32_t el = t & 31u;
+#pragma unroll 1
+        for (uint32_t bl = t >> 5; bl < 16; bl += 8) {
+            const uint32_t b = g16 * 16 + bl;
+            Fx<W> z;
+            if (b < a.batch) {
+                Mp<L> x;
+                mp_load(x, a.in + a.in_stride * b, M, eb * 32 + el);
+                if (fx_from_mp<L, W>(z, x, a.lim)) atomicOr(a.flag, 1u);
+            } else {
 #pragma unroll
-            for (int w = 0; w < W; w++) z.w[w] = 0u;
+                for (int w = 0; w < W; w++) z.w[w] = 0u;
+            }
+#pragma unroll
+            for (int w = 0; w < W; w++) tile[(w * 32 + el) * 17 + bl] = z.w[w];
         }
-#pragma unroll
-        for (int w = 0; w < W; w++) tile[(w * 16 + el) * 17 + bl] = z.w[w];
     }
     __syncthreads();
-    // lane = (transform t % 16, element t / 16): 64-byte runs of a panel row
-    const uint32_t bl = t & 15u, el = t >> 4;
-    uint32_t *p = dst + pan_base<W>(g16 * 16 + bl, eb * 16 + el, N);
-    Fx<W> z;
+    // lane = (transform t % 16, element t / 16 and that + 16)
+    const uint32_t bl = t & 15u;
+#pragma unroll 1
+    for (uint32_t el = t >> 4; el < 32; el += 16) {
+        uint32_t *p = dst + pan_base<W>(g16 * 16 + bl, eb * 32 + el, N);
+        Fx<W> z;
 #pragma unroll
-    for (int w = 0; w < W; w++) z.w[w] = tile[(w * 16 + el) * 17 + bl];
-    pan_put<W>(p, z);
+        for (int w = 0; w < W; w++) z.w[w] = tile[(w * 32 + el) * 17 + bl];
+        pan_put<W>(p, z);
+    }
 }
 
 // A radix-2 pass without twiddles (the first pass when log2 N is odd): a
@@ -666,34 +674,44 @@ fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
 }
 
 // Output conversion: panel layout -> float SoA rounded to P bits, through
-// an LDS transpose (64-byte runs on both sides; block order as fxp_in_kernel).
+// an LDS transpose.  A block covers 32 elements of 16 transforms (block order
+// as fxp_in_kernel), a thread converts two reals: the panel side reads
+// 128-byte runs (16 transforms' word pairs), the limb side writes 128-byte
+// runs (32 elements; with 16 elements, 64-byte runs measured slower).
 template <int W, int L, bool FULL>
 __global__ void __launch_bounds__(256)
 fxp_out_kernel(FxArgs a, const uint32_t *src)
 {
-    __shared__ uint32_t tile[(L + 1) * 16 * 17];
+    __shared__ uint32_t tile[(L + 1) * 32 * 17];
     const uint32_t N = a.N, M = 2 * N, t = threadIdx.x;
     const uint32_t lb = xcd_block(), ng = (a.batch + 15) / 16;
     const uint32_t g16 = lb % ng, eb = lb / ng;
     {
-        // lane = (transform t % 16, element t / 16): 64-byte runs of a panel row
-        const uint32_t bl = t & 15u, el = t >> 4;
-        const uint32_t *p = src + pan_base<W>(g16 * 16 + bl, eb * 16 + el, N);
-        Fx<W> x;
-        pan_get<W>(x, p);
-        Mp<L> z;
-        fx_to_mp<W, L, FULL>(z, x, a.P);
-        tile[el * 17 + bl] = z.zero ? ((uint32_t)kHdrZero | z.neg) : (((uint32_t)z.e << 1) | z.neg);
+        // lane = (transform t % 16, element t / 16 and that + 16)
+        const uint32_t bl = t & 15u;
+#pragma unroll 1
+        for (uint32_t el = t >> 4; el < 32; el += 16) {
+            const uint32_t *p = src + pan_base<W>(g16 * 16 + bl, eb * 32 + el, N);
+            Fx<W> x;
+            pan_get<W>(x, p);
+            Mp<L> z;
+            fx_to_mp<W, L, FULL>(z, x, a.P);
+            tile[el * 17 + bl] =
+                z.zero ? ((uint32_t)kHdrZero | z.neg) : (((uint32_t)z.e << 1) | z.neg);
 #pragma unroll
-        for (int j = 0; j < L; j++) tile[((j + 1) * 16 + el) * 17 + bl] = z.zero ? 0u : z.m[j];
+            for (int j = 0; j < L; j++) tile[((j + 1) * 32 + el) * 17 + bl] = z.zero ? 0u : z.m[j];
+        }
     }
     __syncthreads();
-    // lane = (element t % 16, transform t / 16): 64-byte runs of the output
-    const uint32_t el = t & 15u, bl = t >> 4;
-    if (g16 * 16 + bl >= a.batch) return;           // padding transform
-    uint32_t *o = a.out + a.out_stride * (g16 * 16 + bl) + eb * 16 + el;
+    // lane = (element t % 32, transform t / 32 and that + 8)
+    const uint32_t el = t & 31u;
+#pragma unroll 1
+    for (uint32_t bl = t >> 5; bl < 16; bl += 8) {
+        if (g16 * 16 + bl >= a.batch) break;          // padding transforms
+        uint32_t *o = a.out + a.out_stride * (g16 * 16 + bl) + eb * 32 + el;
 #pragma unroll
-    for (int j = 0; j <= L; j++) o[(size_t)j * M] = tile[(j * 16 + el) * 17 + bl];
+        for (int j = 0; j <= L; j++) o[(size_t)j * M] = tile[(j * 32 + el) * 17 + bl];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -707,7 +725,7 @@ static hipError_t launch_fxp(const FxArgs &a, int s0, int s1, hipStream_t st)
     const uint32_t NG = fxp_groups(a.batch);          // panels, the last maybe partial
     // stage 0: input conversion into buf1 (every panel row, padding included)
     if (in_range(0))
-        hipLaunchKernelGGL((fxp_in_kernel<L, W>), dim3((2 * N / 16) * 4 * NG), dim3(256), 0, st,
+        hipLaunchKernelGGL((fxp_in_kernel<L, W>), dim3((2 * N / 32) * 4 * NG), dim3(256), 0, st,
                            a, src);
     FxpPass ps[32];
     uint32_t nent = 0;
@@ -743,7 +761,7 @@ static hipError_t launch_fxp(const FxArgs &a, int s0, int s1, hipStream_t st)
     }
     // the last stage: output conversion
     if (in_range(stage))
-        hipLaunchKernelGGL((fxp_out_kernel<W, L, FULL>), dim3((2 * N / 16) * ((a.batch + 15) / 16)),
+        hipLaunchKernelGGL((fxp_out_kernel<W, L, FULL>), dim3((2 * N / 32) * ((a.batch + 15) / 16)),
                            dim3(256), 0, st, a, (const uint32_t *)src);
     return hipGetLastError();
 }
