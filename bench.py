@@ -13,7 +13,7 @@ baseline and the drop-in call from mpfr_t arrays.
 
 Beside the headline line: the double-word FP64 form at 106 bits against
 the plain and fixed forms at 128 bits ("precision_form"), N = 2^20 at 512
-and 2048 bits ("wide"), and the batched
+and 2048 bits, fixed against plain ("wide"), and the batched
 workload of C transforms of uneven size and precision ("batched"), one
 graph launch of the batched forms against the plain form run one transform
 at a time.
@@ -211,19 +211,20 @@ def measure_precision_form(M, I, torch, O, n, P, Pcmp, batch, steps, seed):
 
 
 def measure_wide(M, I, torch, lgn, steps, seed):
-    """N = 2^lgn at 512 bits (fixed form on panels and plain form, batch 64)
-    and at 2048 bits (fixed form on panels, the only form there; batch 1 in
-    a padded panel).  One generated transform is copied into every slot of
-    the batch on the device (the kernels' time does not depend on the
-    values)."""
+    """N = 2^lgn at 512 bits (batch 64) and at 2048 bits (batch 64 and
+    batch 1): the fixed form (on panels) against the plain form (the
+    reference's op order, 64-limb kernels at 2048 bits).  One generated
+    transform is copied into every slot of the batch on the device (the
+    kernels' time does not depend on the values)."""
     n = 1 << lgn
-    out = {"workload": f"N=2^{lgn}: 512 bits batch 64 (fixed, plain), 2048 bits batch 1 (fixed)"}
+    out = {"workload": f"N=2^{lgn}: 512 bits batch 64, 2048 bits batch 64 and batch 1; "
+                       "fixed (panels) and plain"}
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
-    for P, B, forms in ((512, 64, (("fixed", M.FORM_FIXED), ("plain", M.FORM_MP))),
-                        (2048, 1, (("fixed", M.FORM_FIXED),))):
+    both = (("fixed", M.FORM_FIXED), ("plain", M.FORM_MP))
+    for P, B in ((512, 64), (2048, 64), (2048, 1)):
         x1 = I.mp_random(n, P, seed=seed)
-        for name, form in forms:
+        for name, form in both:
             plan = M.Plan(n, P, form, B)
             one = torch.from_numpy(I.relimb(x1, n, I.limbs_for(P), plan.L).view(np.int32)).cuda()
             x = one.repeat(B)
@@ -239,12 +240,15 @@ def measure_wide(M, I, torch, lgn, steps, seed):
             torch.cuda.synchronize()
             plan.check()
             ms = e0.elapsed_time(e1) / steps
-            out[f"{name}_P{P}"] = {"batch": B, "ms_per_step": round(ms, 3),
-                                   "fft_per_s": round(B / ms * 1e3, 2)}
+            key = f"{name}_P{P}" + ("" if B == 64 else f"_batch{B}")
+            out[key] = {"batch": B, "ms_per_step": round(ms, 3),
+                        "fft_per_s": round(B / ms * 1e3, 2)}
             plan.close()
             del x, y
-    out["fixed_vs_plain_P512"] = round(out["fixed_P512"]["fft_per_s"] /
-                                       out["plain_P512"]["fft_per_s"], 3)
+            torch.cuda.empty_cache()
+    for suf in ("P512", "P2048", "P2048_batch1"):
+        out[f"fixed_vs_plain_{suf}"] = round(out[f"fixed_{suf}"]["fft_per_s"] /
+                                             out[f"plain_{suf}"]["fft_per_s"], 3)
     return out
 
 

@@ -53,7 +53,7 @@ extern "C" {
  * lab/parallel/mp/parallel_mp_fft2.c:121).  Supported call: the top-level
  * (n = N, s = 1, sn = 0), as every caller of the reference makes it
  * (mpFFT.c:47/50).  All four arrays must hold N elements of one precision
- * P (<= 1024 bits); outputs are written into outr/outi in place. */
+ * P (<= 2048 bits); outputs are written into outr/outi in place. */
 void mp_sr24_cp_fft(unsigned long n, unsigned long s, long int sn,
                     mpfr_t *inr, mpfr_t *ini, mpfr_t *outr, mpfr_t *outi);
 
@@ -109,10 +109,12 @@ typedef struct mpfft_plan_s *mpfft_plan;
                               products on the int8 matrix cores; any batch
                               (a last, partial panel is padded), n >= 16
                               (MPFFT_ERR_SIZE below); same format and error
-                              bound.  The only form beyond 1024 bits: prec
-                              up to 2048 (64 limbs), which MPFFT_FORM_FIXED
-                              then runs on panels at any batch; every other
-                              form returns MPFFT_ERR_PREC above 1024      */
+                              bound.  The fixed form's only kernels beyond
+                              1024 bits: prec up to 2048 (64 limbs), which
+                              MPFFT_FORM_FIXED then runs on panels at any
+                              batch (MPFFT_FORM_FIXED_LANE, _DD and _QD
+                              return MPFFT_ERR_PREC above 1024; the plain
+                              op orders run up to 2048)                   */
 #define MPFFT_FORM_DD 11  /* double-word form, 53 <= prec <= 106: every
                               real is hi + lo (two binary64) and every
                               operation an error-free transformation on the
@@ -282,14 +284,14 @@ int mpfft_hex_to_limbs(unsigned long count, long limbs, long prec, const char *t
 
 /* ---- device arithmetic check ---------------------------------------------- */
 
-/* z[i] = x[i] op y[i] correctly rounded at prec (<= 1024) bits on the GPU; op 0 add,
+/* z[i] = x[i] op y[i] correctly rounded at prec (<= 2048) bits on the GPU; op 0 add,
  * 1 sub, 2 mul; x, y, z: host SoA-limb arrays of `count` reals at
  * mpfft_limbs_for_precision(prec) limbs (hdr[count] then limbs). */
 int mpfft_mp_binop_host(int op, unsigned long count, long prec,
                         const uint32_t *x, const uint32_t *y, uint32_t *z);
 /* Limbs per real of the device format for prec bits: the smallest of 1, 2,
  * 3, 4, 6, 8, 12, 16, 24, 32, 64 that holds prec bits; -1 outside [2, 2048].
- * (64 limbs: the fixed form on panels only.) */
+ * (64 limbs: the plain op orders and the fixed form on panels.) */
 long mpfft_limbs_for_precision(long prec);
 
 #ifdef __cplusplus

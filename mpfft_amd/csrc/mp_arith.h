@@ -254,45 +254,51 @@ MPFFT_FN void mp_add(Mp<L> &z, const Mp<L> &x, uint32_t fx,
 
 // ---- multiplication ----------------------------------------------------
 //
-// The significands are re-cut into 29-bit digits so that a whole product
+// The significands are re-cut into B-bit digits so that a whole product
 // column  sum_i A_i * B_(c-i)  accumulates in one 64-bit register with
-// v_mad_u64_u32 and no carry handling: a column has at most D <= 63 terms
-// below 2^58 each plus a carry-in below 2^35, which stays below 2^64.  One
-// mad per digit product; the carry moves on once per column.
+// v_mad_u64_u32 and no carry handling: a column has at most D terms below
+// 2^(2B) each plus a carry-in below 2^(64-B), which stays below 2^64 while
+// D < 2^(64-2B).  B = 29 up to 63 digits (L <= 57); B = 28 beyond (74
+// digits at L = 64, where 256 would fit).  One mad per digit product; the
+// carry moves on once per column.
 
 constexpr int kDigitBits = 29;
 constexpr uint32_t kDigitMask = (1u << kDigitBits) - 1u;
 
 template <int L>
 struct Digits {
-    static constexpr int D = (32 * L + kDigitBits - 1) / kDigitBits;
-    static_assert(D <= 63, "column sums would overflow 64 bits");
+    static constexpr int B = 32 * L <= 63 * kDigitBits ? kDigitBits : 28;
+    static constexpr uint32_t MASK = (1u << B) - 1u;
+    static constexpr int D = (32 * L + B - 1) / B;
+    static_assert(D < (1 << (64 - 2 * B)), "column sums would overflow 64 bits");
 };
 
-// MS-first 32-bit limbs -> LS-first 29-bit digits
+// MS-first 32-bit limbs -> LS-first B-bit digits
 template <int L>
 MPFFT_FN void to_digits(const uint32_t (&m)[L], uint32_t (&d)[Digits<L>::D])
 {
+    constexpr int B = Digits<L>::B;
 #pragma unroll
     for (int k = 0; k < Digits<L>::D; k++) {
-        const int bit = kDigitBits * k, w = bit >> 5, s = bit & 31;
+        const int bit = B * k, w = bit >> 5, s = bit & 31;
         const uint32_t lo = w < L ? m[L - 1 - w] : 0u;
         const uint32_t hi = w + 1 < L ? m[L - 2 - w] : 0u;
-        d[k] = (s == 0 ? lo : fshr(hi, lo, (uint32_t)s)) & kDigitMask;
+        d[k] = (s == 0 ? lo : fshr(hi, lo, (uint32_t)s)) & Digits<L>::MASK;
     }
 }
 
-// Deposit digit c (value v, bits [29c, 29c+29) of the 64L-bit product) into
+// Deposit digit c (value v, bits [Bc, Bc+B) of the 64L-bit product) into
 // the window w[0..L] (product bits [32(L-1), 64L), MS-first) or into sticky.
 template <int L>
 MPFFT_FN void put_digit(uint32_t (&w)[L + 1], uint32_t &sticky, int c, uint32_t v)
 {
-    const int bit = kDigitBits * c, j = bit >> 5, s = bit & 31;
+    constexpr int B = Digits<L>::B;
+    const int bit = B * c, j = bit >> 5, s = bit & 31;
     // low part -> product word j, high part -> word j + 1
     const uint32_t lo = v << s;
     if (j < L - 1) sticky |= lo;
     else if (j < 2 * L) w[2 * L - 1 - j] |= lo;
-    if (s + kDigitBits > 32) {
+    if (s + B > 32) {
         const uint32_t hi = v >> (32 - s);
         if (j + 1 < L - 1) sticky |= hi;
         else if (j + 1 < 2 * L) w[2 * L - 2 - j] |= hi;
@@ -316,8 +322,8 @@ MPFFT_FN void mp_mul_exact(Mp<L> &z, const uint32_t (&A)[Digits<L>::D],
         const int i1 = c < D - 1 ? c : D - 1;
 #pragma unroll
         for (int i = i0; i <= i1; i++) acc = (uint64_t)A[i] * B[c - i] + acc;
-        put_digit<L>(w, sticky, c, (uint32_t)acc & kDigitMask);
-        acc >>= kDigitBits;
+        put_digit<L>(w, sticky, c, (uint32_t)acc & Digits<L>::MASK);
+        acc >>= Digits<L>::B;
     }
     put_digit<L>(w, sticky, 2 * D - 1, (uint32_t)acc);
     if (!(w[0] >> 31)) {
@@ -333,20 +339,25 @@ MPFFT_FN void mp_mul_exact(Mp<L> &z, const uint32_t (&A)[Digits<L>::D],
 }
 
 // Truncated product: columns below K0 are skipped.  Their sum is below
-// 2^E, E = 29*K0 + 36 (a column holds at most K0 < 64 terms below 2^58),
-// so the exact product lies in [T, T + 2^E) for the computed partial sum T.
+// 2^E, E = B*K0 + B + 7: column c < K0 holds at most K0 terms below 2^(2B),
+// so the sum is below K0 2^(2B) 2^(B(K0-1)) 2^B / (2^B - 1) < 2^(BK0+B+7)
+// for K0 <= 126 (at B = 29, E = 29*K0 + 36).  The exact product therefore
+// lies in [T, T + 2^E) for the computed partial sum T.
 // Rounding is monotone, hence RN(T) == RN(T + 2^E - 1) proves that RN(T) is
 // the correctly rounded product; when the two could differ the caller
 // computes the exact product instead.  K0 keeps E at least two bits below
 // the window, whose bottom is bit Wb = 32(L-1).  At 32 limbs (36 digits)
-// this skips columns 0..31, 528 of the 1296 digit products.
+// this skips columns 0..31, 528 of the 1296 digit products; at 64 limbs
+// (74 digits of 28 bits) columns 0..69, 2485 of 5476.
 template <int L>
 struct Trunc {
-    static constexpr int K0 = (32 * L - 70) >= kDigitBits ? (32 * L - 70) / kDigitBits : 0;
+    static constexpr int B = Digits<L>::B;
+    static constexpr int K0 = (32 * L - 70) >= B ? (32 * L - 70) / B : 0;
     static constexpr int Wb = 32 * (L - 1);
-    static constexpr int E = kDigitBits * K0 + 36;
-    static constexpr int J0 = (kDigitBits * K0) >> 5;      // first word below the window
+    static constexpr int E = B * K0 + B + 7;
+    static constexpr int J0 = (B * K0) >> 5;                // first word below the window
     static constexpr int NS = (L - 1) - J0;                 // words [J0, L-1) below the window
+    static_assert(K0 <= 126, "skipped-column bound assumes K0 <= 126");
 };
 
 // (any bit set, all bits set) of the absolute bit range [lo, hi) of the
@@ -388,17 +399,17 @@ MPFFT_FN bool mp_mul_trunc(Mp<L> &z, const uint32_t (&A)[Digits<L>::D],
             const int i1 = c < D - 1 ? c : D - 1;
 #pragma unroll
             for (int i = i0; i <= i1; i++) acc = (uint64_t)A[i] * B[c - i] + acc;
-            v = (uint32_t)acc & kDigitMask;
-            acc >>= kDigitBits;
+            v = (uint32_t)acc & Digits<L>::MASK;
+            acc >>= Digits<L>::B;
         } else {
             v = (uint32_t)acc;
         }
         // deposit digit c: word j of the product (window above L-2, sub below)
-        const int bit = kDigitBits * c, j = bit >> 5, sh = bit & 31;
+        const int bit = Digits<L>::B * c, j = bit >> 5, sh = bit & 31;
         const uint32_t lo = v << sh;
         if (j >= L - 1) { if (j < 2 * L) w[2 * L - 1 - j] |= lo; }
         else sub[j - J0] |= lo;
-        if (sh + kDigitBits > 32) {
+        if (sh + Digits<L>::B > 32) {
             const uint32_t hi = v >> (32 - sh);
             if (j + 1 >= L - 1) { if (j + 1 < 2 * L) w[2 * L - 2 - j] |= hi; }
             else sub[j + 1 - J0] |= hi;
@@ -407,11 +418,11 @@ MPFFT_FN bool mp_mul_trunc(Mp<L> &z, const uint32_t (&A)[Digits<L>::D],
     // predicates on T's bits below the window, for a normalization shift of
     // 0 (window bottom Wb) or 1 (window bottom Wb - 1)
     uint32_t nz_low, dummy, any0, all0, any1, all1;
-    sub_range<L>(sub, kDigitBits * K0, E, nz_low, dummy);
+    sub_range<L>(sub, Digits<L>::B * K0, E, nz_low, dummy);
     sub_range<L>(sub, E, Wb, dummy, all0);
     sub_range<L>(sub, E, Wb - 1, dummy, all1);
-    sub_range<L>(sub, kDigitBits * K0, Wb, any0, dummy);
-    sub_range<L>(sub, kDigitBits * K0, Wb - 1, any1, dummy);
+    sub_range<L>(sub, Digits<L>::B * K0, Wb, any0, dummy);
+    sub_range<L>(sub, Digits<L>::B * K0, Wb - 1, any1, dummy);
     const uint32_t s = (w[0] >> 31) ^ 1u;
     if (s) {
 #pragma unroll

@@ -34,6 +34,13 @@ __device__ __forceinline__ uint32_t wrap_idx(int32_t sn, uint32_t s, uint32_t j,
     return (uint32_t)(v < 0 ? v + N : v);
 }
 
+// Blocks per CU for the leaf and level kernels: two up to 32 limbs; one at
+// 64 limbs, where a lane's numbers need more than 256 registers and the
+// level kernel's LDS parking slots (2 x 65 words x 256 threads, 133 KB) fill
+// a CU's LDS alone.
+template <int L>
+constexpr int kMpBlocks = L > 32 ? 1 : 2;
+
 // The kernels below run each of their additions (and the level kernel each
 // of its multiplications) through one instance of mp_add / mp_mul inside a
 // loop whose trip count is uniform across the wave; operands are selected
@@ -77,7 +84,7 @@ mp_to_aos_kernel(LeafArgs a)
 // operands, so recomputing y and a per lane changes no result bit.
 // Inputs are gathered as whole elements from the element-major copy.
 template <int L, bool FULL>
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(256, kMpBlocks<L>)
 mp_leaf_kernel(LeafArgs a)
 {
     const uint32_t N = a.N, M = 2 * N;
@@ -119,7 +126,7 @@ mp_leaf_kernel(LeafArgs a)
 // Two waves per SIMD (256 registers per lane): one wave per SIMD leaves the
 // dependent multiply chains nothing to overlap with.
 template <int L, bool FULL, bool FOUR_M>
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(256, kMpBlocks<L>)
 mp_level_kernel(LevelArgs a)
 {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -395,7 +402,8 @@ static hipError_t launch_binop(int op, uint32_t count, const uint32_t *x,
 
 // limb counts with compiled kernels; a precision uses the smallest one that
 // holds it
-// (64 limbs: the fixed form on panels only; every other form stops at 32)
+// (64 limbs: the plain op orders and the fixed form on panels; the per-lane
+// fixed form and the word forms stop below)
 static const int kLimbs[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 64};
 
 int limbs_for_precision(long P)
@@ -421,6 +429,7 @@ hipError_t mp_fft_launch(int L, int P, const LeafArgs &la, const LevelArgs *lv,
     case 16: return full ? CALLF(16, true) : CALLF(16, false);
     case 24: return full ? CALLF(24, true) : CALLF(24, false);
     case 32: return full ? CALLF(32, true) : CALLF(32, false);
+    case 64: return full ? CALLF(64, true) : CALLF(64, false);
     default: return hipErrorInvalidValue;
     }
 #undef CALLF
@@ -450,6 +459,7 @@ hipError_t mp_unswap_scale_launch(int L, uint32_t *data, uint64_t stride, uint32
     case 16: return CALLU(16);
     case 24: return CALLU(24);
     case 32: return CALLU(32);
+    case 64: return CALLU(64);
     default: return hipErrorInvalidValue;
     }
 #undef CALLU
@@ -481,6 +491,7 @@ hipError_t mp_err_launch(int L, int P, const uint32_t *x, const uint32_t *y, uin
     case 16: return full ? CALLE(16, true) : CALLE(16, false);
     case 24: return full ? CALLE(24, true) : CALLE(24, false);
     case 32: return full ? CALLE(32, true) : CALLE(32, false);
+    case 64: return full ? CALLE(64, true) : CALLE(64, false);
     default: return hipErrorInvalidValue;
     }
 #undef CALLE
@@ -502,6 +513,7 @@ hipError_t mp_binop_launch(int L, int P, int op, uint32_t count, const uint32_t 
     case 16: return full ? CALLB(16, true) : CALLB(16, false);
     case 24: return full ? CALLB(24, true) : CALLB(24, false);
     case 32: return full ? CALLB(32, true) : CALLB(32, false);
+    case 64: return full ? CALLB(64, true) : CALLB(64, false);
     default: return hipErrorInvalidValue;
     }
 #undef CALLB

@@ -31,6 +31,7 @@ thread_local int g_last_error = MPFFT_OK;
 
 constexpr long kMaxPrec = 2048;          // largest compiled limb count * 32 (panels)
 constexpr long kMaxLimbsAll = 32;        // limb counts every kernel family compiles
+constexpr long kMaxLimbsMp = 64;         // the plain op orders' kernels
 constexpr long kExpLimit = 1L << 29;     // accepted input exponent range
 constexpr unsigned long kMaxN = 1UL << 24;
 constexpr long kDdMinPrec = 53;          // precisions of the double- and quad-word forms
@@ -829,8 +830,9 @@ int mpfft_plan_create(mpfft_plan *plan, unsigned long n, long prec, int form, un
         p->P = prec;
         p->L = limbs_for_precision(prec);
         p->words = 2 * n * (size_t)(p->L + 1);
-        // beyond 32 limbs only the panel kernels of the fixed form exist
-        if (p->L > kMaxLimbsAll &&
+        // beyond 32 limbs: the plain op orders, and the panel kernels of the
+        // fixed form
+        if (p->L > kMaxLimbsAll && !is_mp(form) &&
             !((form == MPFFT_FORM_FIXED || form == MPFFT_FORM_FIXED_PANEL) &&
               fxp_eligible((uint32_t)n, batch)))
             return MPFFT_ERR_PREC;
@@ -1154,7 +1156,7 @@ int mpfft_mp_binop_host(int op, unsigned long count, long prec, const uint32_t *
     if (!x || !y || !z || op < 0 || op > 2 || count == 0 || count > (1UL << 26))
         return MPFFT_ERR_ARG;
     const long L = mpfft_limbs_for_precision(prec);
-    if (L < 1 || L > kMaxLimbsAll) return MPFFT_ERR_PREC;
+    if (L < 1 || L > kMaxLimbsMp) return MPFFT_ERR_PREC;
     const size_t bytes = sizeof(uint32_t) * count * (L + 1);
     DevBuf dx, dy, dz;
     int rc;

@@ -207,6 +207,7 @@ int main(int argc, char **argv)
     fail |= run_L<16>(trials / 2 + 1);
     fail |= run_L<24>(trials / 4 + 1);
     fail |= run_L<32>(trials / 4 + 1);
+    fail |= run_L<64>(trials / 8 + 1);
     printf(fail ? "FAIL\n" : "PASS\n");
     return fail;
 }

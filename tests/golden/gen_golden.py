@@ -12,6 +12,8 @@ writes:
   digests.json       SHA-256 of large outputs, input digests, op counters
 
 Usage: python tests/golden/gen_golden.py
+       python tests/golden/gen_golden.py --wide-only   (only the digests
+       above 1024 bits, added to the existing digests.json)
 """
 import hashlib
 import json
@@ -33,6 +35,18 @@ MP_PRECS = [53, 64, 128, 256, 1024]
 MP_LARGE = [(1 << 16, 256), (1 << 16, 1024), (1 << 20, 128)]
 D_SIZES = [1, 2, 4, 8, 16, 32, 64, 1024]
 D_LARGE = [1 << 16]
+# above 1024 bits (64 limbs): CODE 3 and CODE 2 outputs as digests
+MP_WIDE = [(16, 2048), (1024, 2048), (4096, 1500), (1 << 16, 2048)]
+
+
+def wide(dig):
+    dig["mp_wide"] = {}
+    for n, P in MP_WIDE:
+        x = I.mp_random(n, P, seed=SEED)
+        for code in ((3, 2) if n <= 4096 else (3,)):
+            _, y, _ = O.ref_mp(f"mp{code}", n, P, x)
+            dig["mp_wide"][f"c{code}_N{n}_P{P}"] = {"in_sha": sha(x), "out_sha": sha(y)}
+        print("wide", n, P, flush=True)
 
 
 def sha(a):
@@ -72,6 +86,15 @@ def special_inputs(n, P):
 def main():
     if not O.have_ref():
         sys.exit("oracle/_ref libraries missing: make -C oracle with the reference tree present")
+    if "--wide-only" in sys.argv:
+        path = os.path.join(OUT, "digests.json")
+        with open(path) as f:
+            dig = json.load(f)
+        wide(dig)
+        with open(path, "w") as f:
+            json.dump(dig, f, indent=1, sort_keys=True)
+        print("added wide digests to", path)
+        return
     mp = {}
     dig = {"seed": SEED, "mp": {}, "mp_large": {}, "mp_counts": {}, "d": {}, "d_large": {}}
     for n in MP_SIZES:
@@ -113,6 +136,7 @@ def main():
         dig["mp_large"][f"N{n}_P{P}"] = {"in_sha": sha(x), "out_sha": sha(y)}
         dig["mp_counts"][str(n)] = cnt
         print("large", n, P, flush=True)
+    wide(dig)
     np.savez_compressed(os.path.join(OUT, "mp_golden.npz"), **mp)
 
     d = {}

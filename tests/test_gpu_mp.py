@@ -25,10 +25,10 @@ def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def run_fft(n, P, x, batch=1):
+def run_fft(n, P, x, batch=1, form=M.FORM_MP):
     """x: SoA-limb transform(s) at ceil(P/32) limbs; result likewise."""
     Lo = O.limbs(P)
-    plan = M.Plan(n, P, M.FORM_MP, batch)
+    plan = M.Plan(n, P, form, batch)
     xs = np.concatenate([I.relimb(t, n, Lo, plan.L)
                          for t in np.split(np.asarray(x, dtype=np.uint32), batch)])
     y = plan.execute_host(xs)
@@ -65,7 +65,7 @@ def operands(count, P, seed):
     return x, y
 
 
-@pytest.mark.parametrize("P", [53, 64, 100, 128, 160, 256, 333, 512, 700, 1024])
+@pytest.mark.parametrize("P", [53, 64, 100, 128, 160, 256, 333, 512, 700, 1024, 1500, 2048])
 @pytest.mark.parametrize("op", [0, 1, 2])
 def test_device_ops_match_mpfr(P, op):
     count = 8192
@@ -120,6 +120,31 @@ def test_fft_large_matches_reference_digest(key, digests):
     assert sha(x) == digests["mp_large"][key]["in_sha"]
     print(f"case {key}", flush=True)
     assert sha(run_fft(n, P, x)) == digests["mp_large"][key]["out_sha"]
+
+
+WIDE_KEYS = ["c3_N16_P2048", "c2_N16_P2048", "c3_N1024_P2048", "c2_N1024_P2048",
+             "c3_N4096_P1500", "c2_N4096_P1500", "c3_N65536_P2048"]
+
+
+@pytest.mark.parametrize("key", WIDE_KEYS)
+def test_fft_wide_matches_reference_digest(key, digests):
+    # above 1024 bits (64 limbs, 28-bit digit products): CODE 3 and CODE 2
+    # against the reference's own outputs
+    code, nk, pk = key.split("_")
+    n, P = int(nk[1:]), int(pk[1:])
+    form = M.FORM_MP if code == "c3" else M.FORM_MP_C2
+    x = I.mp_random(n, P, seed=digests["seed"])
+    assert sha(x) == digests["mp_wide"][key]["in_sha"]
+    print(f"case {key}", flush=True)
+    assert sha(run_fft(n, P, x, form=form)) == digests["mp_wide"][key]["out_sha"]
+
+
+@pytest.mark.parametrize("n,P,seed", [(1, 2048, 1), (2, 1025, 2), (4, 2000, 3), (8, 2048, 4),
+                                      (64, 1056, 5), (512, 2047, 6)])
+def test_fft_wide_matches_oracle(n, P, seed):
+    x = I.mp_random(n, P, seed=seed)
+    print(f"case n={n} P={P} seed={seed}", flush=True)
+    assert np.array_equal(run_fft(n, P, x), O.mp_fft(n, P, x))
 
 
 def test_batch_equals_single_transforms():
@@ -185,6 +210,21 @@ def test_mpfr_entry_point_large_chunked(digests):
     assert lib.mpfft_last_error() == 0
     y = soa_from_mpfr_pair(n, P, outr, outi)
     assert sha(y) == digests["mp_large"][f"N{n}_P{P}"]["out_sha"]
+
+
+@pytest.mark.parametrize("key", ["c3_N1024_P2048", "c3_N4096_P1500"])
+def test_mpfr_entry_point_wide(key, digests):
+    # the drop-in call above 1024 bits: 64-limb significands marshalled from
+    # mpfr_t, the reference's own result
+    _, nk, pk = key.split("_")
+    n, P = int(nk[1:]), int(pk[1:])
+    x = I.mp_random(n, P, seed=digests["seed"])
+    inr, ini = mpfr_pair_from_soa(n, P, x)
+    outr, outi = MA.MpfrArray(n, P), MA.MpfrArray(n, P)
+    lib = M.lib()
+    lib.mp_sr24_cp_fft(n, 1, 0, inr.ptr(), ini.ptr(), outr.ptr(), outi.ptr())
+    assert lib.mpfft_last_error() == 0
+    assert sha(soa_from_mpfr_pair(n, P, outr, outi)) == digests["mp_wide"][key]["out_sha"]
 
 
 def test_mpfr_plan_timing():

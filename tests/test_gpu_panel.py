@@ -180,8 +180,7 @@ def test_panel_large_n():
 def test_panel_2048_bits(n, P, b):
     # reals of 66 words (P up to 2048, 64 limbs): the panel kernels only;
     # against the 2P-bit oracle, no worse than the plain order (the oracle's
-    # restatement of the reference at P bits; the GPU plain form stops at
-    # 1024 bits)
+    # restatement of the reference at P bits)
     xs = inputs(n, P, b, seed=2048 + n)
     plan = M.Plan(n, P, M.FORM_FIXED, b)
     assert plan.L == 64
@@ -192,10 +191,16 @@ def test_panel_2048_bits(n, P, b):
 
 
 def test_panel_2048_bits_forms_checked():
-    # beyond 1024 bits every other form (and the panels below n = 16) refuses
-    for form in (M.FORM_MP, M.FORM_MP_C1, M.FORM_FIXED_LANE, M.FORM_DD):
+    # beyond 1024 bits the per-lane fixed form and the word forms (and the
+    # panels below n = 16) refuse; the plain op orders run up to 2048 bits
+    for form in (M.FORM_FIXED_LANE, M.FORM_DD, M.FORM_QD):
         with pytest.raises(M.MpfftError) as e:
             M.Plan(64, 2048, form, 1)
+        assert e.value.code == -2
+    for form in (M.FORM_MP, M.FORM_MP_C1, M.FORM_MP_C2):
+        M.Plan(64, 2048, form, 1).close()
+        with pytest.raises(M.MpfftError) as e:
+            M.Plan(64, 2049, form, 1)
         assert e.value.code == -2
     with pytest.raises(M.MpfftError) as e:
         M.Plan(8, 2048, M.FORM_FIXED, 64)

@@ -58,7 +58,7 @@ def device_inverse(n, P, X, batch=1):
 
 
 @pytest.mark.parametrize("n,P", [(1, 128), (2, 128), (4, 256), (8, 64), (64, 1024),
-                                 (1024, 256), (2048, 1024), (512, 100)])
+                                 (1024, 256), (2048, 1024), (512, 100), (256, 2048)])
 def test_inverse_matches_oracle(n, P):
     x = I.mp_random(n, P, seed=700 + n + P)
     X = O.mp_fft(n, P, x)

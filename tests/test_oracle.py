@@ -118,6 +118,19 @@ def test_oracle_large_digest(digests):
     assert sha(O.mp_fft(n, P, x)) == digests["mp_large"][f"N{n}_P{P}"]["out_sha"]
 
 
+WIDE_CASES = [(3, 16, 2048), (2, 16, 2048), (3, 1024, 2048), (2, 1024, 2048),
+              (3, 4096, 1500), (2, 4096, 1500), (3, 1 << 16, 2048)]
+
+
+@pytest.mark.parametrize("code,n,P", WIDE_CASES)
+def test_oracle_wide_digest(code, n, P, digests):
+    # above 1024 bits (64 limbs): the reference's own outputs as digests
+    x = I.mp_random(n, P, seed=digests["seed"])
+    d = digests["mp_wide"][f"c{code}_N{n}_P{P}"]
+    assert sha(x) == d["in_sha"]
+    assert sha(O.mp_fft(n, P, x, code=code)) == d["out_sha"]
+
+
 @pytest.mark.parametrize("n", [1024, 1 << 16])
 def test_op_counts(n, digests):
     # reference counters (serial_mp_fft3.c:123-250) and Yavne's total
@@ -262,7 +275,7 @@ def test_double_lut_matches_reference(d_golden):
 
 @pytest.mark.skipif(not O.have_ref(), reason="reference build (oracle/_ref) not present")
 def test_oracle_vs_live_reference():
-    for n, P in [(512, 200), (128, 96), (2048, 333)]:
+    for n, P in [(512, 200), (128, 96), (2048, 333), (256, 2048), (64, 1800)]:
         x = I.mp_random(n, P, seed=n + P)
         assert np.array_equal(O.mp_fft(n, P, x), O.ref_mp("mp3", n, P, x)[1])
         for code in (1, 2):
