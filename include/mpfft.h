@@ -102,19 +102,18 @@ typedef struct mpfft_plan_s *mpfft_plan;
                               MPFFT_FORM_FIXED_PANEL; otherwise as
                               MPFFT_FORM_FIXED_LANE                        */
 #define MPFFT_FORM_FIXED_LANE 9 /* the fixed form with per-lane twiddle
-                              products (29-bit digit multiply-adds) at any
-                              batch; same format and error bound          */
+                              products (29-bit digit multiply-adds; 28-bit
+                              at 2048 bits) at any batch; same format and
+                              error bound                                 */
 #define MPFFT_FORM_FIXED_PANEL 10 /* the fixed form with transforms
                               interleaved in panels of 64 and the twiddle
                               products on the int8 matrix cores; any batch
                               (a last, partial panel is padded), n >= 16
                               (MPFFT_ERR_SIZE below); same format and error
-                              bound.  The fixed form's only kernels beyond
-                              1024 bits: prec up to 2048 (64 limbs), which
-                              MPFFT_FORM_FIXED then runs on panels at any
-                              batch (MPFFT_FORM_FIXED_LANE, _DD and _QD
-                              return MPFFT_ERR_PREC above 1024; the plain
-                              op orders run up to 2048)                   */
+                              bound.  All three fixed forms and the plain op
+                              orders take prec up to 2048 (64 limbs);
+                              MPFFT_FORM_DD and _QD return MPFFT_ERR_PREC
+                              beyond their own limits                     */
 #define MPFFT_FORM_DD 11  /* double-word form, 53 <= prec <= 106: every
                               real is hi + lo (two binary64) and every
                               operation an error-free transformation on the

@@ -241,8 +241,8 @@ int mpfft_batch_create(mpfft_batch *batch, unsigned int count, const unsigned lo
         }
         off += g.words * g.idx.size();
         const uint32_t cnt = (uint32_t)g.idx.size();
-        if (form == MPFFT_FORM_FIXED)      // (reals beyond 32 limbs: panels only)
-            g.form = mpfft::fxp_preferred((uint32_t)g.N, cnt, (int)g.L + 2) || g.L > 32
+        if (form == MPFFT_FORM_FIXED)
+            g.form = mpfft::fxp_preferred((uint32_t)g.N, cnt, (int)g.L + 2)
                          ? MPFFT_FORM_FIXED_PANEL
                          : MPFFT_FORM_FIXED_LANE;
         else if (form == MPFFT_FORM_FIXED_PANEL && !mpfft::fxp_eligible((uint32_t)g.N, cnt))

@@ -23,17 +23,25 @@ struct Fx {
     uint32_t w[W];
 };
 
-// digits of a twiddle magnitude: |t| < 2, value T / 2^F, T < 2^(F+1)
+// digits of a twiddle magnitude: |t| < 2, value T / 2^F, T < 2^(F+1).
+// Digit width B: 29 bits while |X| has at most 63 digits (W <= 57), 28
+// beyond (76 digits at W = 66); a product column then holds at most 63
+// (resp. 76) terms below 2^(2B) plus a carry below 2^(64-B): below 2^64.
 template <int W>
 struct FxDig {
     static constexpr int F = 32 * (W - 1);
-    static constexpr int DX = (32 * W + kDigitBits - 1) / kDigitBits;   // |X| < 2^(32W-1)
-    static constexpr int DT = (F + 1 + kDigitBits - 1) / kDigitBits;    // T < 2^(F+1)
-    // columns below K0 are skipped.  Column c < K0 sums at most 64 digit
-    // products below 2^58 (DX, DT <= 63) at weight 2^(29 c), so the dropped
-    // part is below 2^64 * 2^(29 (K0 - 1)) * (1 + 2^-28) < 2^(29 K0 + 35.01)
-    // <= 2^(F - 4.99): under 1/32 of the output's last unit
-    static constexpr int K0 = (F - 40) >= 0 ? (F - 40) / kDigitBits : 0;
+    static constexpr int B = (32 * W + kDigitBits - 1) / kDigitBits <= 63 ? kDigitBits : 28;
+    static constexpr uint32_t MASK = (1u << B) - 1u;
+    static constexpr int DX = (32 * W + B - 1) / B;        // |X| < 2^(32W-1)
+    static constexpr int DT = (F + 1 + B - 1) / B;         // T < 2^(F+1)
+    // columns below K0 are skipped.  Column c < K0 sums at most DX digit
+    // products below 2^(2B) at weight 2^(B c), so the dropped part is below
+    // DX 2^(2B) 2^(B (K0 - 1)) (1 + 2^(1-B)): at B = 29 (DX <= 63) below
+    // 2^(29 K0 + 35.01), at B = 28 (DX <= 76) below 2^(28 K0 + 34.25); with
+    // B K0 <= F - 40 that is under 2^(F - 4.99), 1/32 of the output's last
+    // unit
+    static constexpr int K0 = (F - 40) >= 0 ? (F - 40) / B : 0;
+    static_assert(DX < (1 << (64 - 2 * B)) && DX <= 76, "column sums would overflow 64 bits");
 };
 
 template <int W>
@@ -79,16 +87,17 @@ MPFFT_FN void fx_abs(Fx<W> &z, const Fx<W> &x, uint32_t neg)
     for (int j = 0; j < W; j++) z.w[j] = add_cc(x.w[j] ^ m, 0, c);
 }
 
-// |X| (W words, LS first) -> DX digits of 29 bits, LS first
+// |X| (W words, LS first) -> DX digits of B bits, LS first
 template <int W>
 MPFFT_FN void fx_to_digits(const Fx<W> &x, uint32_t (&d)[FxDig<W>::DX])
 {
+    constexpr int B = FxDig<W>::B;
 #pragma unroll
     for (int k = 0; k < FxDig<W>::DX; k++) {
-        const int bit = kDigitBits * k, wi = bit >> 5, s = bit & 31;
+        const int bit = B * k, wi = bit >> 5, s = bit & 31;
         const uint32_t lo = wi < W ? x.w[wi] : 0u;
         const uint32_t hi = wi + 1 < W ? x.w[wi + 1] : 0u;
-        d[k] = (s == 0 ? lo : fshr(hi, lo, (uint32_t)s)) & kDigitMask;
+        d[k] = (s == 0 ? lo : fshr(hi, lo, (uint32_t)s)) & FxDig<W>::MASK;
     }
 }
 
@@ -99,9 +108,7 @@ template <int W>
 MPFFT_FN void fx_mul(Fx<W> &z, Fx<W> &x, const uint32_t (&T)[FxDig<W>::DT], uint32_t tneg)
 {
     constexpr int F = FxDig<W>::F, DX = FxDig<W>::DX, DT = FxDig<W>::DT, K0 = FxDig<W>::K0;
-    // the per-lane product exists up to W = 34 (the panel kernels, whose
-    // product runs on the matrix cores, also take wider reals)
-    static_assert(DX <= 63 && DT <= 63, "column sums would overflow 64 bits");
+    constexpr int B = FxDig<W>::B;
     const uint32_t xneg = fx_sign(x);
     fx_abs(x, x, xneg);
     uint32_t A[DX];
@@ -115,18 +122,18 @@ MPFFT_FN void fx_mul(Fx<W> &z, Fx<W> &x, const uint32_t (&T)[FxDig<W>::DT], uint
         const int i1 = c < DX - 1 ? c : DX - 1;
 #pragma unroll
         for (int i = i0; i <= i1; i++) acc = (uint64_t)A[i] * T[c - i] + acc;
-        // digit c covers product bits [29c, 29c + 29) = result bits
-        // [29c - F, 29c - F + 29); bits below result bit 0 are dropped
-        const uint32_t v = (c == DX + DT - 1) ? (uint32_t)acc : ((uint32_t)acc & kDigitMask);
-        acc >>= kDigitBits;
-        const int b = kDigitBits * c - F;
-        if (b + kDigitBits <= 0) continue;
+        // digit c covers product bits [Bc, Bc + B) = result bits
+        // [Bc - F, Bc - F + B); bits below result bit 0 are dropped
+        const uint32_t v = (c == DX + DT - 1) ? (uint32_t)acc : ((uint32_t)acc & FxDig<W>::MASK);
+        acc >>= B;
+        const int b = B * c - F;
+        if (b + B <= 0) continue;
         if (b < 0) {
             z.w[0] |= v >> (-b);
         } else {
             const int j = b >> 5, s = b & 31;
             if (j < W) z.w[j] |= v << s;
-            if (s + kDigitBits > 32 && j + 1 < W) z.w[j + 1] |= v >> (32 - s);
+            if (s + B > 32 && j + 1 < W) z.w[j + 1] |= v >> (32 - s);
         }
     }
     fx_abs(z, z, xneg ^ tneg);           // negate when the signs differ

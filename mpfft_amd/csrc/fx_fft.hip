@@ -63,6 +63,13 @@ __device__ __forceinline__ uint32_t tw_load(uint32_t (&T)[FxDig<W>::DT], const u
     return (tw + (size_t)FxDig<W>::DT * MT)[q];
 }
 
+// Blocks per CU of the pass kernels: two up to 34 words per real; one at
+// 66 words (2048 bits), where a lane's reals and the 76-digit product
+// operands need more than 256 registers and the two LDS parking slots take
+// 2 x 66 x 256 words (135 KB).
+template <int W>
+constexpr int kFxBlocks = W > 34 ? 1 : 2;
+
 }  // namespace
 
 // Pass I/O.  The first pass reads the float SoA input and converts on the
@@ -139,7 +146,7 @@ fx_out_kernel(FxArgs a, const uint32_t *src)
 // that read one twiddle for the different transforms of a batch run side
 // by side and share it in L2.
 template <int L, int W, bool FULL, int IO>
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(256, kFxBlocks<W>)
 fx_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t toff)
 {
     const uint32_t N = a.N, Q = N >> 2;
@@ -192,7 +199,7 @@ fx_r4_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t
 
 // The radix-2 pass (first pass when log2 N is odd: Ns = 1, no twiddles).
 template <int L, int W, bool FULL, int IO>
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(256, kFxBlocks<W>)
 fx_r2_kernel(FxArgs a, const uint32_t *src, uint32_t *dst)
 {
     const uint32_t N = a.N, H = N >> 1;
@@ -216,6 +223,7 @@ fx_r2_kernel(FxArgs a, const uint32_t *src, uint32_t *dst)
 template <int L, int W, bool FULL>
 static hipError_t launch_fx(const FxArgs &a, int s0, int s1, hipStream_t st)
 {
+    static_assert(FxDig<W>::B == fx_lane_digit_bits(W), "host twiddle digits differ");
     const uint32_t N = a.N, M = 2 * N;
     uint32_t lg = 0;
     while ((1u << lg) < N) lg++;
@@ -286,6 +294,7 @@ hipError_t fx_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStrea
     case 16: return CALLX(16);
     case 24: return CALLX(24);
     case 32: return CALLX(32);
+    case 64: return CALLX(64);
     default: return hipErrorInvalidValue;
     }
 #undef CALLX

@@ -82,6 +82,10 @@ struct FxArgs {
 
 // fixed-point words per real for limb count L (F = 32 (L + 1) fraction bits)
 constexpr int fx_words(int L) { return L + 2; }
+// digit width of the per-lane twiddle product for reals of W words (the
+// host cuts the per-lane twiddle tables into digits of this width): 29 bits
+// while a real has at most 63 digits (W <= 57), 28 beyond
+constexpr int fx_lane_digit_bits(int W) { return (32 * W + 28) / 29 <= 63 ? 29 : 28; }
 
 // passes of the fixed form: a radix-2 pass when log2 N is odd, then radix 4
 inline int fx_passes(uint32_t N)

@@ -351,7 +351,7 @@ int build_mp_plan(mpfft_plan_s *p)
 // Ns < N), entries 3k + r - 1 (k < Ns, r = 1..3) hold w^t = c + i d =
 // exp(-2 pi i t / N), t = r k N / (4 Ns), rounded to F = 32 (L + 1)
 // fraction bits (MPFR at F + 64 bits, then one rounding to nearest).
-//   per-lane kernels: c, d - c and c + d, each as DT 29-bit digits of its
+//   per-lane kernels: c, d - c and c + d, each as DT B-bit digits of its
 //     magnitude (least significant first) and a sign word; SoA over the
 //     3 T3 reals;
 //   panel kernels: c, d and -d per entry (entry-major), each as the byte
@@ -361,14 +361,14 @@ int build_fx_plan(mpfft_plan_s *p)
 {
     const unsigned long N = p->N;
     const long W = p->L + 2, F = 32 * (W - 1);
-    const long DT = (F + 1 + 28) / 29;
+    const long B = fx_lane_digit_bits((int)W);            // 29, or 28 at W = 66
+    const long DT = (F + 1 + B - 1) / B;
     int lg0 = 0;
     while ((1UL << lg0) < N) lg0++;
     if (p->form == MPFFT_FORM_FIXED_PANEL && !fxp_eligible((uint32_t)N, p->batch))
         return MPFFT_ERR_SIZE;
     p->fx_panel = p->form == MPFFT_FORM_FIXED_PANEL ||
-                  (p->form == MPFFT_FORM_FIXED &&
-                   (fxp_preferred((uint32_t)N, p->batch, (int)W) || p->L > kMaxLimbsAll));
+                  (p->form == MPFFT_FORM_FIXED && fxp_preferred((uint32_t)N, p->batch, (int)W));
     int ncu = 0;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device));
     p->ncu = ncu > 0 ? (uint32_t)ncu : 1u;
@@ -465,9 +465,10 @@ int build_fx_plan(mpfft_plan_s *p)
                 size_t cnt = 0;
                 mpz_export(words.data(), &cnt, -1, sizeof(uint32_t), 0, 0, z);
                 for (long k = 0; k < DT; k++) {
-                    const long bit = 29 * k, wi = bit >> 5, sh = bit & 31;
+                    const long bit = B * k, wi = bit >> 5, sh = bit & 31;
                     const uint64_t lo = words[wi], hi = words[wi + 1];
-                    tab[(size_t)k * MT + q] = (uint32_t)(((hi << 32 | lo) >> sh) & 0x1fffffffu);
+                    tab[(size_t)k * MT + q] =
+                        (uint32_t)(((hi << 32 | lo) >> sh) & ((1ull << B) - 1));
                 }
             }
         }
@@ -830,11 +831,9 @@ int mpfft_plan_create(mpfft_plan *plan, unsigned long n, long prec, int form, un
         p->P = prec;
         p->L = limbs_for_precision(prec);
         p->words = 2 * n * (size_t)(p->L + 1);
-        // beyond 32 limbs: the plain op orders, and the panel kernels of the
-        // fixed form
-        if (p->L > kMaxLimbsAll && !is_mp(form) &&
-            !((form == MPFFT_FORM_FIXED || form == MPFFT_FORM_FIXED_PANEL) &&
-              fxp_eligible((uint32_t)n, batch)))
+        // beyond 32 limbs: the plain op orders and the fixed form (panels
+        // and per-lane kernels at 64 limbs)
+        if (p->L > kMaxLimbsAll && !is_mp(form) && !is_fixed(form))
             return MPFFT_ERR_PREC;
         rc = word_count(form)       ? build_dd_plan(p.get())
              : is_fixed(form)      ? build_fx_plan(p.get())

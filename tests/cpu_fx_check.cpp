@@ -65,7 +65,7 @@ static int fails = 0;
 template <int W>
 static void check_case(int trials)
 {
-    constexpr int F = FxDig<W>::F, DT = FxDig<W>::DT;
+    constexpr int F = FxDig<W>::F, DT = FxDig<W>::DT, B = FxDig<W>::B;
     mpz_t a, b, c, e, t;
     mpz_inits(a, b, c, e, t, (mpz_ptr)0);
     long worst = 0, nonexact = 0;
@@ -88,11 +88,11 @@ static void check_case(int trials)
         mpz_set_ui(t, 0);
         const int tb = (n % 7 == 0) ? F + 1 : 1 + (int)(rnd64() % (uint64_t)(F + 1));
         for (int k = DT - 1; k >= 0; k--) {
-            T[k] = rnd32() & kDigitMask;
-            if (29 * k >= tb) T[k] = 0;
-            else if (29 * k + 29 > tb) T[k] &= (1u << (tb - 29 * k)) - 1u;
+            T[k] = rnd32() & FxDig<W>::MASK;
+            if (B * k >= tb) T[k] = 0;
+            else if (B * k + B > tb) T[k] &= (1u << (tb - B * k)) - 1u;
         }
-        for (int k = DT - 1; k >= 0; k--) { mpz_mul_2exp(t, t, 29); mpz_add_ui(t, t, T[k]); }
+        for (int k = DT - 1; k >= 0; k--) { mpz_mul_2exp(t, t, B); mpz_add_ui(t, t, T[k]); }
         const uint32_t tneg = (uint32_t)(rnd64() & 1);
         // |x| < 2^(32W-3) keeps |x t| / 2^F < 2^(32W-2): no overflow
         Fx<W> xx = x;
@@ -109,8 +109,8 @@ static void check_case(int trials)
         if (d) nonexact++;
         if (labs(d) > worst) worst = labs(d);
     }
-    printf("W=%d sums exact, products within %ld unit (%ld of %d not truncation-exact)\n",
-           W, worst, nonexact, trials);
+    printf("W=%d (%d-bit digits) sums exact, products within %ld unit (%ld of %d not "
+           "truncation-exact)\n", W, B, worst, nonexact, trials);
     mpz_clears(a, b, c, e, t, (mpz_ptr)0);
 }
 
@@ -189,6 +189,7 @@ int main(int argc, char **argv)
     check_case<18>(trials);
     check_case<26>(trials);
     check_case<34>(trials);
+    check_case<66>(trials);                      // 28-bit digits
     check_conv<1, true>(trials, 32);
     check_conv<2, false>(trials, 53);
     check_conv<8, true>(trials, 256);

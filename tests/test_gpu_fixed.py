@@ -17,9 +17,9 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def run_fixed(n, P, x, batch=1):
+def run_fixed(n, P, x, batch=1, form=M.FORM_FIXED):
     Lo = O.limbs(P)
-    plan = M.Plan(n, P, M.FORM_FIXED, batch)
+    plan = M.Plan(n, P, form, batch)
     xs = np.concatenate([I.relimb(t, n, Lo, plan.L) for t in np.split(x, batch)])
     y = plan.execute_host(xs)
     plan.close()
@@ -43,6 +43,43 @@ def test_fixed_error_no_worse_than_plain(n, P):
     assert zb_f == 0
     assert nw_f <= max(nw_p, 0.5), (nw_f, nw_p)
     assert mu_f <= max(mu_p, 1.0), (mu_f, mu_p)
+
+
+@pytest.mark.parametrize("n,P,b", [(1, 2048, 1), (4, 1500, 2), (16, 2048, 1), (64, 1056, 3),
+                                   (1024, 2048, 1), (4096, 2000, 2)])
+def test_fixed_lane_wide(n, P, b):
+    # the per-lane kernels at 66-word reals (28-bit digit products): against
+    # the 2P-bit oracle no worse than the plain form, and within 2 ulp of the
+    # panel kernels (the two differ only next to a rounding boundary)
+    xs = [I.mp_random(n, P, seed=1900 + n + P + i) for i in range(b)]
+    plan = M.Plan(n, P, M.FORM_FIXED_LANE, b)
+    assert plan.L == 64
+    plan.close()
+    print(f"case n={n} P={P} b={b}", flush=True)
+    got = np.split(run_fixed(n, P, np.concatenate(xs), batch=b, form=M.FORM_FIXED_LANE), b)
+    for x, y in zip(xs, got):
+        star = O.mp_fft(n, P, x, Pw=2 * P)
+        mu_f, nw_f, zb_f = errors(n, P, y, star)
+        mu_p, nw_p, _ = errors(n, P, O.mp_fft(n, P, x), star)
+        assert zb_f == 0
+        assert nw_f <= max(nw_p, 0.5) and mu_f <= max(mu_p, 1.0), (mu_f, nw_f, mu_p, nw_p)
+    if n >= 16:
+        yp = np.split(run_fixed(n, P, np.concatenate(xs), batch=b, form=M.FORM_FIXED_PANEL), b)
+        for y, z in zip(got, yp):
+            mu, _, zb = O.err_stats(n, P, y, O.limbs(P), z, O.limbs(P))
+            assert zb == 0 and mu <= 2.0, (mu,)
+
+
+def test_fixed_wide_small_batch_takes_lanes():
+    # above 1024 bits MPFFT_FORM_FIXED picks the per-lane kernels where a
+    # panel would be mostly padding, and the panels at a full batch
+    lane = M.Plan(4096, 2048, M.FORM_FIXED, 1)
+    pan = M.Plan(4096, 2048, M.FORM_FIXED_PANEL, 1)
+    full = M.Plan(4096, 2048, M.FORM_FIXED, 64)
+    assert lane.stages() == 6                    # six radix-4 passes, no conversions
+    assert pan.stages() != lane.stages() and full.stages() == pan.stages()
+    for p in (lane, pan, full):
+        p.close()
 
 
 def test_fixed_full_size_1024_bits():

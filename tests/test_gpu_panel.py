@@ -191,23 +191,22 @@ def test_panel_2048_bits(n, P, b):
 
 
 def test_panel_2048_bits_forms_checked():
-    # beyond 1024 bits the per-lane fixed form and the word forms (and the
-    # panels below n = 16) refuse; the plain op orders run up to 2048 bits
-    for form in (M.FORM_FIXED_LANE, M.FORM_DD, M.FORM_QD):
+    # beyond 1024 bits the word forms refuse (and the panels below n = 16);
+    # the plain op orders and both fixed-form kernel families run up to 2048
+    # bits
+    for form in (M.FORM_DD, M.FORM_QD):
         with pytest.raises(M.MpfftError) as e:
             M.Plan(64, 2048, form, 1)
         assert e.value.code == -2
-    for form in (M.FORM_MP, M.FORM_MP_C1, M.FORM_MP_C2):
+    for form in (M.FORM_MP, M.FORM_MP_C1, M.FORM_MP_C2, M.FORM_FIXED, M.FORM_FIXED_LANE):
         M.Plan(64, 2048, form, 1).close()
         with pytest.raises(M.MpfftError) as e:
             M.Plan(64, 2049, form, 1)
         assert e.value.code == -2
     with pytest.raises(M.MpfftError) as e:
-        M.Plan(8, 2048, M.FORM_FIXED, 64)
-    assert e.value.code == -2
-    with pytest.raises(M.MpfftError) as e:
-        M.Plan(64, 2049, M.FORM_FIXED, 64)
-    assert e.value.code == -2
+        M.Plan(8, 2048, M.FORM_FIXED_PANEL, 64)
+    assert e.value.code == -1
+    M.Plan(8, 2048, M.FORM_FIXED, 64).close()           # per-lane kernels below n = 16
 
 
 def test_panel_2048_bits_large_n_impulse():
