@@ -58,10 +58,12 @@ def lane_check(n, P, y, ylane):
 
 
 def test_batch_layout_and_plain_bits():
-    # uneven sizes (1 .. 512) and precisions, some not a multiple of 32
-    # bits; the plain form is the reference's op order bit for bit
-    ns = np.array([1, 2, 4, 8, 16, 64, 512, 512, 64, 32, 256, 128, 8, 512], dtype=np.int64)
-    Ps = np.array([64, 100, 128, 256, 53, 128, 256, 256, 1024, 200, 64, 512, 128, 100],
+    # uneven sizes (1 .. 512) and precisions (53 .. 2048), some not a
+    # multiple of 32 bits; the plain form is the reference's op order bit
+    # for bit
+    ns = np.array([1, 2, 4, 8, 16, 64, 512, 512, 64, 32, 256, 128, 8, 512, 64, 16],
+                  dtype=np.int64)
+    Ps = np.array([64, 100, 128, 256, 53, 128, 256, 256, 1024, 200, 64, 512, 128, 100, 2048, 1500],
                   dtype=np.int64)
     b = M.Batch(ns, Ps, M.FORM_MP)
     # packed layout: every transform owns a disjoint range of the buffers
@@ -124,15 +126,17 @@ def test_batch_panels_any_count(count):
 
 def test_batch_fixed_small_sizes():
     # the fixed form's edge sizes in one batch: n = 1 .. 16 (per-lane kernels
-    # below 16), a 2048-bit group (panels only), precisions not a multiple
-    # of 32 bits; each transform against the 2P-bit oracle
-    ns = np.array([1, 2, 4, 8, 16, 16, 32, 4], dtype=np.int64)
-    Ps = np.array([128, 256, 64, 512, 1024, 100, 2048, 1024], dtype=np.int64)
+    # below 16), groups above 1024 bits (panels at n = 32, the per-lane
+    # kernels at n = 4096 and n = 2), precisions not a multiple of 32 bits;
+    # each transform against the 2P-bit oracle
+    ns = np.array([1, 2, 4, 8, 16, 16, 32, 4, 4096, 2], dtype=np.int64)
+    Ps = np.array([128, 256, 64, 512, 1024, 100, 2048, 1024, 1800, 2048], dtype=np.int64)
     xs = [I.mp_random(int(n), int(P), seed=1700 + i) for i, (n, P) in enumerate(zip(ns, Ps))]
     b = M.Batch(ns, Ps, M.FORM_FIXED)
     ys = run_batch(b, xs)
     forms = b.forms()
     assert forms[6] == M.FORM_FIXED_PANEL and forms[0] == M.FORM_FIXED_LANE
+    assert forms[8] == M.FORM_FIXED_LANE and forms[9] == M.FORM_FIXED_LANE  # 64 limbs, lanes
     for x, y, n, P in zip(xs, ys, ns, Ps):
         oracle_check(int(n), int(P), x, y)
     b.close()
