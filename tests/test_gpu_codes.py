@@ -41,7 +41,8 @@ def test_codes_match_reference_golden(code, n, P, mp_golden):
 
 
 @pytest.mark.parametrize("code", [1, 2])
-@pytest.mark.parametrize("n,P,seed", [(4096, 1024, 5), (2048, 100, 6), (512, 333, 7)])
+@pytest.mark.parametrize("n,P,seed", [(4096, 1024, 5), (2048, 100, 6), (512, 333, 7),
+                                      (1024, 2048, 8), (256, 1111, 9)])
 def test_codes_match_oracle(code, n, P, seed):
     x = I.mp_random(n, P, seed=seed)
     assert np.array_equal(run(code, n, P, x), O.mp_fft(n, P, x, code=code))
