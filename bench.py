@@ -252,6 +252,17 @@ def measure_wide(M, I, torch, lgn, steps, seed):
     return out
 
 
+def auto_form_name(M, n, P, B):
+    """what a plan of the library's chosen form (MPFFT_FORM_AUTO) runs at
+    this configuration"""
+    names = {M.FORM_FIXED_PANEL: "fixed (panels)", M.FORM_FIXED_LANE: "fixed_lane",
+             M.FORM_DD: "double_word"}
+    plan = M.Plan(n, P, M.FORM_AUTO, B)
+    f = plan.resolved_form()
+    plan.close()
+    return names.get(f, str(f))
+
+
 def measure_mpfr(M, I, n, P, seed, reps=3):
     """the drop-in path: one transform from and to mpfr_t arrays
     (marshalling, PCIe transfers and the GPU transform), per call"""
@@ -526,6 +537,7 @@ def main():
         "forms": out,
         "chosen": chosen,
         "chosen_vs_plain": round(fft_s / res["plain"]["fft_s"], 4),
+        "library_auto_form": auto_form_name(M, n, P, B),
         "roofline": roof,
         "roofline_hbm": {"achieved": round(bytes_fft * fft_s / 1e9, 2), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(bytes_fft * fft_s / 1e9 / HBM_PEAK_GBS, 5),

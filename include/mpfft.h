@@ -131,6 +131,15 @@ typedef struct mpfft_plan_s *mpfft_plan;
                               bits; |exponent| of the input <= 800
                               (MPFFT_ERR_VALUE); normwise error below
                               2^-prec + (8 + 32 log2 n) 2^-212            */
+#define MPFFT_FORM_AUTO 13 /* the chosen form: resolved at plan creation to
+                              the fastest form whose error against a
+                              2P-bit oracle measured no worse than the plain
+                              form's, componentwise and normwise, on
+                              MI355X: MPFFT_FORM_DD for 53 <= prec <= 106,
+                              MPFFT_FORM_FIXED otherwise (which takes the
+                              panel or the per-lane kernels); the resolved
+                              form's input range applies.
+                              mpfft_plan_form() names the kernels chosen  */
 #define MPFFT_FORM_D5  5   /* double, CODE 5 op order (4m2a, 3 butterflies) */
 #define MPFFT_FORM_D6  6   /* double, CODE 6/7 op order (1 butterfly)      */
 
@@ -143,6 +152,12 @@ int mpfft_plan_destroy(mpfft_plan plan);
 
 /* limbs per real used by the device format, and words per transform */
 int mpfft_plan_info(mpfft_plan plan, long *limbs, size_t *words_per_transform);
+
+/* The form the plan runs (no counterpart in the reference): the one it was
+ * created with, MPFFT_FORM_AUTO resolved, and MPFFT_FORM_FIXED as the
+ * kernel family it took (MPFFT_FORM_FIXED_PANEL or _FIXED_LANE); a negative
+ * status for a null plan. */
+int mpfft_plan_form(mpfft_plan plan);
 
 /* One transform through mpfr_t arrays (marshalling included).  The
  * significands are marshalled on up to 16 host threads into page-locked
@@ -226,9 +241,12 @@ typedef struct mpfft_batch_s *mpfft_batch;
 /* `count` transforms; transform i has size n[i] (a power of two) at prec[i]
  * bits.  form: MPFFT_FORM_MP, _MP_C1, _MP_C2 (the reference's op orders;
  * every transform bit-identical to a plan of that form), MPFFT_FORM_FIXED
- * (autotuned fixed form), MPFFT_FORM_FIXED_LANE or _FIXED_PANEL (one kernel
- * family; groups with n < 16 run on the per-lane kernels).  Builds every
- * group's plans (twiddle tables) before it returns. */
+ * (each group on the panel or the per-lane kernels by the rule of a fixed-
+ * form plan of its size and count), MPFFT_FORM_FIXED_LANE or _FIXED_PANEL
+ * (one kernel family; groups with n < 16 run on the per-lane kernels),
+ * MPFFT_FORM_DD or _QD, or MPFFT_FORM_AUTO (per group, the resolution of a
+ * plan of that form).  Builds every group's plans (twiddle tables) before
+ * it returns. */
 int mpfft_batch_create(mpfft_batch *batch, unsigned int count, const unsigned long *n,
                        const long *prec, int form);
 int mpfft_batch_destroy(mpfft_batch batch);

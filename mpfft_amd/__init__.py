@@ -20,8 +20,9 @@ FORM_FIXED_LANE = 9                     # the fixed form on its per-lane kernels
 FORM_FIXED_PANEL = 10                   # the fixed form on panels (any batch, n >= 16)
 FORM_DD = 11                            # double-word FP64 form, 53 <= P <= 106
 FORM_QD = 12                            # quad-word FP64 form, 53 <= P <= 212
+FORM_AUTO = 13                          # the chosen form (resolved at plan creation)
 FIXED_FORMS = (FORM_FIXED, FORM_FIXED_LANE, FORM_FIXED_PANEL)
-MP_FORMS = (FORM_MP, FORM_MP_C1, FORM_MP_C2, FORM_DD, FORM_QD) + FIXED_FORMS
+MP_FORMS = (FORM_MP, FORM_MP_C1, FORM_MP_C2, FORM_DD, FORM_QD, FORM_AUTO) + FIXED_FORMS
 OK = 0
 ERRORS = {
     -1: "MPFFT_ERR_SIZE", -2: "MPFFT_ERR_PREC", -3: "MPFFT_ERR_VALUE",
@@ -33,7 +34,7 @@ ERRORS = {
 EXPORTS = [
     "mp_sr24_cp_fft", "mp_sr24_cp_fft_v2", "d_sr24_cp_fft", "d_sr24_cp_fft_v2",
     "mpfft_last_error", "mpfft_strerror", "mpfft_plan_create", "mpfft_plan_destroy",
-    "mpfft_plan_info", "mpfft_execute_mpfr", "mpfft_execute_limbs",
+    "mpfft_plan_info", "mpfft_plan_form", "mpfft_execute_mpfr", "mpfft_execute_limbs",
     "mpfft_execute_limbs_host", "mpfft_execute_limbs_range", "mpfft_plan_stages",
     "mpfft_execute_f64", "mpfft_execute_f64_host",
     "mpfft_plan_sync", "mpfft_mpfr_to_limbs", "mpfft_limbs_to_mpfr",
@@ -78,6 +79,7 @@ def lib():
     L.mpfft_execute_limbs_host.argtypes = [VP, P, P]
     L.mpfft_execute_limbs_range.argtypes = [VP, P, P, P, I, I]
     L.mpfft_plan_stages.argtypes = [VP]
+    L.mpfft_plan_form.argtypes = [VP]
     L.mpfft_execute_f64.argtypes = [VP, P, P, P]
     L.mpfft_execute_f64_host.argtypes = [VP, P, P]
     L.mpfft_plan_sync.argtypes = [VP]
@@ -172,6 +174,13 @@ class Plan:
 
     def stages(self):
         return int(lib().mpfft_plan_stages(self.h))
+
+    def resolved_form(self):
+        """the form the plan runs (FORM_AUTO resolved, FORM_FIXED as its
+        kernel family)"""
+        rc = int(lib().mpfft_plan_form(self.h))
+        check(rc if rc < 0 else 0, "plan_form")
+        return rc
 
     def execute_range(self, d_in, d_out, s0, s1, stream=None):
         s = ctypes.c_void_p(stream) if stream else None

@@ -211,7 +211,9 @@ int mpfft_batch_create(mpfft_batch *batch, unsigned int count, const unsigned lo
     *batch = nullptr;
     const bool fixed = form == MPFFT_FORM_FIXED || form == MPFFT_FORM_FIXED_LANE ||
                        form == MPFFT_FORM_FIXED_PANEL;
-    if (!fixed && form != MPFFT_FORM_MP && form != MPFFT_FORM_MP_C1 && form != MPFFT_FORM_MP_C2)
+    const bool words = form == MPFFT_FORM_DD || form == MPFFT_FORM_QD;
+    if (!fixed && !words && form != MPFFT_FORM_AUTO && form != MPFFT_FORM_MP &&
+        form != MPFFT_FORM_MP_C1 && form != MPFFT_FORM_MP_C2)
         return MPFFT_ERR_FORM;
     std::unique_ptr<mpfft_batch_s> b(new mpfft_batch_s);
     b->form = form;
@@ -241,7 +243,12 @@ int mpfft_batch_create(mpfft_batch *batch, unsigned int count, const unsigned lo
         }
         off += g.words * g.idx.size();
         const uint32_t cnt = (uint32_t)g.idx.size();
-        if (form == MPFFT_FORM_FIXED)
+        // MPFFT_FORM_AUTO: the double-word form where it applies, the fixed
+        // form elsewhere (the rule of mpfft_plan_create)
+        const bool dd = form == MPFFT_FORM_AUTO && g.P >= 53 && g.P <= 106;
+        if (dd)
+            g.form = MPFFT_FORM_DD;
+        else if (form == MPFFT_FORM_FIXED || form == MPFFT_FORM_AUTO)
             g.form = mpfft::fxp_preferred((uint32_t)g.N, cnt, (int)g.L + 2)
                          ? MPFFT_FORM_FIXED_PANEL
                          : MPFFT_FORM_FIXED_LANE;
