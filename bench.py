@@ -252,6 +252,26 @@ def measure_wide(M, I, torch, lgn, steps, seed):
     return out
 
 
+def device_copy_rate(torch, mib=2048, reps=5):
+    """HBM rate a plain device-to-device copy reaches (read + write bytes
+    over its time, HIP events): the practical streaming ceiling the pass
+    kernels are compared with beside the published peak"""
+    src = torch.empty(mib << 18, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    rate = 2 * (mib << 20) * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return rate
+
+
 def auto_form_name(M, n, P, B):
     """what a plan of the library's chosen form (MPFFT_FORM_AUTO) runs at
     this configuration"""
@@ -495,6 +515,7 @@ def main():
             prod_waves += 3 * (n // 4) * (ns - 1) // ns * (B // 64)
             ns *= 4
         mfma_ops = prod_waves * tiles * 4 * 2 * (32 * 32 * 32 * 2)
+        copy_gbs = device_copy_rate(torch)
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": profiled_traffic(chosen, n, P, B),
@@ -503,7 +524,11 @@ def main():
                 "algorithmic_bytes_per_launch": bytes_launch,
                 "mfma": {"achieved": round(mfma_ops / kern_s / 1e12, 2), "peak": MFMA_I8_PEAK_T,
                          "unit": "TOP/s (int8)", "frac": round(mfma_ops / kern_s / 1e12
-                                                               / MFMA_I8_PEAK_T, 4)}}
+                                                               / MFMA_I8_PEAK_T, 4)},
+                "device_copy": {"achieved": round(copy_gbs, 1), "unit": "GB/s",
+                                "what": "2 GiB device-to-device copy (torch), read + write bytes "
+                                        "over its time",
+                                "pass_frac_of_copy": round(achieved / copy_gbs, 4)}}
     else:
         prod_per_fft = (cmul(n) if chosen == "plain" else fixed_products(n)) * Lc * Lc
         kern_s = rc["parts_ms"][1 if chosen == "plain" else 0] * 1e-3
