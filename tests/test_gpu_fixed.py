@@ -70,6 +70,20 @@ def test_fixed_lane_wide(n, P, b):
             assert zb == 0 and mu <= 2.0, (mu,)
 
 
+@pytest.mark.parametrize("n", [1, 64, 4096])
+def test_fixed_lane_wide_impulse(n):
+    # 66-word reals on the per-lane kernels: x = delta_0 gives exactly 1
+    P = 2048
+    L = O.limbs(P)
+    hdr = np.full(2 * n, np.int32(-2**31), dtype=np.int32)
+    lim = np.zeros((L, 2 * n), dtype=np.uint32)
+    hdr[0], lim[0, 0] = 2, 0x80000000
+    y = run_fixed(n, P, I.join(hdr, lim), form=M.FORM_FIXED_LANE)
+    h, lm = I.split(y, n, L)
+    assert (h[:n] == 2).all() and (lm[0, :n] == 0x80000000).all() and (lm[1:, :n] == 0).all()
+    assert ((h[n:] & ~1) == np.int32(-2**31)).all() and (lm[:, n:] == 0).all()
+
+
 def test_fixed_wide_small_batch_takes_lanes():
     # above 1024 bits MPFFT_FORM_FIXED picks the per-lane kernels where a
     # panel would be mostly padding, and the panels at a full batch

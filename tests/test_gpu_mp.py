@@ -147,6 +147,40 @@ def test_fft_wide_matches_oracle(n, P, seed):
     assert np.array_equal(run_fft(n, P, x), O.mp_fft(n, P, x))
 
 
+def wide_special_inputs(n, P, seed):
+    """impulse, constant 0.75 - 0.75i, and a mix of signed zeros with a few
+    random entries, as SoA transforms at ceil(P/32) limbs"""
+    L, M2 = O.limbs(P), 2 * n
+    hdr = np.full(M2, -2**31, dtype=np.int32)
+    lim = np.zeros((L, M2), dtype=np.uint32)
+    hdr[0], lim[0, 0] = 2, 0x80000000
+    out = [I.join(hdr, lim)]
+    hdr = np.zeros(M2, dtype=np.int32)
+    hdr[n:] = 1
+    lim = np.zeros((L, M2), dtype=np.uint32)
+    lim[0, :] = 0xC0000000
+    out.append(I.join(hdr, lim))
+    rng = np.random.default_rng(seed)
+    hx, lx = I.split(I.mp_random(n, P, seed=seed), n, L)
+    hx, lx = hx.copy(), lx.copy()
+    z = rng.random(M2) < 0.8
+    hx[z] = np.int32(-2**31) | rng.integers(0, 2, size=int(z.sum()), dtype=np.int32)
+    lx[:, z] = 0
+    out.append(I.join(hx, lx))
+    return out
+
+
+@pytest.mark.parametrize("n", [4, 16, 256])
+@pytest.mark.parametrize("P", [1500, 2048])
+def test_fft_wide_special_inputs(n, P):
+    # 64 limbs: impulse, constant and signed-zero inputs bit-exact to the
+    # oracle (its special-input behaviour is pinned to the reference's
+    # fixtures up to 1024 bits and follows MPFR's signed-zero rules beyond)
+    for k, x in enumerate(wide_special_inputs(n, P, seed=n + P)):
+        print(f"case n={n} P={P} input {k}", flush=True)
+        assert np.array_equal(run_fft(n, P, x), O.mp_fft(n, P, x)), k
+
+
 def test_batch_equals_single_transforms():
     n, P, b = 256, 256, 3
     xs = [I.mp_random(n, P, seed=40 + i) for i in range(b)]
