@@ -153,6 +153,9 @@ int time_graph(mpfft_batch_s *b, hipGraphExec_t exec, double *ms)
 
 int tune(mpfft_batch_s *b, const uint32_t *in, uint32_t *out)
 {
+    // a graph launched before on a caller's stream finishes first: the
+    // timing runs below write the same buffers, and its exec is replaced
+    if (b->last) HIP_TRY(hipStreamSynchronize(b->last));
     // 1. every group alone on the batch's stream (its weight in the dealing)
     for (auto &g : b->groups) {
         int rc = run_group(g, in, out, b->stream);                    // warm-up

@@ -190,6 +190,13 @@ def test_batch_graph_replays():
     b.execute_device(dx2.data_ptr(), dy2.data_ptr(), stream=s.cuda_stream)
     b.check()
     assert torch.equal(dy2.cpu(), ref)
+    # tuning again while a launch on the caller stream may still run: the
+    # tuner waits for it, then times and recaptures
+    b.execute_device(dx2.data_ptr(), dy2.data_ptr(), stream=s.cuda_stream)
+    b.tune(dx2.data_ptr(), dy2.data_ptr())
+    b.execute_device(dx2.data_ptr(), dy2.data_ptr())
+    b.check()
+    assert torch.equal(dy2.cpu(), ref)
     b.close()
 
 
