@@ -220,9 +220,11 @@ int mpfft_execute_f64_host(mpfft_plan plan, const double *h_in, double *h_out);
 /* Wait for the plan's stream. */
 int mpfft_plan_sync(mpfft_plan plan);
 
-/* Wait for the plan's stream and report input-range violations of the
- * fixed form since the last check (MPFFT_ERR_VALUE); MPFFT_OK otherwise.
- * The host and mpfr_t entry points check by themselves. */
+/* Wait for the plan's own stream and report input-range violations of the
+ * fixed, double- and quad-word forms since the last check (MPFFT_ERR_VALUE);
+ * MPFFT_OK otherwise.  After executes enqueued on a caller's stream, wait
+ * for that stream first.  The host and mpfr_t entry points check by
+ * themselves. */
 int mpfft_plan_check(mpfft_plan plan);
 
 /* ---- batched form --------------------------------------------------------- */
@@ -268,12 +270,15 @@ int mpfft_batch_schedule(mpfft_batch batch, int *streams, double *ms);
 
 /* Every transform, d_in -> d_out (packed layout, no overlap), as one graph
  * launch on `stream` (a hipStream_t, NULL = the batch's own); tunes first on
- * the first call.  Returns without waiting. */
+ * the first call.  Returns without waiting.  A caller's stream must stay
+ * alive until the next mpfft_batch_check, _tune or execute on other buffers
+ * (each of which waits for it). */
 int mpfft_batch_execute_limbs(mpfft_batch batch, const uint32_t *d_in, uint32_t *d_out,
                               void *stream);
 
 /* Wait for the stream of the last execute and report input-range violations
- * of fixed-form groups since the last check (MPFFT_ERR_VALUE). */
+ * of fixed-, double- and quad-word-form groups since the last check
+ * (MPFFT_ERR_VALUE). */
 int mpfft_batch_check(mpfft_batch batch);
 
 /* ---- conversion ----------------------------------------------------------- */
