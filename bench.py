@@ -571,43 +571,68 @@ def main():
     if not a.no_err:
         line["error_seconds"] = err_s
     if not a.no_mpfr:
-        line["mpfr_path"] = measure_mpfr(M, I, n, P, a.seed)
+        side_leg(line, "mpfr_path", lambda: measure_mpfr(M, I, n, P, a.seed))
     if a.dd_prec:
-        line["precision_form"] = measure_precision_form(M, I, torch, O, n, a.dd_prec, 128, B,
-                                                        max(2, a.steps // 2), a.seed)
+        side_leg(line, "precision_form",
+                 lambda: measure_precision_form(M, I, torch, O, n, a.dd_prec, 128, B,
+                                                max(2, a.steps // 2), a.seed))
     if a.wide_lgn:
-        line["wide"] = measure_wide(M, I, torch, a.wide_lgn, 2, a.seed)
+        side_leg(line, "wide", lambda: measure_wide(M, I, torch, a.wide_lgn, 2, a.seed))
     if a.batched_count:
-        line["batched"] = measure_batched(M, I, torch, a.seed, a.batched_count,
-                                          max(2, a.steps // 2))
+        side_leg(line, "batched", lambda: measure_batched(M, I, torch, a.seed, a.batched_count,
+                                                          max(2, a.steps // 2)))
     if not a.no_cpu:
-        # bounded sample of 3 repeats (the reference's M >= 3): the reference's
-        # OpenMP CODE 2 (4 threads, hard-coded at parallel_mp_fft2.c:262) when
-        # its build is present, and the oracle restatement on every host core
-        # and on 4
-        nthr = min(os.cpu_count() or 1, 16)
-        xs = I.mp_random(n, P, seed=a.seed)
-        cpu = {}
-        if O.have_ref():
-            t = O.ref_mp_time("mpp2", n, P, xs, 3)
-            cpu = {"value": round(1.0 / t, 4), "unit": "FFT/s", "cores": 4, "kind": "reference",
-                   "sample": f"3 FFTs N={n} P={P}, reference OpenMP CODE 2 (4 threads), "
-                             "LUT and conversion excluded"}
-        t = O.mp_fft_time(n, P, xs, 3, nthr)
-        port = {"value": round(1.0 / t, 4), "unit": "FFT/s", "cores": nthr, "kind": "port",
-                "sample": f"3 FFTs N={n} P={P}, oracle restatement, {nthr} OpenMP threads, "
-                          "LUT and conversion excluded"}
-        t4 = O.mp_fft_time(n, P, xs, 3, 4)
-        port4 = dict(port, value=round(1.0 / t4, 4), cores=4,
-                     sample=f"3 FFTs N={n} P={P}, oracle restatement, 4 OpenMP threads, "
-                            "LUT and conversion excluded")
-        if cpu:
-            line["cpu_baseline"] = cpu
-            line["cpu_port"] = port
-        else:
-            line["cpu_baseline"] = port
-        line["cpu_port_4_threads"] = port4
+        side_leg(line, None, lambda: cpu_legs(O, I, n, P, a.seed))
     print(json.dumps(line))
+
+
+def side_leg(line, key, fn):
+    """Run one side leg of the bench.  A Python-level failure (an allocation
+    the device cannot hold, a library status) is recorded in the line under
+    the leg's key instead of losing the headline measurement; key None: fn
+    returns several keys."""
+    try:
+        r = fn()
+        if key is None:
+            line.update(r)
+        else:
+            line[key] = r
+    except Exception as e:                      # noqa: BLE001
+        msg = f"{type(e).__name__}: {e}"
+        line[key or "cpu_baseline"] = {"error": msg}
+        print(f"bench: {key or 'cpu'} leg failed: {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_legs(O, I, n, P, seed):
+    """bounded CPU baselines; returns the line's cpu_* keys"""
+    line = {}
+    # bounded sample of 3 repeats (the reference's M >= 3): the reference's
+    # OpenMP CODE 2 (4 threads, hard-coded at parallel_mp_fft2.c:262) when
+    # its build is present, and the oracle restatement on every host core
+    # and on 4
+    nthr = min(os.cpu_count() or 1, 16)
+    xs = I.mp_random(n, P, seed=seed)
+    cpu = {}
+    if O.have_ref():
+        t = O.ref_mp_time("mpp2", n, P, xs, 3)
+        cpu = {"value": round(1.0 / t, 4), "unit": "FFT/s", "cores": 4, "kind": "reference",
+               "sample": f"3 FFTs N={n} P={P}, reference OpenMP CODE 2 (4 threads), "
+                         "LUT and conversion excluded"}
+    t = O.mp_fft_time(n, P, xs, 3, nthr)
+    port = {"value": round(1.0 / t, 4), "unit": "FFT/s", "cores": nthr, "kind": "port",
+            "sample": f"3 FFTs N={n} P={P}, oracle restatement, {nthr} OpenMP threads, "
+                      "LUT and conversion excluded"}
+    t4 = O.mp_fft_time(n, P, xs, 3, 4)
+    port4 = dict(port, value=round(1.0 / t4, 4), cores=4,
+                 sample=f"3 FFTs N={n} P={P}, oracle restatement, 4 OpenMP threads, "
+                        "LUT and conversion excluded")
+    if cpu:
+        line["cpu_baseline"] = cpu
+        line["cpu_port"] = port
+    else:
+        line["cpu_baseline"] = port
+    line["cpu_port_4_threads"] = port4
+    return line
 
 
 if __name__ == "__main__":
