@@ -45,6 +45,11 @@
 #ifndef FXP_ACACHE
 #define FXP_ACACHE 0
 #endif
+// radix-8 pass: up to this many words per real the next item's operands are
+// fetched right after the products (more registers live through stages 1-3)
+#ifndef FXP_R8_EARLY_MAXW
+#define FXP_R8_EARLY_MAXW 10
+#endif
 
 namespace mpfft {
 
@@ -542,6 +547,7 @@ fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
     // writes and stage 3 reads set B, so an item needs two barriers
     constexpr uint32_t SETB = 8 * SLOT;
     uint32_t *c8t = lds + 16 * SLOT;
+    constexpr bool kEarlyFetch = W <= FXP_R8_EARLY_MAXW;
     for (uint32_t i = threadIdx.x; i < (uint32_t)G.RW; i += blockDim.x)
         c8t[i] = a.tw[(size_t)c8e * TWN + i];
     __syncthreads();                                   // c8t staged
@@ -600,6 +606,10 @@ fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             mf_cmul<W>(B, R, R + G.RW, R + 2 * G.RW, lane, slot);
         }
+        // re, im and tw are consumed: up to FXP_R8_EARLY_MAXW words per real
+        // (registers to spare) the next item's loads start here and overlap
+        // both barriers and stages 1-3
+        if (kEarlyFetch && nx < nitems) fetch(nx, re, im, tw);
         __syncthreads();
         // stage 1: u = v_r + v_(r+4) (r < 4) or v_(r-4) - v_r (r >= 4)
         Fx<W> ur, ui;
@@ -651,10 +661,10 @@ fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
 #pragma unroll
             for (int i = 0; i < W; i++) slotb[(W + i) * 64 + lane] = ui.w[i];
         }
-        // the next item's operands load during stage 3 and the stores (issued
-        // here, after both products, to keep them out of the products'
-        // registers)
-        if (nx < nitems) fetch(nx, re, im, tw);
+        // wider reals: the next item's operands load during stage 3 and the
+        // stores (issued here, after both products, to keep them out of the
+        // products' registers)
+        if (!kEarlyFetch && nx < nitems) fetch(nx, re, im, tw);
         __syncthreads();
         // stage 3: output q = q0 + 2 q1 with q0 = r / 4, q1 = r % 4: the DFT4 of
         // the slots of inputs 4 q0 + r1, r1 = 0..3
