@@ -161,7 +161,7 @@ int mpfft_plan_form(mpfft_plan plan);
 
 /* One transform through mpfr_t arrays (marshalling included).  The
  * significands are marshalled on up to 16 host threads into page-locked
- * staging, in chunks whose transfers overlap the conversion of the next. */
+ * staging, with one contiguous copy each way. */
 int mpfft_execute_mpfr(mpfft_plan plan, mpfr_t *inr, mpfr_t *ini,
                        mpfr_t *outr, mpfr_t *outi);
 
