@@ -170,6 +170,47 @@ int mpfr_put(mpfr_srcptr x, uint32_t *hdr, uint32_t *limb, size_t M, size_t i, l
     return MPFFT_OK;
 }
 
+// mpfr_put for the points k0 .. k1-1 of one array (SoA positions base + k),
+// in blocks of 32 points: the headers and significand pointers first, then
+// the limb rows one at a time, so the stores run along each row
+int mpfr_put_range(mpfr_t *x, unsigned long k0, unsigned long k1, size_t base, uint32_t *hdr,
+                   uint32_t *limb, size_t M, long L)
+{
+    constexpr unsigned kBlk = 32;
+    const mp_limb_t *d[kBlk];
+    long n64[kBlk];
+    for (unsigned long b0 = k0; b0 < k1; b0 += kBlk) {
+        const unsigned nb = (unsigned)std::min<unsigned long>(kBlk, k1 - b0);
+        for (unsigned u = 0; u < nb; u++) {
+            mpfr_srcptr v = x[b0 + u];
+            const size_t i = base + b0 + u;
+            d[u] = nullptr;
+            n64[u] = 0;
+            if (mpfr_nan_p(v) || mpfr_inf_p(v)) return MPFFT_ERR_VALUE;
+            const uint32_t neg = mpfr_signbit(v) ? 1u : 0u;
+            if (mpfr_zero_p(v)) { hdr[i] = 0x80000000u | neg; continue; }
+            const long e = mpfr_get_exp(v);
+            if (e >= kExpLimit || e <= -kExpLimit) return MPFFT_ERR_VALUE;
+            hdr[i] = ((uint32_t)e << 1) | neg;
+            d[u] = (const mp_limb_t *)mpfr_custom_get_significand(v);
+            n64[u] = (mpfr_get_prec(v) + 63) / 64;
+        }
+        for (long j = 0; j < L; j++) {
+            const long t = j >> 1;           // 64-bit limb index from the top
+            uint32_t *row = limb + j * M + base + b0;
+            for (unsigned u = 0; u < nb; u++) {
+                uint32_t w = 0;
+                if (t < n64[u]) {
+                    const mp_limb_t v = d[u][n64[u] - 1 - t];
+                    w = (j & 1) ? (uint32_t)v : (uint32_t)(v >> 32);
+                }
+                row[u] = w;
+            }
+        }
+    }
+    return MPFFT_OK;
+}
+
 void mpfr_take(mpfr_ptr x, const uint32_t *hdr, const uint32_t *limb, size_t M, size_t i, long L)
 {
     const uint32_t h = hdr[i], neg = h & 1u;
@@ -1051,11 +1092,8 @@ int mpfft_execute_mpfr(mpfft_plan p, mpfr_t *inr, mpfr_t *ini, mpfr_t *outr, mpf
         th.emplace_back([&, w] {
             for (unsigned c = w; c < C; c += nthr) {
                 const unsigned long k0 = c * cl, k1 = std::min(n, k0 + cl);
-                int r = MPFFT_OK;
-                for (unsigned long k = k0; k < k1 && !r; k++) {
-                    r = mpfr_put(inr[k], hi, hi + M, M, k, L);
-                    if (!r) r = mpfr_put(ini[k], hi, hi + M, M, n + k, L);
-                }
+                int r = k1 > k0 ? mpfr_put_range(inr, k0, k1, 0, hi, hi + M, M, L) : MPFFT_OK;
+                if (!r && k1 > k0) r = mpfr_put_range(ini, k0, k1, n, hi, hi + M, M, L);
                 state[c].store(r ? r : 1, std::memory_order_release);
             }
         });
