@@ -1035,12 +1035,8 @@ int mpfft_mpfr_to_limbs(unsigned long n, long limbs, mpfr_t *re, mpfr_t *im, uin
     if (!re || !im || !h_out || limbs < 1) return MPFFT_ERR_ARG;
     const size_t M = 2 * n;
     uint32_t *hdr = h_out, *limb = h_out + M;
-    for (unsigned long k = 0; k < n; k++) {
-        int rc = mpfr_put(re[k], hdr, limb, M, k, limbs);
-        if (!rc) rc = mpfr_put(im[k], hdr, limb, M, n + k, limbs);
-        if (rc) return rc;
-    }
-    return MPFFT_OK;
+    const int rc = mpfr_put_range(re, 0, n, 0, hdr, limb, M, limbs);
+    return rc ? rc : mpfr_put_range(im, 0, n, n, hdr, limb, M, limbs);
 }
 
 int mpfft_limbs_to_mpfr(unsigned long n, long limbs, const uint32_t *h_in, mpfr_t *re, mpfr_t *im)
