@@ -74,6 +74,21 @@ def test_inverse_batch():
         assert np.array_equal(got[i], oracle_inverse(n, P, Xs[i]))
 
 
+@pytest.mark.parametrize("n,P", [(1024, 2048), (4096, 1500), (16, 1056)])
+def test_roundtrip_wide_error_scale(n, P):
+    # 64 limbs: the statistics themselves underflow the doubles (2^-P), so
+    # the check is on their log2 fields: the reference's round trip of a
+    # correctly rounded P-bit transform stays within a few units of 2^-P
+    x = I.mp_random(n, P, seed=950 + n + P)
+    plan = M.Plan(n, P, M.FORM_MP, 1)
+    st = plan.roundtrip_host(I.relimb(x, n, O.limbs(P), plan.L))[0]
+    plan.close()
+    for j in (4, 5):                       # log2 rRMSE, real and imaginary parts
+        assert -P - 4 < st[j] < -P + 6, (j, st[j])
+    for j in (6, 7):                       # log2 max |error|
+        assert -P - 4 < st[j] < -P + 8, (j, st[j])
+
+
 @pytest.mark.parametrize("n,P", [(64, 128), (1024, 256), (1024, 1024), (256, 53)])
 def test_roundtrip_stats_match_oracle(n, P):
     x = I.mp_random(n, P, seed=900 + n + P)
