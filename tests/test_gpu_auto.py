@@ -88,3 +88,32 @@ def test_batch_auto_groups():
         plan.close()
         assert np.array_equal(ys[i], yp[same.index(i)]), (n, P)
     b.close()
+
+
+def test_batch_quad_word_groups():
+    # the quad-word form as a batch form: every transform equals a plan of
+    # that form over its group; a precision beyond the form is refused
+    ns = np.array([256, 16, 256, 1024], dtype=np.int64)
+    Ps = np.array([150, 212, 150, 60], dtype=np.int64)
+    xs = [I.mp_random(int(n), int(P), seed=2800 + i) for i, (n, P) in enumerate(zip(ns, Ps))]
+    b = M.Batch(ns, Ps, M.FORM_QD)
+    assert (b.forms() == M.FORM_QD).all()
+    host = b.pack([I.relimb(x, int(n), O.limbs(int(P)), int(L))
+                   for x, n, P, L in zip(xs, b.ns, b.Ps, b.limbs)])
+    dx = torch.from_numpy(host.view(np.int32)).cuda()
+    dy = torch.zeros_like(dx)
+    b.execute_device(dx.data_ptr(), dy.data_ptr())
+    b.check()
+    ys = b.unpack(dy.cpu().numpy().view(np.uint32))
+    b.close()
+    for i in range(len(xs)):
+        n, P = int(ns[i]), int(Ps[i])
+        same = [j for j in range(len(xs)) if int(ns[j]) == n and int(Ps[j]) == P]
+        plan = M.Plan(n, P, M.FORM_QD, len(same))
+        yp = np.split(plan.execute_host(np.concatenate(
+            [I.relimb(xs[j], n, O.limbs(P), plan.L) for j in same])), len(same))
+        plan.close()
+        assert np.array_equal(ys[i], yp[same.index(i)]), (n, P)
+    with pytest.raises(M.MpfftError) as e:
+        M.Batch(np.array([64]), np.array([300]), M.FORM_QD)
+    assert e.value.code == -2
