@@ -246,12 +246,9 @@ int mpfft_batch_create(mpfft_batch *batch, unsigned int count, const unsigned lo
         }
         off += g.words * g.idx.size();
         const uint32_t cnt = (uint32_t)g.idx.size();
-        // MPFFT_FORM_AUTO: the double-word form where it applies, the fixed
-        // form elsewhere (the rule of mpfft_plan_create)
-        const bool dd = form == MPFFT_FORM_AUTO && g.P >= 53 && g.P <= 106;
-        if (dd)
-            g.form = MPFFT_FORM_DD;
-        else if (form == MPFFT_FORM_FIXED || form == MPFFT_FORM_AUTO)
+        // MPFFT_FORM_AUTO goes to the plan as it is (mpfft_plan_create
+        // resolves it, with the group's count as the batch)
+        if (form == MPFFT_FORM_FIXED)
             g.form = mpfft::fxp_preferred((uint32_t)g.N, cnt, (int)g.L + 2)
                          ? MPFFT_FORM_FIXED_PANEL
                          : MPFFT_FORM_FIXED_LANE;
@@ -291,6 +288,7 @@ int mpfft_batch_create(mpfft_batch *batch, unsigned int count, const unsigned lo
     for (auto &x : th) x.join();
     for (int rc : rcs)
         if (rc) return rc;                   // the destructor frees what was built
+    for (auto &g : b->groups) g.form = mpfft_plan_form(g.plan);   // what each plan runs
     *batch = b.release();
     return MPFFT_OK;
 }
