@@ -75,19 +75,22 @@ __device__ __forceinline__ DD dd_mul(const DD &x, const DD &y)
     return fast_two_sum(p, e);
 }
 
-// P-bit limb value (53 <= P <= 106) -> hi + lo, exact.  The significand's
-// top 53 bits rounded to nearest (ties to even) give hi; the remainder, at
-// most 53 bits wide because P <= 106, is lo.  Returns 1 when the exponent
-// lies outside [-lim, lim] (binary64 could not hold lo).
+// P-bit limb value (53 <= P <= 106) times 2^-eoff (the transform's block
+// exponent, an exact shift) -> hi + lo, exact.  The significand's top 53
+// bits rounded to nearest (ties to even) give hi; the remainder, at most 53
+// bits wide because P <= 106, is lo.  A scaled exponent e below -lim
+// (binary64 could not hold lo; the value is below 2^-lim of the transform's
+// largest input) is taken as zero; one above lim returns 1.
 template <int L>
-__device__ __forceinline__ uint32_t dd_from_mp(DD &z, const Mp<L> &x, int lim)
+__device__ __forceinline__ uint32_t dd_from_mp(DD &z, const Mp<L> &x, int lim, int32_t eoff)
 {
-    if (x.zero) {
+    const int32_t e = x.e - eoff;
+    if (x.zero || e < -lim) {
         z.hi = x.neg ? -0.0 : 0.0;
         z.lo = 0.0;
         return 0;
     }
-    if (x.e > lim || x.e < -lim) {
+    if (e > lim) {
         z.hi = 0.0;
         z.lo = 0.0;
         return 1;
@@ -102,8 +105,8 @@ __device__ __forceinline__ uint32_t dd_from_mp(DD &z, const Mp<L> &x, int lim)
         h += 1;
         r -= 2 * half;
     }
-    const double hi = __builtin_ldexp((double)h, x.e - 53);
-    const double lo = __builtin_ldexp((double)r, x.e - 106);
+    const double hi = __builtin_ldexp((double)h, e - 53);
+    const double lo = __builtin_ldexp((double)r, e - 106);
     z.hi = x.neg ? -hi : hi;
     z.lo = x.neg ? -lo : lo;
     return 0;
@@ -129,14 +132,15 @@ __device__ __forceinline__ void mp_from_double(Mp<L> &z, double v)
     z.zero = 0u;
 }
 
-// hi + lo rounded once to P bits
+// (hi + lo) 2^eoff rounded once to P bits
 template <int L, bool FULL>
-__device__ __forceinline__ void dd_to_mp(Mp<L> &z, const DD &x, int P)
+__device__ __forceinline__ void dd_to_mp(Mp<L> &z, const DD &x, int P, int32_t eoff)
 {
     Mp<L> a, b;
     mp_from_double(a, x.hi);
     mp_from_double(b, x.lo);
     mp_add<L, FULL>(z, a, 0u, b, 0u, P);
+    if (!z.zero) z.e += eoff;
 }
 
 // ---- quad-word numbers (K = 4) ---------------------------------------------
@@ -292,19 +296,21 @@ __device__ __forceinline__ CW_<K> w_add_ib(const CW_<K> &a, const CW_<K> &b)   /
     return {Word<K>::sub(a.re, b.im), Word<K>::add(a.im, b.re)};
 }
 
-// P-bit limb value -> quad-word, exact for P <= 212: the significand cut
-// into four 53-bit pieces (all of the value's sign), then renormalised.
-// Returns 1 when |exponent| > lim (the last piece would leave the normal
-// binary64 range).
+// P-bit limb value times 2^-eoff (the block exponent) -> quad-word, exact
+// for P <= 212: the significand cut into four 53-bit pieces (all of the
+// value's sign), then renormalised.  A scaled exponent below -lim (the last
+// piece would leave the normal binary64 range; the value is below 2^-lim of
+// the transform's largest input) is taken as zero; one above lim returns 1.
 template <int L>
-__device__ __forceinline__ uint32_t qd_from_mp(QD &z, const Mp<L> &x, int lim)
+__device__ __forceinline__ uint32_t qd_from_mp(QD &z, const Mp<L> &x, int lim, int32_t eoff)
 {
     z = {{0.0, 0.0, 0.0, 0.0}};
-    if (x.zero) {
+    const int32_t e = x.e - eoff;
+    if (x.zero || e < -lim) {
         z.c[0] = x.neg ? -0.0 : 0.0;
         return 0;
     }
-    if (x.e > lim || x.e < -lim) return 1;
+    if (e > lim) return 1;
     double c[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -314,7 +320,7 @@ __device__ __forceinline__ uint32_t qd_from_mp(QD &z, const Mp<L> &x, int lim)
                        w2 = li + 2 < L ? x.m[li + 2] : 0u;
         uint64_t top = (w0 << 32) | w1;
         if (off) top = (top << off) | (w2 >> (32 - off));
-        const double v = __builtin_ldexp((double)(top >> 11), x.e - 53 * (i + 1));
+        const double v = __builtin_ldexp((double)(top >> 11), e - 53 * (i + 1));
         c[i] = x.neg ? -v : v;
     }
     z = q_renorm(c[0], c[1], c[2], c[3]);
@@ -326,7 +332,7 @@ __device__ __forceinline__ uint32_t qd_from_mp(QD &z, const Mp<L> &x, int lim)
 // the partial sums lie ~2^-53 under the result, so their rounding cannot
 // move the final one but at an exact tie)
 template <int L, bool FULL>
-__device__ __forceinline__ void qd_to_mp(Mp<L> &z, const QD &x, int P)
+__device__ __forceinline__ void qd_to_mp(Mp<L> &z, const QD &x, int P, int32_t eoff)
 {
     Mp<L> a, b;
     mp_from_double(a, x.c[2]);
@@ -336,20 +342,21 @@ __device__ __forceinline__ void qd_to_mp(Mp<L> &z, const QD &x, int P)
     mp_add<L, true>(b, a, 0u, b, 0u, 32 * L);
     mp_from_double(a, x.c[0]);
     mp_add<L, FULL>(z, a, 0u, b, 0u, P);
+    if (!z.zero) z.e += eoff;
 }
 
 template <int K, int L>
-__device__ __forceinline__ uint32_t w_from_mp(RW_<K> &z, const Mp<L> &x, int lim)
+__device__ __forceinline__ uint32_t w_from_mp(RW_<K> &z, const Mp<L> &x, int lim, int32_t eoff)
 {
-    if constexpr (K == 2) return dd_from_mp(z, x, lim);
-    else return qd_from_mp(z, x, lim);
+    if constexpr (K == 2) return dd_from_mp(z, x, lim, eoff);
+    else return qd_from_mp(z, x, lim, eoff);
 }
 
 template <int K, int L, bool FULL>
-__device__ __forceinline__ void w_to_mp(Mp<L> &z, const RW_<K> &x, int P)
+__device__ __forceinline__ void w_to_mp(Mp<L> &z, const RW_<K> &x, int P, int32_t eoff)
 {
-    if constexpr (K == 2) dd_to_mp<L, FULL>(z, x, P);
-    else qd_to_mp<L, FULL>(z, x, P);
+    if constexpr (K == 2) dd_to_mp<L, FULL>(z, x, P, eoff);
+    else qd_to_mp<L, FULL>(z, x, P, eoff);
 }
 
 // ---- passes ------------------------------------------------------------------
@@ -365,12 +372,13 @@ __device__ __forceinline__ CW_<K> w_load(const FxArgs &a, const double2 *src, ui
     CW_<K> v;
     if constexpr (IN) {
         const uint32_t *in = a.in + a.in_stride * b;
+        const int32_t eo = a.eoff[b];
         Mp<L> x;
         uint32_t bad = 0;
         mp_load(x, in, 2 * a.N, e);
-        bad |= w_from_mp<K, L>(v.re, x, a.lim);
+        bad |= w_from_mp<K, L>(v.re, x, a.lim, eo);
         mp_load(x, in, 2 * a.N, a.N + e);
-        bad |= w_from_mp<K, L>(v.im, x, a.lim);
+        bad |= w_from_mp<K, L>(v.im, x, a.lim, eo);
         if (bad) atomicOr(a.flag, 1u);
     } else {
         const double2 *p = src + a.stride * b;
@@ -386,10 +394,11 @@ __device__ __forceinline__ void w_store(const FxArgs &a, double2 *dst, uint32_t 
 {
     if constexpr (OUT) {
         uint32_t *out = a.out + a.out_stride * b;
+        const int32_t eo = a.eoff[b];
         Mp<L> z;
-        w_to_mp<K, L, FULL>(z, v.re, a.P);
+        w_to_mp<K, L, FULL>(z, v.re, a.P, eo);
         mp_store(z, out, 2 * a.N, e);
-        w_to_mp<K, L, FULL>(z, v.im, a.P);
+        w_to_mp<K, L, FULL>(z, v.im, a.P, eo);
         mp_store(z, out, 2 * a.N, a.N + e);
     } else {
         double2 *p = dst + a.stride * b;

@@ -157,19 +157,21 @@ MPFFT_FN void fx_store(const Fx<W> &x, uint32_t *__restrict__ buf, uint32_t M, u
 
 // ---- conversions ---------------------------------------------------------------
 
-// P-bit float (L limbs, MS first, value 0.m 2^e) -> fixed point with F
-// fraction bits, truncated toward zero.  Returns 1 when |x| >= 2^lim
+// P-bit float (L limbs, MS first, value 0.m 2^e) scaled by 2^-eoff (the
+// transform's block exponent: an exact shift) -> fixed point with F
+// fraction bits, truncated toward zero.  Returns 1 when |x 2^-eoff| >= 2^lim
 // (the caller's range limit; the result is then saturated to zero).
 template <int L, int W>
-MPFFT_FN uint32_t fx_from_mp(Fx<W> &z, const Mp<L> &x, int lim)
+MPFFT_FN uint32_t fx_from_mp(Fx<W> &z, const Mp<L> &x, int lim, int32_t eoff = 0)
 {
     constexpr int F = FxDig<W>::F;
 #pragma unroll
     for (int j = 0; j < W; j++) z.w[j] = 0;
     if (x.zero) return 0;
-    if (x.e > lim) return 1;
+    const int32_t e = x.e - eoff;
+    if (e > lim) return 1;
     // X = M * 2^sh, M = the L limbs as an integer, sh = e - 32L + F
-    const int32_t sh = x.e - 32 * L + F;
+    const int32_t sh = e - 32 * L + F;
     uint32_t t[W];
 #pragma unroll
     for (int j = 0; j < W; j++) t[j] = j < L ? x.m[L - 1 - j] : 0u;   // LS first
@@ -211,10 +213,11 @@ MPFFT_FN uint32_t fx_from_mp(Fx<W> &z, const Mp<L> &x, int lim)
     return 0;
 }
 
-// fixed point -> P-bit float, correctly rounded (to nearest, ties to even);
-// zero converts to +0
+// fixed point -> P-bit float, correctly rounded (to nearest, ties to even),
+// times 2^eoff (the block exponent taken off by fx_from_mp); zero converts
+// to +0
 template <int W, int L, bool FULL>
-MPFFT_FN void fx_to_mp(Mp<L> &z, const Fx<W> &x, int P)
+MPFFT_FN void fx_to_mp(Mp<L> &z, const Fx<W> &x, int P, int32_t eoff = 0)
 {
     constexpr int F = FxDig<W>::F;
     const uint32_t neg = fx_sign(x);
@@ -251,7 +254,7 @@ MPFFT_FN void fx_to_mp(Mp<L> &z, const Fx<W> &x, int P)
     round_window<L, FULL>(win, sticky != 0, e, P);
 #pragma unroll
     for (int j = 0; j < L; j++) z.m[j] = win[j];
-    z.e = e; z.neg = neg; z.zero = 0;
+    z.e = e + eoff; z.neg = neg; z.zero = 0;
 }
 
 }  // namespace mpfft

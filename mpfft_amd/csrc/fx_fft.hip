@@ -82,11 +82,12 @@ __device__ __forceinline__ void fx_pass_load(Fx<W> &re, Fx<W> &im, const FxArgs 
     const uint32_t N = a.N, M = 2 * N;
     if (IO & 1) {
         const uint32_t *in = a.in + a.in_stride * b;
+        const int32_t eo = a.eoff[b];
         Mp<L> x;
         mp_load(x, in, M, i);
-        uint32_t bad = fx_from_mp<L, W>(re, x, a.lim);
+        uint32_t bad = fx_from_mp<L, W>(re, x, a.lim, eo);
         mp_load(x, in, M, N + i);
-        bad |= fx_from_mp<L, W>(im, x, a.lim);
+        bad |= fx_from_mp<L, W>(im, x, a.lim, eo);
         if (bad) atomicOr(a.flag, 1u);
     } else {
         fx_load(re, src, M, i);
@@ -101,15 +102,93 @@ __device__ __forceinline__ void fx_pass_store(const Fx<W> &re, const Fx<W> &im, 
     const uint32_t N = a.N, M = 2 * N;
     if (IO & 2) {
         uint32_t *out = a.out + a.out_stride * b;
+        const int32_t eo = a.eoff[b];
         Mp<L> z;
-        fx_to_mp<W, L, FULL>(z, re, a.P);
+        fx_to_mp<W, L, FULL>(z, re, a.P, eo);
         mp_store(z, out, M, o);
-        fx_to_mp<W, L, FULL>(z, im, a.P);
+        fx_to_mp<W, L, FULL>(z, im, a.P, eo);
         mp_store(z, out, M, N + o);
     } else {
         fx_store(re, dst, M, o);
         fx_store(im, dst, M, N + o);
     }
+}
+
+// ---- block exponent -----------------------------------------------------------
+//
+// The header word of a nonzero real is (exp << 1) | sign, of a zero
+// INT32_MIN | sign, so the arithmetic shift hdr >> 1 is the exponent of a
+// nonzero real and -2^30 (below every exponent MPFR allows) for a zero.
+
+constexpr int32_t kNoExp = -(1 << 30);
+
+__device__ __forceinline__ int32_t wave_max(int32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int32_t u = __shfl_xor(v, o, 64);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+// grid (eoff_nblk, batch): block c of transform b writes the largest
+// exponent of its part of the 2N header words to eoff_part[b nblk + c]
+__global__ void __launch_bounds__(256)
+fx_eoff_part_kernel(FxArgs a)
+{
+    __shared__ int32_t red[4];
+    const uint32_t M = 2 * a.N, nb = a.eoff_nblk, c = blockIdx.x, b = blockIdx.y;
+    const uint32_t chunk = (M + nb - 1) / nb, i0 = c * chunk;
+    const uint32_t i1 = i0 + chunk < M ? i0 + chunk : M;
+    const int32_t *h = reinterpret_cast<const int32_t *>(a.in + a.in_stride * b);
+    int32_t m0 = kNoExp, m1 = kNoExp;
+    uint32_t i = i0 + threadIdx.x;
+#pragma unroll 1
+    for (; i + 256 < i1; i += 512) {
+        const int32_t u = h[i] >> 1, v = h[i + 256] >> 1;
+        m0 = u > m0 ? u : m0;
+        m1 = v > m1 ? v : m1;
+    }
+    if (i < i1) {
+        const int32_t u = h[i] >> 1;
+        m0 = u > m0 ? u : m0;
+    }
+    int32_t m = wave_max(m0 > m1 ? m0 : m1);
+    if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = red[0];
+#pragma unroll
+        for (int w = 1; w < 4; w++) m = red[w] > m ? red[w] : m;
+        a.eoff_part[(size_t)b * nb + c] = m;
+    }
+}
+
+// one thread per transform: eoff[b] from the partial maxima
+__global__ void __launch_bounds__(256)
+fx_eoff_final_kernel(FxArgs a)
+{
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= a.batch) return;
+    const int32_t *p = a.eoff_part + (size_t)b * a.eoff_nblk;
+    int32_t m = kNoExp;
+#pragma unroll 1
+    for (uint32_t c = 0; c < a.eoff_nblk; c++) m = p[c] > m ? p[c] : m;
+    if (m == kNoExp) m = 0;                                  // all zero
+    if (m >= kInputExpLimit || m <= -kInputExpLimit) {
+        atomicOr(a.flag, 1u);
+        m = 0;
+    }
+    a.eoff[b] = m;
+}
+
+hipError_t fx_eoff_launch(const FxArgs &a, hipStream_t st)
+{
+    if (a.eoff_nblk != fx_eoff_blocks(a.N)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fx_eoff_part_kernel, dim3(a.eoff_nblk, a.batch), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(fx_eoff_final_kernel, dim3((a.batch + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError();
 }
 
 // float SoA (hdr + L limbs) -> fixed point; flags inputs beyond the range
@@ -123,7 +202,7 @@ fx_in_kernel(FxArgs a)
     Mp<L> x;
     mp_load(x, a.in + a.in_stride * blockIdx.y, M, i);
     Fx<W> z;
-    if (fx_from_mp<L, W>(z, x, a.lim)) atomicOr(a.flag, 1u);
+    if (fx_from_mp<L, W>(z, x, a.lim, a.eoff[blockIdx.y])) atomicOr(a.flag, 1u);
     fx_store(z, a.buf0 + a.stride * blockIdx.y, M, i);
 }
 
@@ -138,7 +217,7 @@ fx_out_kernel(FxArgs a, const uint32_t *src)
     Fx<W> x;
     fx_load(x, src + a.stride * blockIdx.y, M, i);
     Mp<L> z;
-    fx_to_mp<W, L, FULL>(z, x, a.P);
+    fx_to_mp<W, L, FULL>(z, x, a.P, a.eoff[blockIdx.y]);
     mp_store(z, a.out + a.out_stride * blockIdx.y, M, i);
 }
 

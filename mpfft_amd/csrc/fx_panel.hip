@@ -385,7 +385,7 @@ fxp_in_kernel(FxArgs a, uint32_t *dst)
             if (b < a.batch) {
                 Mp<L> x;
                 mp_load(x, a.in + a.in_stride * b, M, eb * 32 + el);
-                if (fx_from_mp<L, W>(z, x, a.lim)) atomicOr(a.flag, 1u);
+                if (fx_from_mp<L, W>(z, x, a.lim, a.eoff[b])) atomicOr(a.flag, 1u);
             } else {
 #pragma unroll
                 for (int w = 0; w < W; w++) z.w[w] = 0u;
@@ -698,14 +698,15 @@ fxp_out_kernel(FxArgs a, const uint32_t *src)
     const uint32_t g16 = lb % ng, eb = lb / ng;
     {
         // lane = (transform t % 16, element t / 16 and that + 16)
-        const uint32_t bl = t & 15u;
+        const uint32_t bl = t & 15u, b = g16 * 16 + bl;
+        const int32_t eo = b < a.batch ? a.eoff[b] : 0;     // padding transforms: none
 #pragma unroll 1
         for (uint32_t el = t >> 4; el < 32; el += 16) {
-            const uint32_t *p = src + pan_base<W>(g16 * 16 + bl, eb * 32 + el, N);
+            const uint32_t *p = src + pan_base<W>(b, eb * 32 + el, N);
             Fx<W> x;
             pan_get<W>(x, p);
             Mp<L> z;
-            fx_to_mp<W, L, FULL>(z, x, a.P);
+            fx_to_mp<W, L, FULL>(z, x, a.P, eo);
             tile[el * 17 + bl] =
                 z.zero ? ((uint32_t)kHdrZero | z.neg) : (((uint32_t)z.e << 1) | z.neg);
 #pragma unroll

@@ -75,10 +75,37 @@ struct FxArgs {
     uint32_t ntw;               // twiddle entries in all
     uint32_t *flag;             // set to nonzero when an input exceeds the range
     uint32_t N, batch;
-    int lim;                    // inputs must have exponent <= lim (|x| < 2^lim)
+    int lim;                    // scaled inputs x 2^-eoff must have exponent <= lim
+                                // (|x 2^-eoff| < 2^lim); the word forms: >= -lim
+                                // too, smaller ones are taken as zero
     int P;
     uint32_t ncu;               // compute units of the device (persistent grids)
+    // block exponent: per transform b, eoff[b] = the largest exponent of its
+    // 2N input reals (0 when all are zero); the conversions divide the input
+    // by 2^eoff[b] and multiply the output by it (exact shifts), so the
+    // fixed-point and word forms see |x| < 1 whatever the input's scale
+    int32_t *eoff;              // batch entries
+    int32_t *eoff_part;         // batch x eoff_nblk partial maxima of the scan
+    uint32_t eoff_nblk;         // blocks of the scan per transform (fx_eoff_blocks)
 };
+
+// Exponent range of the limb format's inputs: |exponent| < kInputExpLimit
+// (MPFR's default range is 1 - 2^30 .. 2^30 - 1; an FFT of up to 2^24
+// points keeps every intermediate and output inside it from there).
+constexpr int32_t kInputExpLimit = 1 << 29;
+
+// blocks per transform of the block-exponent scan: one per 8192 header words
+inline uint32_t fx_eoff_blocks(uint32_t N)
+{
+    const uint32_t b = (2 * N + 8191) / 8192;
+    return b < 1 ? 1u : b;
+}
+
+// The block-exponent scan of the fixed and word forms: eoff[b] for every
+// transform of a.in (two launches, no memset: it can be captured in a
+// graph).  Flags (a.flag) a transform whose largest exponent is outside
+// the limb format's input range.
+hipError_t fx_eoff_launch(const FxArgs &a, hipStream_t st);
 
 // fixed-point words per real for limb count L (F = 32 (L + 1) fraction bits)
 constexpr int fx_words(int L) { return L + 2; }

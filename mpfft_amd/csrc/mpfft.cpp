@@ -13,6 +13,8 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -107,13 +109,18 @@ struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
     ~DevBuf() { if (p) (void)hipFree(p); }
+    // bytes is the size of what p holds: 0 after a failed allocation, so a
+    // size comparison never takes a failed buffer for a usable one
     int alloc(size_t b)
     {
         if (p) { (void)hipFree(p); p = nullptr; }
-        bytes = b;
+        bytes = 0;
         if (!b) return MPFFT_OK;
-        return hipMalloc(&p, b) == hipSuccess ? MPFFT_OK : MPFFT_ERR_HIP;
+        if (hipMalloc(&p, b) != hipSuccess) { p = nullptr; return MPFFT_ERR_HIP; }
+        bytes = b;
+        return MPFFT_OK;
     }
+    bool holds(size_t b) const { return p && bytes == b; }
     template <class T> int upload(const std::vector<T> &v)
     {
         int rc = alloc(sizeof(T) * (v.empty() ? 1 : v.size()));
@@ -135,9 +142,13 @@ struct PinnedBuf {
     {
         if (b == bytes && p) return MPFFT_OK;
         if (p) { (void)hipHostFree(p); p = nullptr; }
+        bytes = 0;
+        if (hipHostMalloc(&p, b, hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return MPFFT_ERR_HIP;
+        }
         bytes = b;
-        return hipHostMalloc(&p, b, hipHostMallocDefault) == hipSuccess ? MPFFT_OK
-                                                                        : MPFFT_ERR_HIP;
+        return MPFFT_OK;
     }
     template <class T> T *as() const { return (T *)p; }
 };
@@ -292,6 +303,7 @@ struct mpfft_plan_s {
     std::vector<LevelArgs> levels;
     // fixed form
     DevBuf fx_tw, fx_buf0, fx_buf1, fx_flag;
+    DevBuf fx_eoff;          // block exponents: batch, then batch x fx_eoff_blocks(N) partials
     int fx_lim = 0;
     uint32_t fx_ntw = 0;
     bool fx_panel = false;   // panel kernels (matrix-core twiddle products)
@@ -523,7 +535,9 @@ int build_fx_plan(mpfft_plan_s *p)
     const size_t fxw = 2 * N * (size_t)W * (p->fx_panel ? fxp_groups(p->batch) * kPanel : p->batch);
     if ((rc = p->fx_buf0.alloc(sizeof(uint32_t) * fxw)) ||
         (rc = p->fx_buf1.alloc(sizeof(uint32_t) * fxw)) ||
-        (rc = p->fx_flag.upload(std::vector<uint32_t>{0u})))
+        (rc = p->fx_flag.upload(std::vector<uint32_t>{0u})) ||
+        (rc = p->fx_eoff.alloc(sizeof(int32_t) * (size_t)p->batch *
+                               (1 + fx_eoff_blocks((uint32_t)N)))))
         return rc;
     p->fx_ntw = (uint32_t)T3;
     p->fx_lim = 29 - lg0;
@@ -589,7 +603,9 @@ int build_dd_plan(mpfft_plan_s *p)
     if ((rc = p->fx_tw.upload(tab))) return rc;
     const size_t bytes = K * sizeof(double) * 2 * N * (size_t)p->batch;
     if ((rc = p->fx_buf0.alloc(bytes)) || (rc = p->fx_buf1.alloc(bytes)) ||
-        (rc = p->fx_flag.upload(std::vector<uint32_t>{0u})))
+        (rc = p->fx_flag.upload(std::vector<uint32_t>{0u})) ||
+        (rc = p->fx_eoff.alloc(sizeof(int32_t) * (size_t)p->batch *
+                               (1 + fx_eoff_blocks((uint32_t)N)))))
         return rc;
     p->fx_ntw = (uint32_t)ent.size();
     p->fx_lim = K == 2 ? kDdExpLimit : kQdExpLimit;
@@ -611,8 +627,13 @@ int run_fx(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t s
     a.N = (uint32_t)p->N; a.batch = p->batch;
     a.lim = p->fx_lim; a.P = (int)p->P;
     a.ncu = p->ncu;
+    a.eoff = p->fx_eoff.as<int32_t>();
+    a.eoff_part = a.eoff + p->batch;
+    a.eoff_nblk = fx_eoff_blocks((uint32_t)p->N);
     const int K = word_count(p->form);
     if (K) a.stride = (uint64_t)p->N * K;                      // K/2 double2 per real
+    // stage 0 reads the input: the block exponents come first
+    if (s0 == 0 && s1 > 0 && fx_eoff_launch(a, st) != hipSuccess) return MPFFT_ERR_HIP;
     hipError_t e = K            ? dd_fft_launch(K, (int)p->L, (int)p->P, a, s0, s1, st)
                    : p->fx_panel ? fxp_fft_launch((int)p->L, (int)p->P, a, s0, s1, st)
                                  : fx_fft_launch((int)p->L, (int)p->P, a, s0, s1, st);
@@ -762,7 +783,7 @@ int ensure_stage(mpfft_plan_s *p)
 {
     const size_t bytes = (is_limb_form(p->form) ? sizeof(uint32_t) : sizeof(double)) *
                          p->words * p->batch;
-    if (p->d_in.bytes != bytes) {
+    if (!p->d_in.holds(bytes) || !p->d_out.holds(bytes)) {
         int rc;
         if ((rc = p->d_in.alloc(bytes)) || (rc = p->d_out.alloc(bytes))) return rc;
     }
@@ -815,6 +836,23 @@ int legacy_d(unsigned long n, unsigned long s, long sn, const double *inr, const
     memcpy(outr, hout.data(), sizeof(double) * n);
     memcpy(outi, hout.data() + n, sizeof(double) * n);
     return MPFFT_OK;
+}
+
+// The reference entry points return void and their callers read no status
+// (mpFFT.c:84-91), so a failure would leave the outputs unwritten without a
+// sign: every failure is recorded (mpfft_last_error) and printed to stderr,
+// and a fatal one (a HIP error or no usable GPU) aborts the process.
+// MPFFT_NO_ABORT=1 in the environment turns the abort into the record and
+// the message only, for callers that check mpfft_last_error themselves.
+void legacy_report(const char *name, int rc)
+{
+    g_last_error = rc;
+    if (rc == MPFFT_OK) return;
+    fprintf(stderr, "mpfft: %s: %s (%d); outputs not written\n", name, mpfft_strerror(rc), rc);
+    if (rc == MPFFT_ERR_HIP || rc == MPFFT_ERR_NODEV) {
+        const char *e = getenv("MPFFT_NO_ABORT");
+        if (!(e && e[0] == '1')) abort();
+    }
 }
 
 }  // namespace
@@ -950,7 +988,7 @@ int mpfft_roundtrip_stats(mpfft_plan p, const uint32_t *d_x, const uint32_t *d_y
     hipStream_t st = stream ? (hipStream_t)stream : p->stream;
     const uint32_t nblk = (uint32_t)((2 * p->N + kErrBlock - 1) / kErrBlock);
     const size_t nd = (size_t)nblk * p->batch * kErrPartial;
-    if (p->errbuf.bytes != nd * sizeof(double)) {
+    if (!p->errbuf.holds(nd * sizeof(double))) {
         int rc = p->errbuf.alloc(nd * sizeof(double));
         if (rc) return rc;
     }
@@ -973,7 +1011,7 @@ int mpfft_roundtrip_host(mpfft_plan p, const uint32_t *h_x, double *out)
         std::lock_guard<std::mutex> g(p->mu);
         HIP_TRY(hipSetDevice(p->device));
         if ((rc = ensure_stage(p))) return rc;
-        if (p->d_aux.bytes != p->d_in.bytes && (rc = p->d_aux.alloc(p->d_in.bytes))) return rc;
+        if (!p->d_aux.holds(p->d_in.bytes) && (rc = p->d_aux.alloc(p->d_in.bytes))) return rc;
         HIP_TRY(hipMemcpyAsync(p->d_in.p, h_x, p->d_in.bytes, hipMemcpyHostToDevice, p->stream));
         if ((rc = run_mp(p, p->d_in.as<uint32_t>(), p->d_out.as<uint32_t>(), p->stream)))
             return rc;
@@ -1209,25 +1247,27 @@ int mpfft_mp_binop_host(int op, unsigned long count, long prec, const uint32_t *
 void mp_sr24_cp_fft(unsigned long n, unsigned long s, long int sn, mpfr_t *inr, mpfr_t *ini,
                     mpfr_t *outr, mpfr_t *outi)
 {
-    g_last_error = legacy_mp(n, s, sn, inr, ini, outr, outi);
+    legacy_report("mp_sr24_cp_fft", legacy_mp(n, s, sn, inr, ini, outr, outi));
 }
 
 void mp_sr24_cp_fft_v2(unsigned long n, unsigned long log2n, unsigned long s, long int sn,
                        mpfr_t *inr, mpfr_t *ini, mpfr_t *outr, mpfr_t *outi)
 {
-    g_last_error = log2n != 0 ? MPFFT_ERR_SUBTREE : legacy_mp(n, s, sn, inr, ini, outr, outi);
+    legacy_report("mp_sr24_cp_fft_v2", log2n != 0 ? MPFFT_ERR_SUBTREE
+                                                  : legacy_mp(n, s, sn, inr, ini, outr, outi));
 }
 
 void d_sr24_cp_fft(unsigned long n, unsigned long s, long int sn, double *inr, double *ini,
                    double *outr, double *outi)
 {
-    g_last_error = legacy_d(n, s, sn, inr, ini, outr, outi);
+    legacy_report("d_sr24_cp_fft", legacy_d(n, s, sn, inr, ini, outr, outi));
 }
 
 void d_sr24_cp_fft_v2(unsigned long n, unsigned long log2n, unsigned long s, long int sn,
                       double *inr, double *ini, double *outr, double *outi)
 {
-    g_last_error = log2n != 0 ? MPFFT_ERR_SUBTREE : legacy_d(n, s, sn, inr, ini, outr, outi);
+    legacy_report("d_sr24_cp_fft_v2", log2n != 0 ? MPFFT_ERR_SUBTREE
+                                                 : legacy_d(n, s, sn, inr, ini, outr, outi));
 }
 
 }  // extern "C"

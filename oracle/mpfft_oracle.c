@@ -509,6 +509,44 @@ int orc_err_stats(unsigned long N, long P, long Lx, long Ls,
     return rc ? -2 : 0;
 }
 
+/* Public: componentwise error beyond correct rounding, for the forms whose
+ * contract is "the exact transform within an absolute error A, then rounded
+ * once to P bits": |X - X*| <= ulp_P(X)/2 + A for every component.
+ * out[0] = max over components of max(0, |X - X*| - ulp_P(X)/2) / 2^ea
+ * (ulp_P(0) = 0), i.e. the smallest A/2^ea the result satisfies;
+ * out[1] = number of components with |X - X*| > ulp_P(X)/2.
+ * X at P bits (Lx limbs), X* at higher precision (Ls limbs). */
+int orc_err_excess(unsigned long N, long P, long Lx, long Ls,
+                   const int32_t *hx, const uint32_t *lx,
+                   const int32_t *hs, const uint32_t *ls, long ea, double *out)
+{
+    long M = 2 * (long)N, Pbig = 32 * (Lx > Ls ? Lx : Ls) + 64;
+    mpfr_t x, s, d, h;
+    mpfr_inits2(Pbig, x, s, d, h, (mpfr_ptr)0);
+    double worst = 0.0, over = 0.0;
+    int rc = 0;
+    for (long i = 0; i < M; i++) {
+        rc |= soa_get(x, hx, lx, Lx, M, i);
+        rc |= soa_get(s, hs, ls, Ls, M, i);
+        mpfr_sub(d, x, s, RN);          /* exact at Pbig */
+        mpfr_abs(d, d, RN);
+        if (!mpfr_zero_p(x)) {
+            mpfr_set_ui_2exp(h, 1, mpfr_get_exp(x) - P - 1, RN);   /* ulp_P(X)/2 */
+            mpfr_sub(d, d, h, RN);
+        }
+        if (mpfr_sgn(d) > 0) {
+            over += 1;
+            mpfr_mul_2si(d, d, -ea, RN);
+            double u = mpfr_get_d(d, RN);
+            if (u > worst) worst = u;
+        }
+    }
+    out[0] = worst;
+    out[1] = over;
+    mpfr_clears(x, s, d, h, (mpfr_ptr)0);
+    return rc ? -2 : 0;
+}
+
 /* Public: the reference's round-trip accuracy check (test(),
  * serial_mp_fft3.c:346-427).  Given the input x and the forward result X
  * (both SoA at P bits), run the forward FFT on swap(X) (re<->im), divide

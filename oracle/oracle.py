@@ -38,6 +38,7 @@ def lib():
         L.orc_mp_binop.argtypes = [I, Lg, Lg, P, P, P, P, P, P]
         L.orc_mp_input_in2.argtypes = [U, Lg, P, P]
         L.orc_err_stats.argtypes = [U, Lg, Lg, Lg, P, P, P, P, P]
+        L.orc_err_excess.argtypes = [U, Lg, Lg, Lg, P, P, P, P, Lg, P]
         L.orc_mp_roundtrip.argtypes = [U, Lg, P, P, P, P, P]
         L.orc_mp_counts.argtypes = [U, P]
         L.orc_d_lut.argtypes = [U, P, P, P]
@@ -121,6 +122,81 @@ def err_stats(n, P, x, Lx, star, Ls):
     if rc:
         raise RuntimeError(f"orc_err_stats rc={rc}")
     return float(out[0]), float(out[1]), int(out[2])
+
+
+def err_excess(n, P, x, Lx, star, Ls, ea):
+    """(max over components of (|x - star| - ulp_P(x)/2)^+ / 2^ea, number of
+    components not within half an ulp of star): the absolute error, in units
+    of 2^ea, beyond one correct rounding to P bits."""
+    x, hx, lx = _split(x, n, Lx)
+    s, hs, ls = _split(star, n, Ls)
+    out = np.zeros(2, dtype=np.float64)
+    rc = lib().orc_err_excess(n, P, Lx, Ls, _p(hx), _p(lx), _p(hs), _p(ls), int(ea), _p(out))
+    if rc:
+        raise RuntimeError(f"orc_err_excess rc={rc}")
+    return float(out[0]), int(out[1])
+
+
+def max_exponent(x, n):
+    """largest exponent of the 2n reals of a SoA transform (None if all are
+    zero): the block exponent of the fixed and word forms."""
+    h = np.ascontiguousarray(x[:2 * n]).view(np.int32) >> 1
+    m = int(h.max())
+    return None if m == -(1 << 30) else m
+
+
+def fixed_bound(n):
+    """The fixed form's absolute error bound before its one rounding to P
+    bits, in units of 2^(emax - F): 2 (log2 n + 2) n, emax = the largest
+    input exponent, F = 32 (L + 1) the fraction bits at L device limbs.
+    Derivation, in max-norm over complex values with u = 2^-F: after the
+    block exponent is taken off |x| < sqrt(2); the input conversion
+    truncates (error < sqrt(2) u); sums are exact; a pass of radix R
+    multiplies the errors before it by at most R (every output sums R
+    inputs with unit coefficients); before a pass whose inputs are below
+    sqrt(2) S (S: product of the radices so far) a twiddle product adds at
+    most 3 u of truncation (two real products of <= 1.04 u per part) and
+    2.42 S u from the twiddle table's rounding to F bits (per-lane Gauss
+    tables; sqrt(2) S u for the panels' c, d tables), and a radix-8 pass's
+    internal products by 1/sqrt(2) add 6 u + 8 S u after its radix-2
+    stage.  Summed: below n (1.21 log2 n + 1) for radix 4 and
+    n (0.81 log2 n + 2) for radix 8, within 2 (log2 n + 2) n."""
+    lg = max(int(n).bit_length() - 1, 0)
+    return float(2 * (lg + 2) * n)
+
+
+def fixed_check(n, P, L_dev, x, y, star=None, Ly=None):
+    """Check a fixed-form result y (at Ly limbs, default ceil(P/32)) of input
+    x (ceil(P/32) limbs) against its contract: every component within half
+    an ulp of y plus fixed_bound(n) units of 2^(emax - F) of the exact
+    transform (the 2P-bit oracle), and normwise within 2^-P (one rounding)
+    plus that bound's share.  Returns (excess in units of 2^(emax - F), max
+    ulp, normwise in 2^-P, components beyond half an ulp)."""
+    Lo, L2 = limbs(P), limbs(2 * P)
+    Ly = Lo if Ly is None else Ly
+    if star is None:
+        star = mp_fft(n, P, x, Pw=2 * P)
+    F = 32 * (L_dev + 1)
+    em = max_exponent(x, n)
+    em = 0 if em is None else em
+    exc, over = err_excess(n, P, y, Ly, star, L2, em - F)
+    mu, nw, _ = err_stats(n, P, y, Ly, star, L2)
+    lg = max(int(n).bit_length() - 1, 0)
+    delta = fixed_bound(n) * 2.0 ** (P - F + 1.5)
+    assert exc <= fixed_bound(n), (n, P, exc, fixed_bound(n))
+    assert nw <= 1.0 + delta, (n, P, nw, delta)
+    return exc, mu, nw, over
+
+
+def scale_pow2(x, n, s):
+    """x 2^s for a SoA transform (exact: the exponent of every nonzero real
+    moves by s)."""
+    y = np.array(x, dtype=np.uint32, copy=True)
+    h = y[:2 * n].view(np.int32)
+    nz = h != np.int32(-2**31)
+    nz &= h != np.int32(-2**31 + 1)
+    h[nz] += np.int32(2 * s)
+    return y
 
 
 def roundtrip(n, P, x, X):

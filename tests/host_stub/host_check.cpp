@@ -10,6 +10,8 @@
 #include <cstdlib>
 #include <vector>
 
+void stub_fail_malloc_in(int k);      // stub.cpp
+
 static int fails = 0;
 #define CHECK(c, ...)                                   \
     do {                                                \
@@ -103,9 +105,52 @@ static void batch(int form, std::vector<unsigned long> n, std::vector<long> P, i
     mpfft_batch_destroy(b);
 }
 
-int main()
+// A staging allocation that fails (d_in allocated, d_out not) and a retry:
+// the retry allocates again instead of taking the failed buffer for a
+// usable one (the stand-in kernels would write through a null pointer).
+static void alloc_failure_then_retry()
 {
+    mpfft_plan p;
+    CHECK(mpfft_plan_create(&p, 64, 256, MPFFT_FORM_MP, 2) == 0, "plan");
+    long L;
+    size_t w;
+    mpfft_plan_info(p, &L, &w);
+    std::vector<uint32_t> x(2 * w, 0), y(2 * w, 0);
+    stub_fail_malloc_in(1);                       // d_in succeeds, d_out fails
+    CHECK(mpfft_execute_limbs_host(p, x.data(), y.data()) == MPFFT_ERR_HIP, "failed alloc");
+    CHECK(mpfft_execute_limbs_host(p, x.data(), y.data()) == MPFFT_OK, "retry");
+    stub_fail_malloc_in(0);                       // a plan whose first buffer fails
+    mpfft_plan q = nullptr;
+    CHECK(mpfft_plan_create(&q, 64, 256, MPFFT_FORM_MP, 3) == MPFFT_ERR_HIP, "plan alloc");
+    stub_fail_malloc_in(-1);
+    CHECK(mpfft_execute_limbs_host(p, x.data(), y.data()) == MPFFT_OK, "after");
+    mpfft_plan_destroy(p);
+}
+
+// The reference entry points on failure (argv[1] selects the case; the test
+// runs each in its own process): a precision mismatch is recorded and printed,
+// a HIP error aborts unless MPFFT_NO_ABORT=1.
+static int legacy(const char *what)
+{
+    const unsigned long n = 16;
+    mpfr_t a[4][16];
+    for (int k = 0; k < 4; k++)
+        for (unsigned long i = 0; i < n; i++) {
+            mpfr_init2(a[k][i], (k == 3 && what[0] == 'p') ? 128 : 256);
+            mpfr_set_ui(a[k][i], (unsigned long)i, MPFR_RNDN);
+        }
+    mpfr_t *r = a[0], *im = a[1], *o = a[2], *oi = a[3];
+    if (what[0] == 'h') stub_fail_malloc_in(0);   // the first device allocation fails
+    mp_sr24_cp_fft(n, 1, 0, r, im, o, oi);
+    printf("last_error %d\n", mpfft_last_error());
+    return 0;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc > 1) return legacy(argv[1]);
     plans();
+    alloc_failure_then_retry();
     batch(MPFFT_FORM_MP, {1, 2, 4, 8, 16, 64, 512, 512, 64, 32, 256, 128, 8, 512},
           {64, 100, 128, 256, 53, 128, 256, 256, 1024, 200, 64, 512, 128, 100}, 0);
     srand(11);

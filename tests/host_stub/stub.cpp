@@ -19,7 +19,12 @@ struct hipGraphExec { std::vector<std::function<void()>> ops; };
 static thread_local hipStream_t g_capture = nullptr;
 static std::vector<std::function<void()>> *cur_cap() { return g_capture ? g_capture->cap : nullptr; }
 static void run_or_record(std::function<void()> f) { if (auto c = cur_cap()) c->push_back(f); else f(); }
-hipError_t hipMalloc(void **p, size_t b) { *p = malloc(b ? b : 1); return hipSuccess; }
+// failure injection: the k-th hipMalloc from now fails (k = 0: the next one)
+static int g_malloc_fail_in = -1;
+void stub_fail_malloc_in(int k) { g_malloc_fail_in = k; }
+hipError_t hipMalloc(void **p, size_t b) {
+    if (g_malloc_fail_in >= 0 && g_malloc_fail_in-- == 0) { *p = nullptr; return hipErrorUnknown; }
+    *p = malloc(b ? b : 1); return hipSuccess; }
 hipError_t hipFree(void *p) { free(p); return hipSuccess; }
 hipError_t hipHostMalloc(void **p, size_t b, unsigned) { *p = malloc(b ? b : 1); return hipSuccess; }
 hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
@@ -76,6 +81,16 @@ static void fx_touch(int L, const FxArgs &a, bool panel) {
     for (uint32_t b = 0; b < a.batch; b++) { touch_r(a.in + a.in_stride * b, w); touch_w(a.out + a.out_stride * b, w); }
     const size_t fw = 2 * (size_t)a.N * (L + 2) * (panel ? fxp_groups(a.batch) * kPanel : a.batch);
     touch_w(a.buf0, fw); touch_w(a.buf1, fw); touch_r(a.tw, 1); touch_w(a.flag, 1);
+}
+hipError_t fx_eoff_launch(const FxArgs &a, hipStream_t) {
+    if (a.eoff_nblk != fx_eoff_blocks(a.N)) return hipErrorInvalidValue;
+    FxArgs c = a;
+    run_or_record([=] {
+        for (uint32_t b = 0; b < c.batch; b++) touch_r(c.in + c.in_stride * b, 2 * (size_t)c.N);
+        touch_w((uint32_t *)c.eoff_part, (size_t)c.batch * c.eoff_nblk);
+        touch_w((uint32_t *)c.eoff, c.batch); touch_w(c.flag, 1);
+    });
+    return hipSuccess;
 }
 hipError_t fx_fft_launch(int L, int, const FxArgs &a, int, int, hipStream_t) { FxArgs c = a; run_or_record([=] { fx_touch(L, c, false); }); return hipSuccess; }
 hipError_t fxp_fft_launch(int L, int, const FxArgs &a, int, int, hipStream_t) {

@@ -2,10 +2,13 @@
 fixed-point reals), through the C ABI.
 
 This form does not reproduce the reference's bit pattern; its contract is
-accuracy: against the 2P-bit oracle its error must be no worse than the
-plain form's (which is the reference's, bit for bit), both componentwise
-(max ulp) and normwise.  A wrong index or twiddle anywhere would show as an
-O(1) error.
+accuracy (oracle.fixed_check): every component within half an ulp of the
+result plus 2 (log2 n + 2) n units of 2^(emax - F) of the exact transform
+(the 2P-bit oracle), F = 32 (L + 1) >= P + 32, emax the largest input
+exponent; at P = 1024, n = 2^16 that absolute part is ~2^-20 of an ulp of a
+typical output.  The plain form (the reference's order, bit for bit) is
+10^5 ulp away at large n, so the bar is the form's own, not the plain
+form's.  A wrong index or twiddle anywhere would show as an O(1) error.
 """
 import numpy as np
 import pytest
@@ -33,16 +36,13 @@ def errors(n, P, y, star):
 @pytest.mark.parametrize("n,P", [(1, 128), (2, 128), (4, 256), (8, 256), (16, 1024), (32, 64),
                                  (64, 1024), (128, 100), (1024, 256), (2048, 1024),
                                  (4096, 53), (8192, 512)])
-def test_fixed_error_no_worse_than_plain(n, P):
+def test_fixed_error_within_bound(n, P):
     x = I.mp_random(n, P, seed=1000 + n + P)
     star = O.mp_fft(n, P, x, Pw=2 * P)
-    plain = O.mp_fft(n, P, x)                    # = the GPU plain form, bit for bit
     fixed = run_fixed(n, P, x)
-    mu_f, nw_f, zb_f = errors(n, P, fixed, star)
-    mu_p, nw_p, zb_p = errors(n, P, plain, star)
+    _, _, zb_f = errors(n, P, fixed, star)
     assert zb_f == 0
-    assert nw_f <= max(nw_p, 0.5), (nw_f, nw_p)
-    assert mu_f <= max(mu_p, 1.0), (mu_f, mu_p)
+    O.fixed_check(n, P, M.limbs_for_precision(P), x, fixed, star)
 
 
 @pytest.mark.parametrize("n,P,b", [(1, 2048, 1), (4, 1500, 2), (16, 2048, 1), (64, 1056, 3),
@@ -59,10 +59,9 @@ def test_fixed_lane_wide(n, P, b):
     got = np.split(run_fixed(n, P, np.concatenate(xs), batch=b, form=M.FORM_FIXED_LANE), b)
     for x, y in zip(xs, got):
         star = O.mp_fft(n, P, x, Pw=2 * P)
-        mu_f, nw_f, zb_f = errors(n, P, y, star)
-        mu_p, nw_p, _ = errors(n, P, O.mp_fft(n, P, x), star)
+        _, _, zb_f = errors(n, P, y, star)
         assert zb_f == 0
-        assert nw_f <= max(nw_p, 0.5) and mu_f <= max(mu_p, 1.0), (mu_f, nw_f, mu_p, nw_p)
+        O.fixed_check(n, P, 64, x, y, star)
     if n >= 16:
         yp = np.split(run_fixed(n, P, np.concatenate(xs), batch=b, form=M.FORM_FIXED_PANEL), b)
         for y, z in zip(got, yp):
@@ -105,6 +104,7 @@ def test_fixed_full_size_1024_bits():
     mu_f, nw_f, _ = errors(n, P, fixed, star)
     mu_p, nw_p, _ = errors(n, P, plain, star)
     assert nw_f <= nw_p and mu_f <= mu_p, (mu_f, nw_f, mu_p, nw_p)
+    O.fixed_check(n, P, 32, x, fixed, star)
 
 
 def test_fixed_batch():
@@ -116,10 +116,12 @@ def test_fixed_batch():
 
 
 def test_fixed_input_range_checked():
+    # the limb format's exponent range: |exponent| < 2^29 (a real 2^40 times
+    # the others is accepted since the block exponent follows the largest)
     n, P = 1024, 256
     x = I.mp_random(n, P, seed=5)
     h, _ = I.split(x, n, O.limbs(P))
-    h[17] = np.int32((40 << 1) | (h[17] & 1))    # |x| ~ 2^40, beyond 2^(29 - 10)
+    h[17] = np.int32(((1 << 29) << 1) | (h[17] & 1))
     with pytest.raises(M.MpfftError) as e:
         run_fixed(n, P, x)
     assert e.value.code == -3
