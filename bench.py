@@ -210,20 +210,29 @@ def measure_precision_form(M, I, torch, O, n, P, Pcmp, batch, steps, seed):
     return out
 
 
-def measure_wide(M, I, torch, lgn, steps, seed):
+def measure_wide(M, I, torch, O, lgn, steps, seed, err=True):
     """N = 2^lgn at 512 bits (batch 64) and at 2048 bits (batch 64 and
-    batch 1): the fixed form (on panels) against the plain form (the
-    reference's op order, 64-limb kernels at 2048 bits).  One generated
-    transform is copied into every slot of the batch on the device (the
-    kernels' time does not depend on the values)."""
+    batch 1): the fixed form (on panels at batch 64, on lanes at batch 1)
+    against the plain form (the reference's op order, 64-limb kernels at
+    2048 bits).  One generated transform is copied into every slot of the
+    batch on the device (the kernels' time does not depend on the values);
+    transform 0 of every run is checked against the 2P-bit oracle (max ulp,
+    normwise, and for the fixed form its own bound: half an ulp plus
+    2 (log2 n + 2) n units of 2^(emax - F))."""
     n = 1 << lgn
     out = {"workload": f"N=2^{lgn}: 512 bits batch 64, 2048 bits batch 64 and batch 1; "
-                       "fixed (panels) and plain"}
+                       "fixed (panels at batch 64, lanes at batch 1) and plain"}
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     both = (("fixed", M.FORM_FIXED), ("plain", M.FORM_MP))
+    stars = {}
+    t_err = 0.0
     for P, B in ((512, 64), (2048, 64), (2048, 1)):
         x1 = I.mp_random(n, P, seed=seed)
+        if err and P not in stars:
+            t0 = time.time()
+            stars = {P: O.mp_fft(n, P, x1, Pw=2 * P)}
+            t_err += time.time() - t0
         for name, form in both:
             plan = M.Plan(n, P, form, B)
             one = torch.from_numpy(I.relimb(x1, n, I.limbs_for(P), plan.L).view(np.int32)).cuda()
@@ -243,13 +252,48 @@ def measure_wide(M, I, torch, lgn, steps, seed):
             key = f"{name}_P{P}" + ("" if B == 64 else f"_batch{B}")
             out[key] = {"batch": B, "ms_per_step": round(ms, 3),
                         "fft_per_s": round(B / ms * 1e3, 2)}
+            if err:
+                t0 = time.time()
+                y0 = I.relimb(y[: plan.words].cpu().numpy().view(np.uint32), n, plan.L,
+                              I.limbs_for(P))
+                mu, nw, zb = O.err_stats(n, P, y0, I.limbs_for(P), stars[P],
+                                         I.limbs_for(2 * P))
+                e = {"max_ulp": mu, "normwise_2^-P": nw, "zero_mismatch": zb}
+                if form == M.FORM_FIXED:
+                    try:
+                        exc = O.fixed_check(n, P, plan.L, x1, y0, stars[P])[0]
+                        e["within_fixed_bound"], e["excess_units"] = True, exc
+                    except AssertionError:
+                        e["within_fixed_bound"] = False
+                out[key]["error_vs_2P_oracle"] = e
+                t_err += time.time() - t0
             plan.close()
             del x, y
             torch.cuda.empty_cache()
     for suf in ("P512", "P2048", "P2048_batch1"):
         out[f"fixed_vs_plain_{suf}"] = round(out[f"fixed_{suf}"]["fft_per_s"] /
                                              out[f"plain_{suf}"]["fft_per_s"], 3)
+    if err:
+        out["error_seconds"] = round(t_err, 1)
     return out
+
+
+def valu_probe():
+    """measured VALU issue rates on this GPU (mpfft_amd/valu_probe, built by
+    python -m mpfft_amd.build from mpfft_amd/csrc/valu_probe.hip): lane-
+    operations per second of v_mad_u64_u32 and the other forms it times"""
+    import subprocess
+    exe = os.path.join(ROOT, "mpfft_amd", "valu_probe")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        raise RuntimeError(f"valu_probe rc={r.returncode}: {r.stderr[-300:]}")
+    rates = {}
+    for line in r.stdout.splitlines():
+        if "Tops/s" in line:
+            name, val = line.split("Tops/s")[0].rsplit(None, 1)
+            rates[name.strip()] = float(val)
+    return {"unit": "T lane-ops/s", "rates": rates,
+            "full_rate_lane_ops_published": round(VALU_LANE_OPS / 1e12, 2)}
 
 
 def device_copy_rate(torch, mib=2048, reps=5):
@@ -407,6 +451,10 @@ def main():
                     help="precision of the double-word form leg (0 = skip; the quad-word "
                          "form runs at twice it); compared with the plain and fixed forms "
                          "at 128 (256) bits")
+    ap.add_argument("--dd-n2", type=int, default=1 << 20,
+                    help="size of the second precision-form leg (0 = skip)")
+    ap.add_argument("--dd-n2-batch", type=int, default=16,
+                    help="batch of the second precision-form leg")
     ap.add_argument("--wide-lgn", type=int, default=20,
                     help="log2 N of the wide-precision leg (512 and 2048 bits; 0 = skip)")
     ap.add_argument("--batched-count", type=int, default=4096,
@@ -576,8 +624,20 @@ def main():
         side_leg(line, "precision_form",
                  lambda: measure_precision_form(M, I, torch, O, n, a.dd_prec, 128, B,
                                                 max(2, a.steps // 2), a.seed))
+    if a.dd_prec and a.dd_n2:
+        side_leg(line, "precision_form_N2",
+                 lambda: measure_precision_form(M, I, torch, O, a.dd_n2, a.dd_prec, 128,
+                                                a.dd_n2_batch, 2, a.seed))
     if a.wide_lgn:
-        side_leg(line, "wide", lambda: measure_wide(M, I, torch, a.wide_lgn, 2, a.seed))
+        side_leg(line, "wide", lambda: measure_wide(M, I, torch, O, a.wide_lgn, 2, a.seed,
+                                                    err=not a.no_err))
+    side_leg(line, "valu_probe", valu_probe)
+    if line["roofline"]["bound"] == "valu" and "rates" in line.get("valu_probe", {}):
+        mad = line["valu_probe"]["rates"].get("v_mad_u64_u32 (C)")
+        if mad:
+            line["roofline"]["measured_mad_u64_rate"] = mad
+            line["roofline"]["frac_of_measured_mad_u64"] = round(
+                line["roofline"]["achieved"] / mad, 4)
     if a.batched_count:
         side_leg(line, "batched", lambda: measure_batched(M, I, torch, a.seed, a.batched_count,
                                                           max(2, a.steps // 2)))
@@ -603,25 +663,49 @@ def side_leg(line, key, fn):
         print(f"bench: {key or 'cpu'} leg failed: {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """the host CPU: model name, logical CPUs, CPUs usable by this process,
+    and the OpenMP thread count of the port leg (OMP_NUM_THREADS if set,
+    else the usable CPUs, at most 16: one GPU's share of a GPU box)"""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else \
+        (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    omp = int(env) if env.isdigit() and int(env) > 0 else min(usable, 16)
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable,
+            "omp_threads": omp}
+
+
 def cpu_legs(O, I, n, P, seed):
     """bounded CPU baselines; returns the line's cpu_* keys"""
     line = {}
     # bounded sample of 3 repeats (the reference's M >= 3): the reference's
     # OpenMP CODE 2 (4 threads, hard-coded at parallel_mp_fft2.c:262) when
-    # its build is present, and the oracle restatement on every host core
-    # and on 4
-    nthr = min(os.cpu_count() or 1, 16)
+    # its build is present, and the oracle restatement on the host's CPU
+    # share and on 4 threads.  The host: CPU model, logical CPUs, and the
+    # CPUs this process may run on (a GPU box gives one GPU's work a share
+    # of 16; OMP_NUM_THREADS states it)
+    host = host_cpu()
+    nthr = host["omp_threads"]
     xs = I.mp_random(n, P, seed=seed)
     cpu = {}
     if O.have_ref():
         t = O.ref_mp_time("mpp2", n, P, xs, 3)
         cpu = {"value": round(1.0 / t, 4), "unit": "FFT/s", "cores": 4, "kind": "reference",
-               "sample": f"3 FFTs N={n} P={P}, reference OpenMP CODE 2 (4 threads), "
-                         "LUT and conversion excluded"}
+               "sample": f"3 FFTs N={n} P={P}, reference OpenMP CODE 2 (4 threads, as the "
+                         "reference hard-codes), LUT and conversion excluded",
+               "host": host}
     t = O.mp_fft_time(n, P, xs, 3, nthr)
     port = {"value": round(1.0 / t, 4), "unit": "FFT/s", "cores": nthr, "kind": "port",
             "sample": f"3 FFTs N={n} P={P}, oracle restatement, {nthr} OpenMP threads, "
-                      "LUT and conversion excluded"}
+                      "LUT and conversion excluded", "host": host}
     t4 = O.mp_fft_time(n, P, xs, 3, 4)
     port4 = dict(port, value=round(1.0 / t4, 4), cores=4,
                  sample=f"3 FFTs N={n} P={P}, oracle restatement, 4 OpenMP threads, "
