@@ -263,25 +263,35 @@ __device__ __forceinline__ void mf_operand(MfOperand<W> &B, const Fx<W> &re, con
 // (re, im) *= (c + i d) for the 64 operands of the wave (one per lane), given
 // as B operands; the wave's twiddle reals c, d, -d are digit tables in LDS.
 // REAL: (re, im) *= c (a real twiddle: one product per part, Rd and Rnd unused).
+// TWO: the operands of lanes 32..63 take another twiddle, tables Rc2, Rd2,
+// Rnd2 (the MFMAs of the two 32-lane groups, D0 and D1, get their own A
+// tiles; the B operands are laid out as before).
 // Results go to the wave's LDS slot: word j of re' at slot[j 64 + lane], of
 // im' at slot[(W + j) 64 + lane].  The twiddle tables may lie in the im' part
 // of the slot: all of them are read before the first word of im' is written.
-template <int W, bool REAL = false>
+template <int W, bool REAL = false, bool TWO = false>
 __device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *Rc,
                                         const uint32_t *Rd, const uint32_t *Rnd, uint32_t lane,
-                                        uint32_t *slot)
+                                        uint32_t *slot, const uint32_t *Rc2 = nullptr,
+                                        const uint32_t *Rd2 = nullptr,
+                                        const uint32_t *Rnd2 = nullptr)
 {
+    static_assert(!(REAL && TWO), "one real twiddle per product");
     constexpr MfGeom G = mf_geom(W);
     constexpr int KT = G.KT, MT = G.MT, ND = G.DHI - G.DLO + 1;
     const uint32_t m = lane & 31u, g = lane >> 5;
     // byte offset of diagonal delta: U - (C0 + 32 delta + m - 16 g)
     const uint32_t off0 = (uint32_t)(G.U - G.C0 - 32 * G.DLO) - m + 16 * g;
-#pragma unroll
+    // the two parts one after the other (rolled: overlapping them holds two
+    // sets of accumulators and carries)
+#pragma unroll 1
     for (int h = 0; h < 2; h++) {
         // h = 0: re' = c re + (-d) im;  h = 1: im' = d re + c im
         // (REAL: h = 0: re' = c re;  h = 1: im' = c im)
         const uint32_t *R1 = REAL ? Rc : (h ? Rd : Rc);
         const uint32_t *R2 = h ? Rc : Rnd;
+        const uint32_t *R1b = TWO ? (h ? Rd2 : Rc2) : R1;
+        const uint32_t *R2b = TWO ? (h ? Rc2 : Rnd2) : R2;
 #if FXP_ACACHE
         v4i A1[ND], A2[ND];
 #pragma unroll
@@ -315,10 +325,12 @@ __device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *R
                     D1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, h ? B.i1[q] : B.r1[q], D1, 0, 0, 0);
                     continue;
                 }
+                const v4i b1 = TWO ? mf_atile(R1b, off0 - 32 * dd) : a1;
+                const v4i b2 = TWO ? mf_atile(R2b, off0 - 32 * dd) : a2;
                 D0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, B.r0[q], D0, 0, 0, 0);
-                D1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, B.r1[q], D1, 0, 0, 0);
+                D1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(b1, B.r1[q], D1, 0, 0, 0);
                 D0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a2, B.i0[q], D0, 0, 0, 0);
-                D1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a2, B.i1[q], D1, 0, 0, 0);
+                D1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(b2, B.i1[q], D1, 0, 0, 0);
             }
             // D0 lane l: rows 8 (v/4) + 4 (l/32) + v%4 of operand l % 32,
             // D1 the same of operand l % 32 + 32.  After one swap per v the
@@ -683,6 +695,239 @@ fxp_r8_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_
     }
 }
 
+// Output R of the DFT4 over four LDS operands, R chosen per lane at run time
+// (the two 32-lane halves of a wave may compute different outputs): the
+// same sums as dft4_out, subtraction as addition of the complement.  v_s
+// points at the lane's word 0 of operand s (word i at i 64, imaginary words
+// at (W + i) 64).
+template <int W>
+__device__ __forceinline__ void dft4_rt(Fx<W> &yr, Fx<W> &yi, const uint32_t *v0,
+                                        const uint32_t *v1, const uint32_t *v2,
+                                        const uint32_t *v3, uint32_t R)
+{
+    const uint32_t odd = R & 1u;
+    const uint32_t mo = 0u - odd;                                  // a = v0 -+ v2, c = v1 -+ v3
+    const uint32_t sre = (R == 2u || R == 3u) ? 1u : 0u;           // yr = ar -+ (odd ? ci : cr)
+    const uint32_t sim = (R == 1u || R == 2u) ? 1u : 0u;           // yi = ai -+ (odd ? cr : ci)
+    const uint32_t mre = 0u - sre, mim = 0u - sim;
+    const uint32_t cx = odd ? (uint32_t)W * 64u : 0u;              // part of c added into yr
+    const uint32_t cy = odd ? 0u : (uint32_t)W * 64u;              // part of c added into yi
+    uint32_t ca_r = odd, ca_i = odd, cc_x = odd, cc_y = odd, cy_r = sre, cy_i = sim;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        const uint32_t o = (uint32_t)i * 64u, oi = (uint32_t)(W + i) * 64u;
+        const uint32_t ar = add_cc(v0[o], v2[o] ^ mo, ca_r);
+        const uint32_t ai = add_cc(v0[oi], v2[oi] ^ mo, ca_i);
+        const uint32_t c1 = add_cc(v1[cx + o], v3[cx + o] ^ mo, cc_x);
+        const uint32_t c2 = add_cc(v1[cy + o], v3[cy + o] ^ mo, cc_y);
+        yr.w[i] = add_cc(ar, c1 ^ mre, cy_r);
+        yi.w[i] = add_cc(ai, c2 ^ mim, cy_i);
+    }
+}
+
+// Radix-16 pass layout.  Lane-half h = 2 w + (lane >> 5) of wave w (eight
+// waves) holds input position r = w + 8 (lane >> 5) of 32 transforms, and
+// the DFT16 runs as 4 x 4 (r = r2 + 4 r1, q = m1 + 4 m2):
+//   stage A:  Y_(r2, m1) = sum_r1 v_(r2 + 4 r1) (-i)^(r1 m1)       (DFT4)
+//   internal: Y'_(r2, m1) = w16^(r2 m1) Y_(r2, m1)
+//   stage B:  y_(m1 + 4 m2) = sum_r2 Y'_(r2, m1) (-i)^(r2 m2)      (DFT4)
+// Lane-half h computes Y_(kR2[h], kM1[h]) in stage A and y_q with m1 =
+// kM1[h], m2 = kR2[h] in stage B.  The pairs are placed so that the eight
+// nontrivial internal twiddles fall in waves 0..3, both halves of a wave
+// under one twiddle (one matrix operand A per tile, as in the radix-4
+// pass): wave 0 w16^3 (pairs (1,3), (3,1)), wave 1 w16^1 (pairs (1,1) and
+// (3,3), whose twiddle w16^9 = -w16^1: the upper half's operand is negated
+// first), wave 2 w16^2 ((1,2), (2,1)), wave 3 w16^6 ((2,3), (3,2)); the
+// trivial ones in waves 4..7 (1, and -i in wave 7's upper half).
+__constant__ const uint8_t kR2[16] = {1, 3, 1, 3, 1, 2, 2, 3, 0, 0, 0, 0, 1, 2, 3, 2};
+__constant__ const uint8_t kM1[16] = {3, 1, 1, 3, 2, 1, 3, 2, 0, 1, 2, 3, 0, 0, 0, 2};
+
+// internal twiddle table of wave w < 4, an index into the tables w16^1,
+// w16^3, w16^9, w16^2, w16^6
+__device__ __forceinline__ uint32_t r16_int_table(uint32_t w)
+{
+    return w == 0 ? 1u : w == 1 ? 0u : w + 1;
+}
+
+constexpr int kR16Sets(int W) { return W <= 18 ? 2 : 1; }
+
+// LDS words of the radix-16 pass: the slot sets (eight slots of 2 W words
+// x 64 lanes each), two external twiddle tables per wave, and the five
+// internal twiddle tables
+template <int W>
+constexpr uint32_t r16_lds_words()
+{
+    return (uint32_t)(kR16Sets(W) * 8 * 2 * W * 64 + (16 + 5) * 3 * mf_geom(W).RW);
+}
+
+// A radix-16 pass, panel layout in and out.  One item = one j and half a
+// panel (32 transforms); persistent over items it = block, block + grid, ...
+// (it = 2 (grp N/16 + j) + half, so the two halves of one j run side by
+// side).  Twiddle table: the pass's entries toff + 15 k + r - 1 (k = j mod
+// Ns, r = 1..15); c16e: the entry w^0 = 1, then w^(N/16), w^(3N/16),
+// w^(9N/16), w^(N/8), w^(3N/8) (w16^1, w16^3, w16^9, w16^2, w16^6).  The
+// external product runs for every item (k = 0 and position 0 take the
+// entry 1: exact), so every wave runs the same code; without EXT (the first
+// pass, Ns = 1: every twiddle is 1) it is skipped.  One set of slots (W > 18):
+// four barriers per item (products write, stage A reads, internal products
+// write, stage B reads, all in the one set); two sets: two barriers (stage A
+// reads set A, the internal products write set B, stage B reads set B).
+template <int W, bool EXT>
+__global__ void __launch_bounds__(512, 1)
+fxp_r16_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t toff,
+               uint32_t nitems, uint32_t c16e)
+{
+    extern __shared__ uint32_t lds[];
+    constexpr MfGeom G = mf_geom(W);
+    constexpr uint32_t SLOT = 2 * W * 64, TWN = 3 * G.RW, TWL = (TWN + 63) / 64;
+    constexpr int SETS = kR16Sets(W);
+    const uint32_t N = a.N, J = N >> 4;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    const uint32_t hl = lane >> 5, col = lane & 31u, h = 2 * wv + hl;
+    uint32_t *const setA = lds, *const setB = lds + (SETS - 1) * 8 * SLOT;
+    uint32_t *const tabE = lds + SETS * 8 * SLOT + wv * 2 * TWN;     // this wave's two tables
+    uint32_t *const w16t = lds + SETS * 8 * SLOT + 16 * TWN;         // the internal tables
+    for (uint32_t i = threadIdx.x; i < 5 * TWN; i += blockDim.x)
+        w16t[i] = a.tw[(size_t)(c16e + 1) * TWN + i];
+    __syncthreads();                                   // constant tables staged
+    uint32_t it = blockIdx.x;
+    if (it >= nitems) return;
+    // per lane-half, fixed over items: stage A reads v_(r2 + 4 s) (position r
+    // lives in the slot of wave r % 8, lane half r / 8), stage B reads
+    // Y'_(s, m1), which lane-half g with kR2[g] = s, kM1[g] = m1 holds
+    const uint32_t r2 = kR2[h], m1 = kM1[h];
+    uint32_t sa[4], sb[4];                             // word offsets in lds
+#pragma unroll
+    for (uint32_t s_ = 0; s_ < 4; s_++) {
+        const uint32_t r = r2 + 4 * s_;
+        sa[s_] = (r & 7u) * SLOT + 32 * (r >> 3) + col;
+        uint32_t g = 0;
+#pragma unroll
+        for (uint32_t x = 0; x < 16; x++)
+            if (kR2[x] == s_ && kM1[x] == m1) g = x;
+        sb[s_] = (uint32_t)(SETS - 1) * 8 * SLOT + (g >> 1) * SLOT + 32 * (g & 1u) + col;
+    }
+    const uint32_t q = m1 + 4 * r2;                    // stage B output of this lane-half
+    const uint32_t pos = wv + 8 * hl;                  // input position of this lane-half
+    auto item = [&](uint32_t i, uint32_t &j, uint32_t &grp, uint32_t &half, uint32_t &k) {
+        half = i & 1u;
+        const uint32_t jj = i >> 1;
+        j = jj % J; grp = jj / J; k = j & (Ns - 1);
+    };
+    // operands of item i for this lane, and the wave's two twiddle tables:
+    // positions wv and wv + 8 (the entry "1" for position 0 and for k = 0).
+    // In two parts (the real words and the tables, then the imaginary
+    // words), so that at most one part is held in registers across a product.
+    auto operand = [&](uint32_t i) {
+        uint32_t j, grp, half, k;
+        item(i, j, grp, half, k);
+        return src + pan_elem(grp, j + pos * J, N, W) + pan_lane<W>(32 * half + col);
+    };
+    auto fetch_re = [&](uint32_t i, Fx<W> &re, uint32_t (&ta)[TWL], uint32_t (&tb)[TWL]) {
+        uint32_t j, grp, half, k;
+        item(i, j, grp, half, k);
+        if (!EXT) {
+            pan_get<W>(re, operand(i));
+            return;
+        }
+        const uint32_t ea = (wv == 0 || k == 0) ? c16e : toff + 15 * k + wv - 1;
+        const uint32_t eb = k == 0 ? c16e : toff + 15 * k + wv + 7;
+        const uint32_t *t0 = a.tw + (size_t)ea * TWN, *t1 = a.tw + (size_t)eb * TWN;
+#pragma unroll
+        for (uint32_t u = 0; u < TWL; u++) {
+            ta[u] = (lane + 64 * u < TWN) ? (t0 + 64 * u)[lane] : 0u;
+            tb[u] = (lane + 64 * u < TWN) ? (t1 + 64 * u)[lane] : 0u;
+        }
+        pan_get<W>(re, operand(i));
+    };
+    auto fetch_im = [&](uint32_t i, Fx<W> &im) {
+        pan_get<W>(im, operand(i) + (size_t)N * W * kPanel);
+    };
+    Fx<W> re, im;
+    uint32_t ta[TWL], tb[TWL];
+    fetch_re(it, re, ta, tb);
+    fetch_im(it, im);
+    for (uint32_t n = 0;; n++) {
+        uint32_t j, grp, half, k;
+        item(it, j, grp, half, k);
+        const uint32_t nx = it + gridDim.x;
+        uint32_t *const slotA = setA + wv * SLOT, *const slotB = setB + wv * SLOT;
+        // one set: the slots were last read in the previous item's stage B
+        if (SETS == 1 && n > 0) __syncthreads();
+        // external twiddles w^(r k), r = this lane-half's position
+        if (!EXT) {
+#pragma unroll
+            for (int i = 0; i < W; i++) slotA[i * 64 + lane] = re.w[i];
+#pragma unroll
+            for (int i = 0; i < W; i++) slotA[(W + i) * 64 + lane] = im.w[i];
+            if (nx < nitems) fetch_re(nx, re, ta, tb);
+        } else {
+#pragma unroll
+            for (uint32_t u = 0; u < TWL; u++)
+                if (lane + 64 * u < TWN) {
+                    tabE[lane + 64 * u] = ta[u];
+                    tabE[TWN + lane + 64 * u] = tb[u];
+                }
+            MfOperand<W> Bx;
+            mf_operand<W>(Bx, re, im);
+            // re, im, ta, tb are consumed: the next item's real words and
+            // tables load during the rest of this item
+            if (nx < nitems) fetch_re(nx, re, ta, tb);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            mf_cmul<W, false, true>(Bx, tabE, tabE + G.RW, tabE + 2 * G.RW, lane, slotA,
+                                    tabE + TWN, tabE + TWN + G.RW, tabE + TWN + 2 * G.RW);
+        }
+        __syncthreads();
+        // stage A
+        Fx<W> yr, yi;
+        dft4_rt<W>(yr, yi, lds + sa[0], lds + sa[1], lds + sa[2], lds + sa[3], m1);
+        // internal twiddles (wave-uniform kind).  With one slot set they
+        // write where stage A reads: after a barrier, and the operands are
+        // converted after it too (no matrix operand is held across it)
+        if (wv == 7 && hl == 1) {
+            // -i (a + ib) = b - ia
+            Fx<W> z, t_;
+#pragma unroll
+            for (int i = 0; i < W; i++) z.w[i] = 0;
+            fx_sub(t_, z, yr);
+            yr = yi;
+            yi = t_;
+        }
+        if (SETS == 1) __syncthreads();               // stage A's reads of the one set
+        if (wv < 4) {
+            if (wv == 1 && hl == 1) {                  // w16^9 Y = w16^1 (-Y)
+                Fx<W> z;
+#pragma unroll
+                for (int i = 0; i < W; i++) z.w[i] = 0;
+                fx_sub(yr, z, yr);
+                fx_sub(yi, z, yi);
+            }
+            MfOperand<W> B;
+            mf_operand<W>(B, yr, yi);
+            const uint32_t *t0 = w16t + r16_int_table(wv) * TWN;
+            mf_cmul<W>(B, t0, t0 + G.RW, t0 + 2 * G.RW, lane, slotB);
+        } else {
+#pragma unroll
+            for (int i = 0; i < W; i++) slotB[i * 64 + lane] = yr.w[i];
+#pragma unroll
+            for (int i = 0; i < W; i++) slotB[(W + i) * 64 + lane] = yi.w[i];
+        }
+        // the next item's imaginary words load during stage B
+        if (nx < nitems) fetch_im(nx, im);
+        __syncthreads();
+        // stage B: output q = m1 + 4 m2 of this lane-half
+        dft4_rt<W>(yr, yi, lds + sb[0], lds + sb[1], lds + sb[2], lds + sb[3], r2);
+        uint32_t *p = dst + pan_elem(grp, (j - k) * 16 + k + q * Ns, N, W) +
+                      pan_lane<W>(32 * half + col);
+        pan_put<W>(p, yr);
+        pan_put<W>(p + (size_t)N * W * kPanel, yi);
+        if (nx >= nitems) break;
+        it = nx;
+    }
+}
+
 // Output conversion: panel layout -> float SoA rounded to P bits, through
 // an LDS transpose.  A block covers 32 elements of 16 transforms (block order
 // as fxp_in_kernel), a thread converts two reals: the panel side reads
@@ -740,7 +985,7 @@ static hipError_t launch_fxp(const FxArgs &a, int s0, int s1, hipStream_t st)
                            a, src);
     FxpPass ps[32];
     uint32_t nent = 0;
-    const int np = fxp_plan(N, W, ps, &nent);
+    const int np = fxp_plan(N, W, a.prb, ps, &nent);
     constexpr MfGeom G = mf_geom(W);
     int stage = 1;
     for (int i = 0; i < np; i++, stage++) {
@@ -757,6 +1002,24 @@ static hipError_t launch_fxp(const FxArgs &a, int s0, int s1, hipStream_t st)
                 const uint32_t nblk = nitems < occ * a.ncu ? nitems : occ * a.ncu;
                 hipLaunchKernelGGL((fxp_r4_kernel<W>), dim3(nblk), dim3(256), sh, st, a,
                                    (const uint32_t *)src, dst, P.Ns, P.toff, nitems);
+            } else if (P.R == 16) {
+                if constexpr (W <= kPanelR16MaxW) {
+                    static_assert(sizeof(uint32_t) * r16_lds_words<W>() <= 160 * 1024,
+                                  "radix-16 pass exceeds the LDS of a CU");
+                    const size_t sh = sizeof(uint32_t) * r16_lds_words<W>();
+                    const uint32_t nitems = (N / 16) * NG * 2;
+                    const uint32_t nblk = nitems < a.ncu ? nitems : a.ncu;
+                    if (P.Ns == 1)
+                        hipLaunchKernelGGL((fxp_r16_kernel<W, false>), dim3(nblk), dim3(512), sh,
+                                           st, a, (const uint32_t *)src, dst, P.Ns, P.toff,
+                                           nitems, nent + 1);
+                    else
+                        hipLaunchKernelGGL((fxp_r16_kernel<W, true>), dim3(nblk), dim3(512), sh,
+                                           st, a, (const uint32_t *)src, dst, P.Ns, P.toff,
+                                           nitems, nent + 1);
+                } else {
+                    return hipErrorInvalidValue;    // no radix-16 pass at this width
+                }
             } else if constexpr (W <= kPanelR8MaxW) {
                 const size_t sh =
                     sizeof(uint32_t) * (16 * 2 * W * 64 + G.RW + 8 * mf_table_words(W));

@@ -87,6 +87,7 @@ struct FxArgs {
     int32_t *eoff;              // batch entries
     int32_t *eoff_part;         // batch x eoff_nblk partial maxima of the scan
     uint32_t eoff_nblk;         // blocks of the scan per transform (fx_eoff_blocks)
+    int prb;                    // panel kernels: log2 of the main radix (fxp_plan)
 };
 
 // Exponent range of the limb format's inputs: |exponent| < kInputExpLimit
@@ -225,38 +226,59 @@ inline bool fxp_preferred(uint32_t N, uint32_t batch, int W)
     return 4 * (fxp_groups(batch) * kPanel - batch) <= batch;
 }
 
-// Passes of the fixed form on panels.  Up to W = kPanelR8MaxW words per real:
-// radix 8, after a first pass of radix 2 or 4 when log2 N is not a multiple
-// of 3; wider reals: radix 4 after a radix-2 first pass when log2 N is odd
-// (a radix-8 pass keeps eight operands of W words per lane in LDS and
-// registers; at W = 34 it spills and runs one block per CU, and measured
-// slower than two radix-4 passes' worth of work).  Pass i has radix R and
-// sub-transform size Ns; its twiddles w^(r k) (k < Ns, r = 1 .. R-1) are
-// table entries toff + (R - 1) k + r - 1.  One more entry, at index *nent,
-// holds the real constant 1/sqrt(2) of the radix-8 butterflies.
+// Passes of the fixed form on panels, for a main radix 2^rb: rb = 4 (radix
+// 16: eight waves hold 16 positions of 32 transforms and run the DFT16 as
+// 4 x 4 through LDS; W <= kPanelR16MaxW), rb = 3 (radix 8: 8 positions of
+// 64 transforms; W <= kPanelR8MaxW) or rb = 2 (radix 4, any width).  When
+// log2 N is not a multiple of rb the first pass has radix 2^(log2 N mod rb)
+// (radix 8 there needs W <= kPanelR8MaxW; wider: radix 2 then radix 4).
+// Pass i has radix R and sub-transform size Ns; its twiddles w^(r k) (k <
+// Ns, r = 1 .. R-1) are table entries toff + (R - 1) k + r - 1.  Seven more
+// entries follow at index *nent: w^(N/8) (its real part 1/sqrt(2) is the
+// constant of the radix-8 butterfly), w^0 = 1, then the radix-16 internal
+// twiddles w16^1, w16^3, w16^9, w16^2, w16^6 = w^(N/16), w^(3N/16),
+// w^(9N/16), w^(N/8), w^(3N/8).
 constexpr int kPanelR8MaxW = 18;
+constexpr int kPanelR16MaxW = 34;
+constexpr int kPanelConstEntries = 7;
 
 struct FxpPass {
     uint32_t R, Ns, toff;
 };
 
-inline int fxp_plan(uint32_t N, int W, FxpPass (&p)[32], uint32_t *nent)
+// the main radixes a width allows
+inline bool fxp_radix_ok(int W, int rb)
+{
+    return rb == 2 || (rb == 3 && W <= kPanelR8MaxW) || (rb == 4 && W <= kPanelR16MaxW);
+}
+
+// The default main radix per width: radix 16 up to kPanelR16MaxW words per
+// real, radix 4 beyond.  Measured on MI355X at N = 2^16, batch 64 (FFT/s
+// with the radix-16 plan against the best other): 81.0k / 70.1k (radix 8) at
+// 128 bits, 46.2k / 42.1k at 256, 28.2k / 25.4k at 512, 18.8k / 16.6k
+// (radix 4) at 768, 13.6k / 12.4k at 1024.
+inline int fxp_radix_default(int W) { return W <= kPanelR16MaxW ? 4 : 2; }
+
+inline int fxp_plan(uint32_t N, int W, int rb, FxpPass (&p)[32], uint32_t *nent)
 {
     int lg = 0;
     while ((1u << lg) < N) lg++;
-    const int rb = W <= kPanelR8MaxW ? 3 : 2;          // log2 of the main radix
+    if (!fxp_radix_ok(W, rb)) rb = 2;
     int n = 0;
     uint32_t Ns = 1, toff = 0;
-    if (lg % rb) {
-        const uint32_t R = 1u << (lg % rb);
-        p[n++] = {R, 1, toff};
-        toff += R - 1;
-        Ns = R;
+    auto add = [&](uint32_t R) {
+        p[n++] = {R, Ns, toff};
+        toff += (R - 1) * Ns;
+        Ns *= R;
+    };
+    const int rem = lg % rb;
+    if (rem == 3 && W > kPanelR8MaxW) {
+        add(2);
+        add(4);
+    } else if (rem) {
+        add(1u << rem);
     }
-    for (; Ns < N; Ns <<= rb) {
-        p[n++] = {1u << rb, Ns, toff};
-        toff += ((1u << rb) - 1) * Ns;
-    }
+    while (Ns < N) add(1u << rb);
     *nent = toff;
     return n;
 }

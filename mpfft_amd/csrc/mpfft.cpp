@@ -307,6 +307,7 @@ struct mpfft_plan_s {
     int fx_lim = 0;
     uint32_t fx_ntw = 0;
     bool fx_panel = false;   // panel kernels (matrix-core twiddle products)
+    int fx_prb = 2;          // panel kernels: log2 of the main radix (fxp_plan)
     int fx_npass = 0;        // passes of the panel plan
     uint32_t ncu = 0;        // compute units of the plan's device
     // double forms
@@ -426,17 +427,26 @@ int build_fx_plan(mpfft_plan_s *p)
     const MfGeom G = mf_geom((int)W);
     std::vector<unsigned long> ent;               // t of every entry, in table order
     if (p->fx_panel) {
-        // the panel passes (fxp_plan), then the constant 1/sqrt(2) as the real
-        // part of w^(N/8)
+        // the panel passes (fxp_plan), then the butterflies' constants.  The
+        // main radix: the width's default, or MPFFT_PANEL_RADIX (4, 8 or 16)
+        // from the environment where the width allows it (for timing the
+        // alternatives)
+        p->fx_prb = fxp_radix_default((int)W);
+        if (const char *e = getenv("MPFFT_PANEL_RADIX")) {
+            const int r = atoi(e), rb = r == 16 ? 4 : r == 8 ? 3 : r == 4 ? 2 : 0;
+            if (rb && fxp_radix_ok((int)W, rb)) p->fx_prb = rb;
+        }
         FxpPass ps[32];
         uint32_t nent = 0;
-        const int np = fxp_plan((uint32_t)N, (int)W, ps, &nent);
+        const int np = fxp_plan((uint32_t)N, (int)W, p->fx_prb, ps, &nent);
         for (int i = 0; i < np; i++)
             for (unsigned long k = 0; k < ps[i].Ns; k++)
                 for (unsigned long r = 1; r < ps[i].R; r++)
                     ent.push_back(r * k * (N / ((unsigned long)ps[i].R * ps[i].Ns)));
         if (ent.size() != nent) return MPFFT_ERR_ARG;
-        ent.push_back(N / 8);
+        // the butterflies' constants (kPanelConstEntries of them, fxp_plan)
+        for (unsigned long t : {N / 8, 0UL, N / 16, 3 * N / 16, 9 * N / 16, N / 8, 3 * N / 8})
+            ent.push_back(t);
         p->fx_npass = np;
     } else {
         for (unsigned long Ns = (lg0 & 1) ? 2 : 1; Ns < N; Ns *= 4)
@@ -627,6 +637,7 @@ int run_fx(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t s
     a.N = (uint32_t)p->N; a.batch = p->batch;
     a.lim = p->fx_lim; a.P = (int)p->P;
     a.ncu = p->ncu;
+    a.prb = p->fx_prb;
     a.eoff = p->fx_eoff.as<int32_t>();
     a.eoff_part = a.eoff + p->batch;
     a.eoff_nblk = fx_eoff_blocks((uint32_t)p->N);

@@ -32,14 +32,23 @@ static void plans()
         const MfGeom G = mf_geom(W);
         CHECK(mf_tail_fits(W) || mf_table_words(W) == 3 * G.RW, "W=%d table area", W);
         CHECK(G.KT * 32 >= G.NXB && G.MT * 32 >= G.NCOL, "W=%d tiles", W);
+        for (int rb = 2; rb <= 4; rb++)
         for (int lg = 4; lg <= 24; lg++) {
             const uint32_t N = 1u << lg;
             FxpPass p[32];
             uint32_t nent = 0, prod = 1, ent = 0;
-            const int np = fxp_plan(N, W, p, &nent);
+            const int np = fxp_plan(N, W, rb, p, &nent);
+            const int mb = fxp_radix_ok(W, rb) ? rb : 2;
             for (int i = 0; i < np; i++) {
                 CHECK(p[i].Ns == prod && p[i].toff == ent, "fxp W=%d lg=%d pass %d", W, lg, i);
                 CHECK(p[i].R != 8 || W <= kPanelR8MaxW, "fxp radix 8 at W=%d", W);
+                CHECK(p[i].R != 16 || W <= kPanelR16MaxW, "fxp radix 16 at W=%d", W);
+                // a radix-16 pass needs N >= 16 (one item covers 16 positions)
+                CHECK(p[i].R != 16 || N >= 16, "fxp radix 16 at N=%u", N);
+                // every pass but a first one (or a first two: radix 2 then 4)
+                // has the main radix
+                CHECK(i == 0 || p[i].R == (1u << mb) || (i == 1 && p[0].R == 2 && p[i].R == 4),
+                      "fxp W=%d rb=%d lg=%d pass %d radix %u", W, rb, lg, i, p[i].R);
                 ent += (p[i].R - 1) * p[i].Ns;
                 prod *= p[i].R;
             }

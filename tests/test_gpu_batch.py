@@ -145,9 +145,10 @@ def test_fixed_plan_partial_panel():
     n, P, B = 1024, 128, 120
     xs = [I.mp_random(n, P, seed=900 + i) for i in range(B)]
     plan = M.Plan(n, P, M.FORM_FIXED, B)
-    assert plan.stages() == 1 + 1 + 3 + 1           # panels: radix 2, then radix 8
+    assert plan.stages() == 1 + 3 + 1               # panels: radix 4, then two radix 16
     lane = M.Plan(n, P, M.FORM_FIXED_LANE, B)
-    assert lane.stages() != plan.stages()
+    assert plan.resolved_form() == M.FORM_FIXED_PANEL
+    assert lane.resolved_form() == M.FORM_FIXED_LANE
     x = np.concatenate([I.relimb(t, n, O.limbs(P), plan.L) for t in xs])
     yp = np.split(plan.execute_host(x), B)
     yl = np.split(lane.execute_host(x), B)

@@ -85,10 +85,11 @@ def test_panel_stage_ranges():
     plan = M.Plan(n, P, M.FORM_FIXED, b)
     x = np.concatenate([I.relimb(t, n, O.limbs(P), plan.L) for t in xs])
     S = plan.stages()
-    # input conversion, a radix-2 pass (log2 n = 10 = 1 + 3 * 3), three radix-8
-    # passes (reals of W = 10 words), output conversion
-    assert S == 1 + 1 + 3 + 1
-    assert M.Plan(n, 1024, M.FORM_FIXED, b).stages() == 1 + 5 + 1   # W = 34: five radix-4
+    # input conversion, a radix-4 pass (log2 n = 10 = 2 + 2 * 4), two radix-16
+    # passes, output conversion
+    assert S == 1 + 3 + 1
+    assert M.Plan(n, 1024, M.FORM_FIXED, b).stages() == 1 + 3 + 1   # W = 34 likewise
+    assert M.Plan(n, 2048, M.FORM_FIXED, b).stages() == 1 + 5 + 1   # W = 66: radix 4
     dx = torch.from_numpy(x.view(np.int32)).cuda()
     d1, d2 = torch.empty_like(dx), torch.empty_like(dx)
     plan.execute_device(dx.data_ptr(), d1.data_ptr())
@@ -149,7 +150,7 @@ def test_fixed_impulse(form, batch, n, P):
 
 
 def test_panel_large_n():
-    # N = 2^20 (plan: a radix-4 pass, then six radix-8 passes; buffers of
+    # N = 2^20 (plan: five radix-16 passes; buffers of
     # 64 x 2^21 reals), P = 128: every transform against the per-lane kernels,
     # two against the 2P-bit oracle
     n, P = 1 << 20, 128
@@ -165,7 +166,7 @@ def test_panel_large_n():
             h[nz] -= np.int32(2 * sh)
             xs.append(I.join(h, lm))
     plan = M.Plan(n, P, M.FORM_FIXED, 64)
-    assert plan.stages() == 1 + 1 + 6 + 1
+    assert plan.stages() == 1 + 5 + 1
     plan.close()
     yp = run(n, P, xs, M.FORM_FIXED)
     yl = run(n, P, xs, M.FORM_FIXED_LANE)
@@ -216,3 +217,71 @@ def test_panel_2048_bits_large_n_impulse():
     h, lm = I.split(y, n, O.limbs(P))
     assert (h[:n] == 2).all() and (lm[0, :n] == 0x80000000).all() and (lm[1:, :n] == 0).all()
     assert ((h[n:] & ~1) == np.int32(-2**31)).all() and (lm[:, n:] == 0).all()
+
+
+# ---- radix-16 passes (MPFFT_PANEL_RADIX=16) ----------------------------------
+#
+# The radix-16 panel pass (eight waves, 16 positions of 32 transforms, the
+# DFT16 as 4 x 4 through LDS, two twiddles per wave on the matrix cores) is
+# the default plan up to 34 words per real; MPFFT_PANEL_RADIX=16 makes it
+# explicit here, and the radix-4 and radix-8 passes it replaced stay
+# reachable (MPFFT_PANEL_RADIX=4 / 8) and tested below.  Same bar as above.
+
+@pytest.fixture
+def radix16(monkeypatch):
+    monkeypatch.setenv("MPFFT_PANEL_RADIX", "16")
+
+
+@pytest.mark.parametrize("n,P", [(16, 1024), (32, 256), (64, 53), (128, 32), (256, 1024),
+                                 (512, 100), (2048, 1024), (4096, 768), (8192, 512),
+                                 (1 << 16, 1024)])
+def test_panel_radix16(radix16, n, P):
+    print(f"case radix16 n={n} P={P}", flush=True)
+    b = 64
+    plan = M.Plan(n, P, M.FORM_FIXED_PANEL, b)
+    lg = n.bit_length() - 1
+    r = lg % 4
+    W = plan.L + 2
+    first = 0 if r == 0 else (2 if (r == 3 and W > 18) else 1)
+    assert plan.stages() == 2 + first + lg // 4, (plan.stages(), lg)
+    plan.close()
+    xs = inputs(n, P, b, seed=9700 + n + P)
+    yp = run(n, P, xs, M.FORM_FIXED_PANEL)
+    yl = run(n, P, xs, M.FORM_FIXED_LANE)
+    check_against_lane(n, P, yp, yl)
+    for i in (0, 33, 63):
+        check_against_oracle(n, P, xs[i], yp[i])
+
+
+def test_panel_radix16_stage_ranges(radix16):
+    import torch
+    n, P, b = 4096, 1024, 70                   # two panels, the second partial
+    xs = inputs(n, P, b, seed=78)
+    plan = M.Plan(n, P, M.FORM_FIXED_PANEL, b)
+    x = np.concatenate([I.relimb(t, n, O.limbs(P), plan.L) for t in xs])
+    S = plan.stages()
+    assert S == 1 + 3 + 1
+    dx = torch.from_numpy(x.view(np.int32)).cuda()
+    d1, d2 = torch.empty_like(dx), torch.empty_like(dx)
+    plan.execute_device(dx.data_ptr(), d1.data_ptr())
+    for s in range(S):
+        plan.execute_range(dx.data_ptr(), d2.data_ptr(), s, s + 1)
+    plan.check()
+    assert torch.equal(d1, d2)
+    plan.close()
+    y = np.split(d1.cpu().numpy().view(np.uint32), b)
+    for i in (0, 64, 69):
+        check_against_oracle(n, P, xs[i], I.relimb(y[i], n, M.limbs_for_precision(P), O.limbs(P)))
+
+
+@pytest.mark.parametrize("rad,n,P", [("4", 256, 1024), ("4", 8192, 256), ("8", 1024, 256),
+                                     ("8", 128, 32), ("4", 1 << 16, 1024)])
+def test_panel_other_radixes(monkeypatch, rad, n, P):
+    # the radix-4 and radix-8 plans, on request
+    monkeypatch.setenv("MPFFT_PANEL_RADIX", rad)
+    print(f"case radix{rad} n={n} P={P}", flush=True)
+    xs = inputs(n, P, 64, seed=9900 + n + P)
+    yp = run(n, P, xs, M.FORM_FIXED_PANEL)
+    yl = run(n, P, xs, M.FORM_FIXED_LANE)
+    check_against_lane(n, P, yp, yl)
+    check_against_oracle(n, P, xs[5], yp[5])
