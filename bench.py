@@ -42,9 +42,46 @@ MFMA_I8_PEAK_T = 5000.0          # dense int8 matrix-core peak, TOP/s
 
 PROFILES = {
     "plain": ("mp_level_kernel_N65536_P1024.txt", "mpfft::mp_level_kernel"),
-    "fixed": ("fxp_r4_kernel_N65536_P1024.txt", "mpfft::fxp_r4_kernel"),
+    "fixed": ("fxp_r16_kernel_N65536_P1024.txt", "mpfft::fxp_r16_kernel"),
     "fixed_lane": ("fx_r4_kernel_N65536_P1024.txt", "mpfft::fx_r4_kernel"),
 }
+
+
+def panel_plan(n, W):
+    """the panel passes (radix, Ns) of the default plan, as fxp_plan in
+    mpfft_amd/csrc/kernels.h: radix 16 up to 34 words per real (a first pass
+    of radix 2, 4 or 8 when log2 n is not a multiple of 4; radix 2 then 4
+    for a remainder of 3 above 18 words), radix 4 beyond"""
+    lg = n.bit_length() - 1
+    rb = 4 if W <= 34 else 2
+    out, ns = [], 1
+    rem = lg % rb
+    first = ([2, 4] if rem == 3 and W > 18 else [1 << rem]) if rem else []
+    for R in first + [1 << rb] * ((lg - rem) // rb):
+        out.append((R, ns))
+        ns *= R
+    return out
+
+
+def panel_mfma_ops(n, W, B):
+    """int8 MFMA operations (2 per multiply-add) the panel passes issue per
+    step of B transforms: every complex twiddle product is 8 tiles() MFMAs of
+    32x32x32 per wave (two parts, four per live tile); radix 16: per item
+    (16 positions, 32 transforms) eight waves' external products (none in a
+    first pass) and four waves' internal ones; radix 4 (4 positions, 64
+    transforms): the three waves with r != 0 when k != 0; radix 2: none"""
+    per = 8 * mfma_tiles(W) * 2 * 32 ** 3
+    groups = (B + 63) // 64
+    tot = 0
+    for R, ns in panel_plan(n, W):
+        if R == 16:
+            items = n // 16 * groups * 2
+            tot += items * ((8 if ns > 1 else 0) + 4) * per
+        elif R == 4:
+            tot += 3 * (n // 4) * (ns - 1) // ns * groups * per
+        elif R == 8:
+            tot += (7 * (n // 8) * (ns - 1) // ns + 2 * (n // 8)) * groups * per
+    return tot
 
 
 def cmul(n):
@@ -554,22 +591,19 @@ def main():
         S = rc["S"]
         kern_s = rc["parts_ms"][1] * 1e-3
         launches = S - 2
+        plan = panel_plan(n, W)
         bytes_launch = 2 * (2 * n * W * 4) * B
         achieved = bytes_launch * launches / kern_s / 1e9
-        tiles = mfma_tiles(W)
-        lg = n.bit_length() - 1
-        ns, prod_waves = (2 if lg & 1 else 1), 0
-        while ns < n:
-            prod_waves += 3 * (n // 4) * (ns - 1) // ns * (B // 64)
-            ns *= 4
-        mfma_ops = prod_waves * tiles * 4 * 2 * (32 * 32 * 32 * 2)
+        mfma_ops = panel_mfma_ops(n, W, B)
         copy_gbs = device_copy_rate(torch)
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": profiled_traffic(chosen, n, P, B),
-                "kernel": PROFILES[chosen][1],
+                "kernel": PROFILES[chosen][1] if plan[-1][0] == 16 else "mpfft::fxp_r4_kernel",
                 "kernel_avg_us": round(kern_s / launches * 1e6, 2),
                 "algorithmic_bytes_per_launch": bytes_launch,
+                "passes": [R for R, _ in plan],
+                "levels_per_sweep": round((n.bit_length() - 1) / launches, 2),
                 "mfma": {"achieved": round(mfma_ops / kern_s / 1e12, 2), "peak": MFMA_I8_PEAK_T,
                          "unit": "TOP/s (int8)", "frac": round(mfma_ops / kern_s / 1e12
                                                                / MFMA_I8_PEAK_T, 4)},

@@ -823,13 +823,10 @@ fxp_r16_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32
         item(i, j, grp, half, k);
         return src + pan_elem(grp, j + pos * J, N, W) + pan_lane<W>(32 * half + col);
     };
-    auto fetch_re = [&](uint32_t i, Fx<W> &re, uint32_t (&ta)[TWL], uint32_t (&tb)[TWL]) {
+    auto fetch_re = [&](uint32_t i, Fx<W> &re) { pan_get<W>(re, operand(i)); };
+    auto fetch_tw = [&](uint32_t i, uint32_t (&ta)[TWL], uint32_t (&tb)[TWL]) {
         uint32_t j, grp, half, k;
         item(i, j, grp, half, k);
-        if (!EXT) {
-            pan_get<W>(re, operand(i));
-            return;
-        }
         const uint32_t ea = (wv == 0 || k == 0) ? c16e : toff + 15 * k + wv - 1;
         const uint32_t eb = k == 0 ? c16e : toff + 15 * k + wv + 7;
         const uint32_t *t0 = a.tw + (size_t)ea * TWN, *t1 = a.tw + (size_t)eb * TWN;
@@ -838,14 +835,14 @@ fxp_r16_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32
             ta[u] = (lane + 64 * u < TWN) ? (t0 + 64 * u)[lane] : 0u;
             tb[u] = (lane + 64 * u < TWN) ? (t1 + 64 * u)[lane] : 0u;
         }
-        pan_get<W>(re, operand(i));
     };
     auto fetch_im = [&](uint32_t i, Fx<W> &im) {
         pan_get<W>(im, operand(i) + (size_t)N * W * kPanel);
     };
     Fx<W> re, im;
     uint32_t ta[TWL], tb[TWL];
-    fetch_re(it, re, ta, tb);
+    if (EXT) fetch_tw(it, ta, tb);
+    fetch_re(it, re);
     fetch_im(it, im);
     for (uint32_t n = 0;; n++) {
         uint32_t j, grp, half, k;
@@ -860,7 +857,7 @@ fxp_r16_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32
             for (int i = 0; i < W; i++) slotA[i * 64 + lane] = re.w[i];
 #pragma unroll
             for (int i = 0; i < W; i++) slotA[(W + i) * 64 + lane] = im.w[i];
-            if (nx < nitems) fetch_re(nx, re, ta, tb);
+            if (nx < nitems) fetch_re(nx, re);
         } else {
 #pragma unroll
             for (uint32_t u = 0; u < TWL; u++)
@@ -870,9 +867,9 @@ fxp_r16_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32
                 }
             MfOperand<W> Bx;
             mf_operand<W>(Bx, re, im);
-            // re, im, ta, tb are consumed: the next item's real words and
-            // tables load during the rest of this item
-            if (nx < nitems) fetch_re(nx, re, ta, tb);
+            // re, im, ta, tb are consumed: the next item's real words load
+            // during the rest of this item
+            if (nx < nitems) fetch_re(nx, re);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -914,8 +911,12 @@ fxp_r16_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32
 #pragma unroll
             for (int i = 0; i < W; i++) slotB[(W + i) * 64 + lane] = yi.w[i];
         }
-        // the next item's imaginary words load during stage B
-        if (nx < nitems) fetch_im(nx, im);
+        // the next item's imaginary words and twiddle tables load during
+        // stage B (where few registers are live)
+        if (nx < nitems) {
+            fetch_im(nx, im);
+            if (EXT) fetch_tw(nx, ta, tb);
+        }
         __syncthreads();
         // stage B: output q = m1 + 4 m2 of this lane-half
         dft4_rt<W>(yr, yi, lds + sb[0], lds + sb[1], lds + sb[2], lds + sb[3], r2);
