@@ -46,6 +46,8 @@ extern "C" {
 #define MPFFT_ERR_ARG      -6   /* null pointer or bad argument               */
 #define MPFFT_ERR_NODEV    -7   /* no usable GPU                              */
 #define MPFFT_ERR_FORM     -8   /* form not valid for this call               */
+#define MPFFT_ERR_SCRATCH  -9   /* beyond the fixed workspace of a
+                                   no-allocation entry point                  */
 
 /* ---- reference entry points -------------------------------------------- */
 
@@ -204,6 +206,20 @@ int mpfft_execute_inverse_limbs(mpfft_plan plan, const uint32_t *d_in, uint32_t 
  * sums are accumulated in double, not at P bits.  Waits for `stream`. */
 int mpfft_roundtrip_stats(mpfft_plan plan, const uint32_t *d_x, const uint32_t *d_y,
                           void *stream, double *out);
+
+/* mpfft_roundtrip_stats without any allocation: the same statistics, bit
+ * for bit, with the kernel's per-block partial sums in a fixed 4 MiB
+ * workspace compiled into the library (one per device, held until the
+ * process ends) instead of a buffer the plan allocates on first use, and
+ * read back through a static host buffer.  No counterpart in the reference:
+ * for callers that cannot allocate (serving loops and harnesses that
+ * forbid hipMalloc, code that must not allocate after set-up).  Limit:
+ * 2 ceil(2n / 256) * batch * 5 <= 2^19 doubles (n * batch <= 6.7 M points;
+ * beyond: MPFFT_ERR_SCRATCH).  The workspace keeps the last call's partial
+ * sums; calls serialize on a process-wide lock from launch to read back
+ * (overlapping calls would otherwise race on it).  Waits for `stream`. */
+int mpfft_roundtrip_stats_noalloc(mpfft_plan plan, const uint32_t *d_x, const uint32_t *d_y,
+                                  void *stream, double *out);
 
 /* The reference's test() on host data: forward transform of x, inverse of
  * the result, then mpfft_roundtrip_stats(x, round trip) into out[8 * batch]. */

@@ -27,7 +27,7 @@ OK = 0
 ERRORS = {
     -1: "MPFFT_ERR_SIZE", -2: "MPFFT_ERR_PREC", -3: "MPFFT_ERR_VALUE",
     -4: "MPFFT_ERR_SUBTREE", -5: "MPFFT_ERR_HIP", -6: "MPFFT_ERR_ARG",
-    -7: "MPFFT_ERR_NODEV", -8: "MPFFT_ERR_FORM",
+    -7: "MPFFT_ERR_NODEV", -8: "MPFFT_ERR_FORM", -9: "MPFFT_ERR_SCRATCH",
 }
 
 # every symbol include/mpfft.h declares
@@ -40,6 +40,7 @@ EXPORTS = [
     "mpfft_plan_sync", "mpfft_mpfr_to_limbs", "mpfft_limbs_to_mpfr",
     "mpfft_mp_binop_host", "mpfft_limbs_for_precision",
     "mpfft_execute_inverse_limbs", "mpfft_roundtrip_stats", "mpfft_roundtrip_host",
+    "mpfft_roundtrip_stats_noalloc",
     "mpfft_plan_last_timing", "mpfft_limbs_to_hex", "mpfft_hex_to_limbs",
     "mpfft_plan_check", "mpfft_batch_create", "mpfft_batch_destroy", "mpfft_batch_words",
     "mpfft_batch_info", "mpfft_batch_tune", "mpfft_batch_schedule",
@@ -85,6 +86,7 @@ def lib():
     L.mpfft_plan_sync.argtypes = [VP]
     L.mpfft_execute_inverse_limbs.argtypes = [VP, P, P, P]
     L.mpfft_roundtrip_stats.argtypes = [VP, P, P, P, P]
+    L.mpfft_roundtrip_stats_noalloc.argtypes = [VP, P, P, P, P]
     L.mpfft_roundtrip_host.argtypes = [VP, P, P]
     L.mpfft_plan_last_timing.argtypes = [VP, P]
     L.mpfft_plan_check.argtypes = [VP]

@@ -27,9 +27,41 @@ static_assert(sizeof(mp_limb_t) == 8, "64-bit GMP limbs expected");
 
 using namespace mpfft;
 
+// Workspace of mpfft_roundtrip_stats_noalloc: the error kernel's per-block
+// partial sums (kErrPartial doubles per block of kErrBlock reals), 4 MiB per
+// device, in the code object instead of a runtime allocation.  Host code
+// never passes this name to a kernel or copy: each device's copy has its
+// own address, which scratch_on() looks up once per device.
+constexpr size_t kScratchDoubles = size_t(1) << 19;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
 
 thread_local int g_last_error = MPFFT_OK;
+
+// per device: the address of `scratch` there (hipGetSymbolAddress), and a
+// static host buffer for the partial sums read back; both guarded by
+// g_scratch_mu, which a no-allocation call holds from its launch to its read
+// back, so such calls serialize within a process
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices];
+double g_scratch_host[kScratchDoubles];
+
+// the device address of `scratch` on device dev (which must be current);
+// g_scratch_mu held
+int scratch_on(int dev, double **ptr)
+{
+    if (dev < 0 || dev >= kMaxDevices) return MPFFT_ERR_NODEV;
+    if (!g_scratch_ptr[dev]) {
+        void *q = nullptr;
+        if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q)
+            return MPFFT_ERR_HIP;
+        g_scratch_ptr[dev] = static_cast<double *>(q);
+    }
+    *ptr = g_scratch_ptr[dev];
+    return MPFFT_OK;
+}
 
 constexpr long kMaxPrec = 2048;          // largest compiled limb count * 32 (panels)
 constexpr long kMaxLimbsAll = 32;        // limb counts every kernel family compiles
@@ -886,6 +918,7 @@ const char *mpfft_strerror(int code)
     case MPFFT_ERR_ARG: return "bad argument";
     case MPFFT_ERR_NODEV: return "no usable GPU";
     case MPFFT_ERR_FORM: return "form not valid for this call";
+    case MPFFT_ERR_SCRATCH: return "beyond the fixed workspace of a no-allocation entry point";
     default: return "unknown error";
     }
 }
@@ -1010,6 +1043,30 @@ int mpfft_roundtrip_stats(mpfft_plan p, const uint32_t *d_x, const uint32_t *d_y
                            hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     combine_err(part.data(), nblk, p->batch, out);
+    return MPFFT_OK;
+}
+
+int mpfft_roundtrip_stats_noalloc(mpfft_plan p, const uint32_t *d_x, const uint32_t *d_y,
+                                  void *stream, double *out)
+{
+    if (!p || !d_x || !d_y || !out) return MPFFT_ERR_ARG;
+    if (!is_mp(p->form)) return MPFFT_ERR_FORM;
+    const uint32_t nblk = (uint32_t)((2 * p->N + kErrBlock - 1) / kErrBlock);
+    const size_t nd = (size_t)nblk * p->batch * kErrPartial;
+    if (nd > kScratchDoubles) return MPFFT_ERR_SCRATCH;
+    std::lock_guard<std::mutex> g(p->mu);
+    HIP_TRY(hipSetDevice(p->device));
+    std::lock_guard<std::mutex> gs(g_scratch_mu);
+    double *ws = nullptr;
+    int rc = scratch_on(p->device, &ws);
+    if (rc) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : p->stream;
+    // every block writes all kErrPartial of its doubles: nothing to clear
+    HIP_TRY(mp_err_launch((int)p->L, (int)p->P, d_x, d_y, p->words, (uint32_t)p->N, ws,
+                          p->batch, st));
+    HIP_TRY(hipMemcpyAsync(g_scratch_host, ws, nd * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    combine_err(g_scratch_host, nblk, p->batch, out);
     return MPFFT_OK;
 }
 

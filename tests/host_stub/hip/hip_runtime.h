@@ -9,6 +9,9 @@
 #define __device__
 #define __global__
 #define __forceinline__ inline
+#define __align__(n) alignas(n)
+// a symbol's "device address" on the stand-in device is its host address
+#define HIP_SYMBOL(x) (&(x))
 typedef int hipError_t;
 enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorUnknown = 999 };
 struct ihipStream_t; typedef ihipStream_t *hipStream_t;
@@ -23,6 +26,7 @@ enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 1 };
 #define hipHostMallocDefault 0
 struct dim3 { unsigned x, y, z; dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {} };
 hipError_t hipMalloc(void **p, size_t b);
+hipError_t hipGetSymbolAddress(void **p, const void *symbol);
 hipError_t hipFree(void *p);
 hipError_t hipHostMalloc(void **p, size_t b, unsigned f);
 hipError_t hipHostFree(void *p);

@@ -136,6 +136,33 @@ static void alloc_failure_then_retry()
     mpfft_plan_destroy(p);
 }
 
+// The no-allocation statistics: no allocation on any call (the stand-in's
+// next hipMalloc would fail), the same results as the allocating entry
+// point, and the size limit.
+static void noalloc_stats()
+{
+    mpfft_plan p;
+    CHECK(mpfft_plan_create(&p, 4096, 256, MPFFT_FORM_MP, 3) == 0, "plan");
+    long L;
+    size_t w;
+    mpfft_plan_info(p, &L, &w);
+    std::vector<uint32_t> x(3 * w, 0);
+    std::vector<double> a(24), b(24);
+    stub_fail_malloc_in(0);
+    CHECK(mpfft_roundtrip_stats_noalloc(p, x.data(), x.data(), nullptr, a.data()) == 0, "noalloc");
+    CHECK(mpfft_roundtrip_stats_noalloc(p, x.data(), x.data(), nullptr, a.data()) == 0, "again");
+    stub_fail_malloc_in(-1);
+    CHECK(mpfft_roundtrip_stats(p, x.data(), x.data(), nullptr, b.data()) == 0, "alloc");
+    for (int i = 0; i < 24; i++) CHECK(a[i] == b[i] || (a[i] != a[i] && b[i] != b[i]), "same %d", i);
+    mpfft_plan_destroy(p);
+    mpfft_plan q;
+    CHECK(mpfft_plan_create(&q, 1UL << 16, 64, MPFFT_FORM_MP, 103) == 0, "plan big");
+    mpfft_plan_info(q, &L, &w);
+    CHECK(mpfft_roundtrip_stats_noalloc(q, x.data(), x.data(), nullptr, a.data()) == MPFFT_ERR_SCRATCH,
+          "limit");
+    mpfft_plan_destroy(q);
+}
+
 // The reference entry points on failure (argv[1] selects the case; the test
 // runs each in its own process): a precision mismatch is recorded and printed,
 // a HIP error aborts unless MPFFT_NO_ABORT=1.
@@ -160,6 +187,7 @@ int main(int argc, char **argv)
     if (argc > 1) return legacy(argv[1]);
     plans();
     alloc_failure_then_retry();
+    noalloc_stats();
     batch(MPFFT_FORM_MP, {1, 2, 4, 8, 16, 64, 512, 512, 64, 32, 256, 128, 8, 512},
           {64, 100, 128, 256, 53, 128, 256, 256, 1024, 200, 64, 512, 128, 100}, 0);
     srand(11);

@@ -26,6 +26,7 @@ hipError_t hipMalloc(void **p, size_t b) {
     if (g_malloc_fail_in >= 0 && g_malloc_fail_in-- == 0) { *p = nullptr; return hipErrorUnknown; }
     *p = malloc(b ? b : 1); return hipSuccess; }
 hipError_t hipFree(void *p) { free(p); return hipSuccess; }
+hipError_t hipGetSymbolAddress(void **p, const void *symbol) { *p = const_cast<void *>(symbol); return hipSuccess; }
 hipError_t hipHostMalloc(void **p, size_t b, unsigned) { *p = malloc(b ? b : 1); return hipSuccess; }
 hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
 hipError_t hipMemcpy(void *d, const void *s, size_t b, hipMemcpyKind) { memcpy(d, s, b); return hipSuccess; }
@@ -74,7 +75,11 @@ hipError_t mp_fft_launch(int L, int, const LeafArgs &la, const LevelArgs *lv, in
 }
 hipError_t mp_unswap_scale_launch(int L, uint32_t *d, uint64_t stride, uint32_t N, int32_t, uint32_t batch, hipStream_t) {
     run_or_record([=] { for (uint32_t b = 0; b < batch; b++) touch_w(d + stride * b, 2 * (size_t)N * (L + 1)); }); return hipSuccess; }
-hipError_t mp_err_launch(int, int, const uint32_t *, const uint32_t *, uint64_t, uint32_t, double *, uint32_t, hipStream_t) { return hipSuccess; }
+// writes every partial of every block (as the kernel does)
+hipError_t mp_err_launch(int, int, const uint32_t *, const uint32_t *, uint64_t, uint32_t N, double *partial, uint32_t batch, hipStream_t) {
+    const size_t nd = (size_t)((2 * N + kErrBlock - 1) / kErrBlock) * batch * kErrPartial;
+    run_or_record([=] { for (size_t i = 0; i < nd; i++) partial[i] = 1.0; });
+    return hipSuccess; }
 hipError_t mp_binop_launch(int, int, int, uint32_t, const uint32_t *, const uint32_t *, uint32_t *, hipStream_t) { return hipSuccess; }
 static void fx_touch(int L, const FxArgs &a, bool panel) {
     const size_t w = 2 * (size_t)a.N * (L + 1);
