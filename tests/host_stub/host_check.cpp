@@ -8,6 +8,8 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <string>
+#include <thread>
 #include <vector>
 
 void stub_fail_malloc_in(int k);      // stub.cpp
@@ -182,8 +184,48 @@ static int legacy(const char *what)
     return 0;
 }
 
+// The batched form's host logic from several host threads at once: plans
+// built on threads inside each mpfft_batch_create, tuning (timing, dealing,
+// capture), replays and re-capture on other buffers, and destruction, for
+// two batches used concurrently (run under ThreadSanitizer).
+static void batches_concurrently()
+{
+    auto work = [](int seed) {
+        srand(seed);
+        std::vector<unsigned long> n;
+        std::vector<long> P;
+        const long ps[] = {128, 256, 512, 1024};
+        for (int i = 0; i < 40; i++) {
+            n.push_back(1UL << (4 + rand() % 6));
+            P.push_back(ps[rand() % 4]);
+        }
+        for (int form : {MPFFT_FORM_FIXED, MPFFT_FORM_MP}) {
+            mpfft_batch b;
+            CHECK(mpfft_batch_create(&b, (unsigned)n.size(), n.data(), P.data(), form) == 0, "create");
+            size_t words = 0;
+            mpfft_batch_words(b, &words);
+            std::vector<uint32_t> x(words, 0), y(words, 0), y2(words, 0);
+            CHECK(mpfft_batch_tune(b, x.data(), y.data()) == 0, "tune");
+            for (int r = 0; r < 3; r++)
+                CHECK(mpfft_batch_execute_limbs(b, x.data(), y.data(), nullptr) == 0, "exec");
+            CHECK(mpfft_batch_execute_limbs(b, x.data(), y2.data(), nullptr) == 0, "recapture");
+            CHECK(mpfft_batch_tune(b, x.data(), y.data()) == 0, "retune");
+            mpfft_batch_check(b);
+            mpfft_batch_destroy(b);
+        }
+    };
+    std::thread a(work, 3), c(work, 4);
+    a.join();
+    c.join();
+}
+
 int main(int argc, char **argv)
 {
+    if (argc > 1 && std::string(argv[1]) == "threads") {
+        batches_concurrently();
+        printf(fails ? "FAIL\n" : "PASS\n");
+        return fails ? 1 : 0;
+    }
     if (argc > 1) return legacy(argv[1]);
     plans();
     alloc_failure_then_retry();

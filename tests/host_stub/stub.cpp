@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <cstdio>
+#include <atomic>
 #include <mutex>
 #include <vector>
 #include <functional>
@@ -20,10 +21,12 @@ static thread_local hipStream_t g_capture = nullptr;
 static std::vector<std::function<void()>> *cur_cap() { return g_capture ? g_capture->cap : nullptr; }
 static void run_or_record(std::function<void()> f) { if (auto c = cur_cap()) c->push_back(f); else f(); }
 // failure injection: the k-th hipMalloc from now fails (k = 0: the next one)
-static int g_malloc_fail_in = -1;
+static std::atomic<int> g_malloc_fail_in{-1};
 void stub_fail_malloc_in(int k) { g_malloc_fail_in = k; }
 hipError_t hipMalloc(void **p, size_t b) {
-    if (g_malloc_fail_in >= 0 && g_malloc_fail_in-- == 0) { *p = nullptr; return hipErrorUnknown; }
+    int k = g_malloc_fail_in.load();
+    while (k >= 0 && !g_malloc_fail_in.compare_exchange_weak(k, k - 1)) {}
+    if (k == 0) { *p = nullptr; return hipErrorUnknown; }
     *p = malloc(b ? b : 1); return hipSuccess; }
 hipError_t hipFree(void *p) { free(p); return hipSuccess; }
 hipError_t hipGetSymbolAddress(void **p, const void *symbol) { *p = const_cast<void *>(symbol); return hipSuccess; }
@@ -59,7 +62,7 @@ hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
 namespace mpfft {
 static const int kL[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 64};   // as mp_fft.hip
 int limbs_for_precision(long P) { long need = (P + 31) / 32; for (int L : kL) if (L >= need) return L; return -1; }
-static volatile uint32_t sink;
+static thread_local volatile uint32_t sink;   // keeps the read-touches (one per thread)
 static void touch_r(const uint32_t *p, size_t n) { uint32_t s = 0; for (size_t i = 0; i < n; i++) s += p[i]; sink = s; }
 static void touch_w(uint32_t *p, size_t n) { for (size_t i = 0; i < n; i++) p[i] = 0x5a5a5a5au; }
 hipError_t mp_fft_launch(int L, int, const LeafArgs &la, const LevelArgs *lv, int nlev, uint32_t batch, hipStream_t) {

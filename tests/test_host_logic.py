@@ -60,3 +60,24 @@ def test_legacy_hip_error_aborts(host_check):
     r = _run(host_check, "hip", MPFFT_NO_ABORT="1")
     assert r.returncode == 0, r.stderr[-2000:]
     assert "last_error -5" in r.stdout
+
+
+def test_batches_from_threads_under_tsan(tmp_path):
+    # the batched form from two host threads at once (each batch also builds
+    # its plans on threads), under ThreadSanitizer
+    if subprocess.run(["which", "g++"], capture_output=True).returncode:
+        pytest.skip("g++ not available")
+    exe = tmp_path / "host_check_tsan"
+    src = os.path.join(ROOT, "mpfft_amd", "csrc")
+    stub = os.path.join(ROOT, "tests", "host_stub")
+    cmd = ["g++", "-O1", "-g", "-std=c++17", "-fsanitize=thread", f"-I{stub}",
+           "-I/opt/conda/include", f"-I{os.path.join(ROOT, 'include')}", "-x", "c++",
+           os.path.join(src, "mpfft.cpp"), os.path.join(src, "batch.cpp"),
+           os.path.join(stub, "stub.cpp"), os.path.join(stub, "host_check.cpp"),
+           "-x", "none", "/lib/x86_64-linux-gnu/libmpfr.so.6",
+           "/lib/x86_64-linux-gnu/libgmp.so.10", "-lpthread", "-o", str(exe)]
+    subprocess.run(cmd, check=True, capture_output=True)
+    r = subprocess.run([str(exe), "threads"], capture_output=True, text=True, timeout=900,
+                       env=dict(os.environ, TSAN_OPTIONS="halt_on_error=1"))
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-5000:]
+    assert r.stdout.strip().endswith("PASS")
