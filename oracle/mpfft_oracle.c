@@ -487,7 +487,8 @@ int orc_err_stats(unsigned long N, long P, long Lx, long Ls,
     for (long i = 0; i < M; i++) {
         rc |= soa_get(x, hx, lx, Lx, M, i);
         rc |= soa_get(s, hs, ls, Ls, M, i);
-        mpfr_sub(d, x, s, RN);          /* exact at Pbig */
+        mpfr_sub(d, x, s, RN);  /* exact when the exponents are close, else
+                                   rounded at 64 bits more than either holds */
         mpfr_sqr(tmp, d, RN); mpfr_add(num, num, tmp, RN);
         mpfr_sqr(tmp, s, RN); mpfr_add(den, den, tmp, RN);
         if (mpfr_zero_p(s)) { if (!mpfr_zero_p(d)) zbad += 1; continue; }
@@ -528,7 +529,8 @@ int orc_err_excess(unsigned long N, long P, long Lx, long Ls,
     for (long i = 0; i < M; i++) {
         rc |= soa_get(x, hx, lx, Lx, M, i);
         rc |= soa_get(s, hs, ls, Ls, M, i);
-        mpfr_sub(d, x, s, RN);          /* exact at Pbig */
+        mpfr_sub(d, x, s, RN);  /* exact when the exponents are close, else
+                                   rounded at 64 bits more than either holds */
         mpfr_abs(d, d, RN);
         if (!mpfr_zero_p(x)) {
             mpfr_set_ui_2exp(h, 1, mpfr_get_exp(x) - P - 1, RN);   /* ulp_P(X)/2 */

@@ -249,6 +249,115 @@ def test_err_stats_closed_forms():
     assert O.err_stats(n, P, y, L, z, L2)[2] == 1
 
 
+def _mant(lim, k):
+    """the limbs of component k as one integer, most significant limb first"""
+    v = 0
+    for w in lim[:, k]:
+        v = (v << 32) | int(w)
+    return v
+
+
+def _set_mant(lim, k, v):
+    for i in range(lim.shape[0] - 1, -1, -1):
+        lim[i, k] = v & 0xFFFFFFFF
+        v >>= 32
+
+
+def _exact_pair(n=64, P=128, seed=4):
+    L, L2 = O.limbs(P), O.limbs(2 * P)
+    x = I.mp_random(n, P, seed=seed)
+    y = O.mp_fft(n, P, x)
+    h, _ = I.split(y, n, L)
+    k = int(np.argmax(np.where((h & ~1) == -2**31, -2**31, h)))   # largest |y_k|
+    return n, P, L, L2, x, y, I.relimb(y, n, L, L2), k, int(h[k]) >> 1
+
+
+def test_err_excess_closed_forms():
+    """the excess beyond half an ulp: 0 against an exact copy; 0 at exactly
+    half an ulp (the half-ulp term has the right exponent); a quarter ulp
+    past it reads 0.5 in units of half an ulp; a one-ulp change reads 1 such
+    unit; a zero result against a nonzero value reads that value's size (the
+    unit's sign)"""
+    n, P, L, L2, _, y, star, k, e = _exact_pair()
+    half = e - P - 1                              # half an ulp of y_k = 2^half
+    assert O.err_excess(n, P, y, L, star, L2, half) == (0.0, 0)
+    s = star.copy()
+    _, ls = I.split(s, n, L2)
+    m = _mant(ls, k)
+    _set_mant(ls, k, m + (1 << (32 * L2 - P - 1)))          # + ulp/2
+    assert O.err_excess(n, P, y, L, s, L2, half) == (0.0, 0)
+    _set_mant(ls, k, m + (3 << (32 * L2 - P - 2)))          # + 3/4 ulp
+    assert O.err_excess(n, P, y, L, s, L2, half) == (0.5, 1)
+    assert O.err_excess(n, P, y, L, s, L2, half - 3) == (4.0, 1)
+    bad = y.copy()
+    _, lb = I.split(bad, n, L)
+    lb[L - 1, k] ^= 1                                        # one ulp
+    assert O.err_excess(n, P, bad, L, star, L2, half) == (1.0, 1)
+    hz, lz = I.split(bad, n, L)
+    hz[k], lz[:, k] = -2**31, 0                              # y_k := 0
+    exc, over = O.err_excess(n, P, bad, L, star, L2, e)
+    _, lstar = I.split(star, n, L2)
+    assert over == 1 and exc == pytest.approx(_mant(lstar, k) / 2.0 ** (32 * L2), rel=1e-15)
+    assert 0.5 <= exc < 1.0
+
+
+def test_max_exponent_and_scale_pow2():
+    """the block exponent is the largest header exponent (signs and zeros
+    ignored, None for all zeros); scaling by 2^s moves every nonzero
+    header exponent by s and keeps the signs, limbs and zeros"""
+    n, L = 4, 2
+    hdr = np.array([(-5 << 1) | 1, (3 << 1) | 0, -2**31, -2**31 + 1,
+                    (-7 << 1) | 1, (2 << 1) | 1, -2**31, (1 << 1)], dtype=np.int32)
+    lim = np.full((L, 2 * n), 0x80000000, dtype=np.uint32)
+    lim[:, [2, 3, 6]] = 0
+    x = I.join(hdr, lim)
+    assert O.max_exponent(x, n) == 3
+    zeros = I.join(np.full(2 * n, -2**31, dtype=np.int32), np.zeros((L, 2 * n), np.uint32))
+    assert O.max_exponent(zeros, n) is None
+    for s in (-9, 0, 5):
+        y = O.scale_pow2(x, n, s)
+        hy, ly = I.split(y, n, L)
+        nz = (hdr & ~1) != -2**31
+        assert (hy[nz] >> 1 == (hdr[nz] >> 1) + s).all()
+        assert (hy & 1 == hdr & 1).all() and (hy[~nz] == hdr[~nz]).all()
+        assert np.array_equal(ly, lim) and O.max_exponent(y, n) == 3 + s
+    # exact scaling commutes with the correctly rounded transform
+    xr = I.mp_random(64, 128, seed=8)
+    assert np.array_equal(O.mp_fft(64, 128, O.scale_pow2(xr, 64, -77)),
+                          O.scale_pow2(O.mp_fft(64, 128, xr), 64, -77))
+
+
+def test_fixed_check_accepts_and_rejects():
+    """the fixed form's check: a result equal to the exact values passes;
+    an extra 900 units of 2^(emax - F) past half an ulp passes at n = 64
+    (bound 2 (6 + 2) 64 = 1024) and reads 900; 1100 units fail; a one-ulp
+    change fails (a mistake in F, emax or the bound's direction would fail
+    one of these)"""
+    n, P, L, L2, x, y, star, k, e = _exact_pair()
+    Ld = 4
+    F = 32 * (Ld + 1)
+    emax = O.max_exponent(x, n)
+    assert O.fixed_bound(n) == 1024.0
+    exc, mu, nw, over = O.fixed_check(n, P, Ld, x, y, star)
+    assert (exc, mu, nw, over) == (0.0, 0.0, 0.0, 0)
+    j = e - (emax - F)                        # fraction bit of 2^(emax - F) in y_k
+    assert P + 12 < j <= 32 * L2
+    s = star.copy()
+    _, ls = I.split(s, n, L2)
+    m = _mant(ls, k) + (1 << (32 * L2 - P - 1))
+    _set_mant(ls, k, m + (900 << (32 * L2 - j)))
+    exc, _, _, over = O.fixed_check(n, P, Ld, x, y, s)
+    assert exc == 900.0 and over == 1
+    _set_mant(ls, k, m + (1100 << (32 * L2 - j)))
+    with pytest.raises(AssertionError):
+        O.fixed_check(n, P, Ld, x, y, s)
+    bad = y.copy()
+    _, lb = I.split(bad, n, L)
+    lb[L - 1, k] ^= 1
+    with pytest.raises(AssertionError):
+        O.fixed_check(n, P, Ld, x, bad, star)
+
+
 def test_planted_mistake_is_caught(mp_golden, digests):
     """a one-ulp change anywhere in a 1024-point output must fail the
     bit-exact comparison (the comparison is not vacuous)."""
