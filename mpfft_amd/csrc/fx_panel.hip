@@ -266,10 +266,36 @@ __device__ __forceinline__ void mf_operand(MfOperand<W> &B, const Fx<W> &re, con
 // TWO: the operands of lanes 32..63 take another twiddle, tables Rc2, Rd2,
 // Rnd2 (the MFMAs of the two 32-lane groups, D0 and D1, get their own A
 // tiles; the B operands are laid out as before).
+// AM, how the A tiles are sourced for reals of up to 34 words (where the
+// output tiles are unrolled): 0 built from LDS at each use; 1 built once per
+// diagonal, at the first output tile that needs it, and kept in registers
+// while later tiles use it (W = 34: six builds per table instead of
+// nineteen); 2 built at each use, but the LDS reads for the next live tile
+// are issued before the current tile's matrix products, so their latency
+// overlaps the products and the carry pass (registers for one tile's reads
+// instead of a diagonal cache).
 // Results go to the wave's LDS slot: word j of re' at slot[j 64 + lane], of
 // im' at slot[(W + j) 64 + lane].  The twiddle tables may lie in the im' part
 // of the slot: all of them are read before the first word of im' is written.
-template <int W, bool REAL = false, bool TWO = false>
+// first output tile p whose product needs diagonal d (tile (p, p - d) live)
+constexpr int mf_first(int W, int d)
+{
+    const MfGeom g = mf_geom(W);
+    for (int p = 0; p < g.MT; p++)
+        if (p - d >= 0 && p - d < g.KT && mf_live(W, p, p - d)) return p;
+    return -1;
+}
+
+// the live tile after (p, q) in the order p, then q, as p KT + q (-1: none)
+constexpr int mf_next(int W, int p, int q)
+{
+    const MfGeom g = mf_geom(W);
+    for (int i = p * g.KT + q + 1; i < g.MT * g.KT; i++)
+        if (mf_live(W, i / g.KT, i % g.KT)) return i;
+    return -1;
+}
+
+template <int W, bool REAL = false, bool TWO = false, int AM = 0>
 __device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *Rc,
                                         const uint32_t *Rd, const uint32_t *Rnd, uint32_t lane,
                                         uint32_t *slot, const uint32_t *Rc2 = nullptr,
@@ -279,9 +305,16 @@ __device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *R
     static_assert(!(REAL && TWO), "one real twiddle per product");
     constexpr MfGeom G = mf_geom(W);
     constexpr int KT = G.KT, MT = G.MT, ND = G.DHI - G.DLO + 1;
+    // |S| <= 2 (4W - 3) 2^14 (two products of balanced digits): the 32-bit
+    // form of the carry step below holds up to W = 34; wider reals take
+    // 64-bit sums and keep the output-tile loop rolled
+    constexpr bool kNarrow = 257LL * 2 * (4 * W - 3) * 16384 + (1LL << 16) < (1LL << 31);
+    constexpr int kAM = kNarrow ? AM : 0;
+    constexpr int NTAB = REAL ? 1 : TWO ? 4 : 2;
     const uint32_t m = lane & 31u, g = lane >> 5;
     // byte offset of diagonal delta: U - (C0 + 32 delta + m - 16 g)
     const uint32_t off0 = (uint32_t)(G.U - G.C0 - 32 * G.DLO) - m + 16 * g;
+    const uint32_t sh = off0 & 3u, b0i = off0 >> 2;
     // the two parts one after the other (rolled: overlapping them holds two
     // sets of accumulators and carries)
 #pragma unroll 1
@@ -292,6 +325,12 @@ __device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *R
         const uint32_t *R2 = h ? Rc : Rnd;
         const uint32_t *R1b = TWO ? (h ? Rd2 : Rc2) : R1;
         const uint32_t *R2b = TWO ? (h ? Rc2 : Rnd2) : R2;
+        const uint32_t *tab[4] = {R1, R2, R1b, R2b};
+        // the lane's table offset, opaque here: addresses derived from it
+        // stay inside this loop (a constant offset from one register) rather
+        // than being computed once per tile and kept live across the items
+        uint32_t b0 = b0i;
+        if constexpr (kAM != 0) asm volatile("" : "+v"(b0));
 #if FXP_ACACHE
         v4i A1[ND], A2[ND];
 #pragma unroll
@@ -300,37 +339,98 @@ __device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *R
             A2[dd] = REAL ? A1[dd] : mf_atile(R2, off0 - 32 * dd);
         }
 #endif
-        // |S| <= 2 (4W - 3) 2^14 (two products of balanced digits): the
-        // 32-bit form of the carry step below holds up to W = 34; wider reals
-        // take 64-bit sums
-        constexpr bool kNarrow = 257LL * 2 * (4 * W - 3) * 16384 + (1LL << 16) < (1LL << 31);
         int32_t carry = 0;
         uint32_t *out = slot + (uint32_t)h * W * 64 + lane;
-        // wide reals: the output-tile loop stays rolled (register pressure)
+        // kAM 1: tiles by diagonal (lower half's tables 0, 1; upper half's
+        // 2, 3 with TWO); kAM 2: the reads of the current and the next tile
+        v4i Wt[kAM == 1 ? NTAB : 1][kAM == 1 ? ND : 1];
+        uint32_t rc[kAM == 2 ? NTAB : 1][5], rn[kAM == 2 ? NTAB : 1][5];
+        auto fetch = [&](uint32_t (&r)[kAM == 2 ? NTAB : 1][5], int dd) {
+#pragma unroll
+            for (int t = 0; t < (kAM == 2 ? NTAB : 1); t++)
+#pragma unroll
+                for (int i = 0; i < 5; i++) r[t][i] = tab[t][b0 - 8 * dd + i];
+        };
+        auto align = [&](const uint32_t (&r)[5]) {
+            v4i a;
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = (int)__builtin_amdgcn_alignbyte(r[i + 1], r[i], sh);
+            return a;
+        };
+        if constexpr (kAM == 2) {
+            constexpr int f = mf_next(W, 0, -1);
+            fetch(rc, f / KT - f % KT - G.DLO);
+        }
 #pragma unroll (kNarrow ? MT : 1)
         for (int p = 0; p < MT; p++) {
+            if constexpr (kAM == 1) {
+#pragma unroll
+                for (int q = KT - 1; q >= 0; q--) {
+                    if (!mf_live(W, p, q) || mf_first(W, p - q) != p) continue;
+                    const int dd = p - q - G.DLO;
+#pragma unroll
+                    for (int t = 0; t < NTAB; t++) Wt[t][dd] = mf_atile(tab[t], off0 - 32 * dd);
+                }
+            }
             v16i D0 = {}, D1 = {};
 #pragma unroll
             for (int q = 0; q < KT; q++) {
                 if (!mf_live(W, p, q)) continue;
                 const int dd = p - q - G.DLO;
+                v4i at[NTAB];
+                if constexpr (kAM == 1) {
+#pragma unroll
+                    for (int t = 0; t < NTAB; t++) at[t] = Wt[t][dd];
+                } else if constexpr (kAM == 2) {
+                    // the next live tile's reads go out before this tile's products
+                    const int nx = mf_next(W, p, q);
+                    if (nx >= 0) fetch(rn, nx / KT - nx % KT - G.DLO);
+#pragma unroll
+                    for (int t = 0; t < NTAB; t++) at[t] = align(rc[t]);
+                    if (nx >= 0) {
+#pragma unroll
+                        for (int t = 0; t < NTAB; t++)
+#pragma unroll
+                            for (int i = 0; i < 5; i++) rc[t][i] = rn[t][i];
+                    }
+                } else if constexpr (kAM == 3) {
+                    // all tables' reads for this tile go out together: one
+                    // LDS round trip per tile instead of one per table
+                    uint32_t r3[NTAB][5];
+#pragma unroll
+                    for (int t = 0; t < NTAB; t++)
+#pragma unroll
+                        for (int i = 0; i < 5; i++) r3[t][i] = tab[t][b0 - 8 * dd + i];
+#pragma unroll
+                    for (int t = 0; t < NTAB; t++) at[t] = align(r3[t]);
+                } else {
 #if FXP_ACACHE
-                const v4i a1 = A1[dd], a2 = A2[dd];
+                    at[0] = A1[dd];
+                    if (!REAL) at[1] = A2[dd];
 #else
-                const v4i a1 = mf_atile(R1, off0 - 32 * dd);
-                const v4i a2 = REAL ? a1 : mf_atile(R2, off0 - 32 * dd);
+                    at[0] = mf_atile(R1, off0 - 32 * dd);
+                    if (!REAL) at[REAL ? 0 : 1] = mf_atile(R2, off0 - 32 * dd);
+                    if (TWO) {
+                        at[TWO ? 2 : 0] = mf_atile(R1b, off0 - 32 * dd);
+                        at[TWO ? 3 : 0] = mf_atile(R2b, off0 - 32 * dd);
+                    }
 #endif
+                }
+                const v4i a1 = at[0];
                 if (REAL) {
                     D0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, h ? B.i0[q] : B.r0[q], D0, 0, 0, 0);
                     D1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, h ? B.i1[q] : B.r1[q], D1, 0, 0, 0);
                     continue;
                 }
-                const v4i b1 = TWO ? mf_atile(R1b, off0 - 32 * dd) : a1;
-                const v4i b2 = TWO ? mf_atile(R2b, off0 - 32 * dd) : a2;
+                const v4i a2 = at[REAL ? 0 : 1];
+                const v4i b1 = TWO ? at[TWO ? 2 : 0] : a1;
+                const v4i b2 = TWO ? at[TWO ? 3 : 0] : a2;
                 D0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, B.r0[q], D0, 0, 0, 0);
                 D1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(b1, B.r1[q], D1, 0, 0, 0);
                 D0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a2, B.i0[q], D0, 0, 0, 0);
                 D1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(b2, B.i1[q], D1, 0, 0, 0);
+                // keep the pipeline's order: at most one tile's reads ahead
+                if constexpr (kAM >= 2) __builtin_amdgcn_sched_barrier(0);
             }
             // D0 lane l: rows 8 (v/4) + 4 (l/32) + v%4 of operand l % 32,
             // D1 the same of operand l % 32 + 32.  After one swap per v the
@@ -751,6 +851,12 @@ __device__ __forceinline__ uint32_t r16_int_table(uint32_t w)
 
 constexpr int kR16Sets(int W) { return W <= 18 ? 2 : 1; }
 
+// A-tile sourcing of the radix-16 products (mf_cmul's AM): the diagonal
+// cache up to 18 words per real (measured 6 % faster passes at W = 18); at
+// wider reals it does not fit beside the operands in 256 registers (the
+// spills cost more than the LDS reads saved), nor do the other forms
+constexpr int r16_am(int W) { return W <= 18 ? 1 : 0; }
+
 // LDS words of the radix-16 pass: the slot sets (eight slots of 2 W words
 // x 64 lanes each), two external twiddle tables per wave, and the five
 // internal twiddle tables
@@ -857,7 +963,6 @@ fxp_r16_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32
             for (int i = 0; i < W; i++) slotA[i * 64 + lane] = re.w[i];
 #pragma unroll
             for (int i = 0; i < W; i++) slotA[(W + i) * 64 + lane] = im.w[i];
-            if (nx < nitems) fetch_re(nx, re);
         } else {
 #pragma unroll
             for (uint32_t u = 0; u < TWL; u++)
@@ -867,14 +972,11 @@ fxp_r16_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32
                 }
             MfOperand<W> Bx;
             mf_operand<W>(Bx, re, im);
-            // re, im, ta, tb are consumed: the next item's real words load
-            // during the rest of this item
-            if (nx < nitems) fetch_re(nx, re);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            mf_cmul<W, false, true>(Bx, tabE, tabE + G.RW, tabE + 2 * G.RW, lane, slotA,
-                                    tabE + TWN, tabE + TWN + G.RW, tabE + TWN + 2 * G.RW);
+            mf_cmul<W, false, true, r16_am(W)>(Bx, tabE, tabE + G.RW, tabE + 2 * G.RW, lane, slotA,
+                                          tabE + TWN, tabE + TWN + G.RW, tabE + TWN + 2 * G.RW);
         }
         __syncthreads();
         // stage A
@@ -904,16 +1006,17 @@ fxp_r16_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32
             MfOperand<W> B;
             mf_operand<W>(B, yr, yi);
             const uint32_t *t0 = w16t + r16_int_table(wv) * TWN;
-            mf_cmul<W>(B, t0, t0 + G.RW, t0 + 2 * G.RW, lane, slotB);
+            mf_cmul<W, false, false, r16_am(W)>(B, t0, t0 + G.RW, t0 + 2 * G.RW, lane, slotB);
         } else {
 #pragma unroll
             for (int i = 0; i < W; i++) slotB[i * 64 + lane] = yr.w[i];
 #pragma unroll
             for (int i = 0; i < W; i++) slotB[(W + i) * 64 + lane] = yi.w[i];
         }
-        // the next item's imaginary words and twiddle tables load during
-        // stage B (where few registers are live)
+        // the next item's operands and twiddle tables load during stage B,
+        // where few registers are live (none is held across a product)
         if (nx < nitems) {
+            fetch_re(nx, re);
             fetch_im(nx, im);
             if (EXT) fetch_tw(nx, ta, tb);
         }
