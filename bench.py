@@ -134,7 +134,7 @@ def profiled_traffic(form, n, P, batch):
 def measure(M, I, torch, n, P, batch, steps, warmup, seed, form):
     """Time `steps` transforms of `batch` on device-resident data, with HIP
     events (on the launch stream) around the parts of every step: plain
-    form: leaf pass | combine levels; fixed form on panels: input
+    form: leaf pass (with the levels finished in LDS) | combine levels on HBM; fixed form on panels: input
     conversion | radix passes | output conversion; fixed form on lanes: the
     radix passes (the first and last also convert from and to floats)."""
     plan = M.Plan(n, P, form, batch)
@@ -393,6 +393,31 @@ def measure_mpfr(M, I, n, P, seed, reps=3):
             "note": "one transform per call from mpfr_t arrays, PCIe transfers included"}
 
 
+def measure_lds_subtrees(M, I, torch, n, precs, sizes, batch, steps, seed):
+    """The plain form with its levels up to node size S finished in LDS
+    (MPFFT_SUB_SIZE = S; S = 0: every level an HBM sweep, the default): FFT/s
+    and the two stage times (leaves with the LDS levels | the HBM levels) per
+    precision and S.  Results are bit-identical for every S."""
+    out = {"workload": f"plain form, N={n}, batch {batch}", "unit": "FFT/s"}
+    old = os.environ.get("MPFFT_SUB_SIZE")
+    try:
+        for P in precs:
+            row = {}
+            for S in sizes:
+                os.environ["MPFFT_SUB_SIZE"] = str(S)
+                r = measure(M, I, torch, n, P, batch, steps, 1, seed, M.FORM_MP)
+                row[f"S{S}"] = {"fft_per_s": round(r["fft_s"], 1),
+                                "stage_ms": [round(x, 4) for x in r["parts_ms"]],
+                                "hbm_level_launches": r["S"] - 1}
+            out[f"P{P}"] = row
+    finally:
+        if old is None:
+            os.environ.pop("MPFFT_SUB_SIZE", None)
+        else:
+            os.environ["MPFFT_SUB_SIZE"] = old
+    return out
+
+
 def measure_batched(M, I, torch, seed, count, steps):
     """The batched workload: `count` transforms, log2 n uniform over 6..14, P
     over {128, 256, 512, 1024}, inputs resident in HBM.  Times one graph
@@ -496,6 +521,10 @@ def main():
                     help="log2 N of the wide-precision leg (512 and 2048 bits; 0 = skip)")
     ap.add_argument("--batched-count", type=int, default=4096,
                     help="transforms of the batched uneven-(n, P) workload (0 = skip)")
+    ap.add_argument("--sub-precs", default="128,256",
+                    help="precisions of the plain form's LDS-subtree leg ('' = skip)")
+    ap.add_argument("--sub-sizes", default="0,128,512",
+                    help="subtree sizes finished in LDS for that leg (0 = none)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -665,6 +694,12 @@ def main():
     if a.wide_lgn:
         side_leg(line, "wide", lambda: measure_wide(M, I, torch, O, a.wide_lgn, 2, a.seed,
                                                     err=not a.no_err))
+    if a.sub_precs:
+        side_leg(line, "plain_lds_subtrees",
+                 lambda: measure_lds_subtrees(M, I, torch, n,
+                                              [int(v) for v in a.sub_precs.split(",") if v],
+                                              [int(v) for v in a.sub_sizes.split(",") if v],
+                                              B, max(2, a.steps // 2), a.seed))
     side_leg(line, "valu_probe", valu_probe)
     if line["roofline"]["bound"] == "valu" and "rates" in line.get("valu_probe", {}):
         mad = line["valu_probe"]["rates"].get("v_mad_u64_u32 (C)")

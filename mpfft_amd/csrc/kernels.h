@@ -38,12 +38,45 @@ struct LeafArgs {
     int P;
 };
 
+// Subtrees finished in LDS (plain forms): the maximal subtrees of node size
+// <= S (nodes of size S or S/2 whose parent is larger; the whole transform
+// when N <= S), one workgroup each: its leaves, then every combine level
+// inside it, on the subtree held in LDS, then one write back.
+constexpr int kSubMaxLevel = 16;
+struct SubArgs {
+    LeafArgs leaf;              // inputs (element-major copy), leaf positions, out
+    const uint32_t *lut;        // twiddles as in LevelArgs
+    uint32_t lut_m, lut_e;
+    int code;
+    const uint32_t *sub;        // per subtree: offset << 1 | kind (0: size 2^lgs[0],
+                                // 1: size 2^lgs[1])
+    uint32_t nsub;
+    const uint32_t *rel;        // node offsets inside a subtree, per kind and level
+    uint32_t rel_first[2][kSubMaxLevel], rel_cnt[2][kSubMaxLevel];
+    uint32_t lut_off[kSubMaxLevel];
+    uint32_t lgs[2];
+    uint32_t nthr;              // threads per workgroup
+};
+
 // kernels exist for these limb counts; a precision P uses the smallest
 // count >= ceil(P/32)
 int limbs_for_precision(long P);
 
+// Default subtree size finished in LDS (0: none; MPFFT_SUB_SIZE sets one).
+// Measured at N = 2^16, batch 64, against the level kernels on HBM: equal
+// at 64 and 128 bits (S = 512: 69.0 k vs 68.9 k FFT/s; S = 128: 46.4 k vs
+// 45.9 k), slower at 256 bits (best S = 128: 25.5 k vs 26.6 k) and 512 bits
+// (best S = 128: 11.0 k vs 13.7 k).  A subtree held in LDS costs 4 (L+1)
+// words per lane beside the 2 (L+1) parking words, which caps a CU near 12
+// waves where the level kernels run about 24, and at these precisions a
+// level's HBM sweep costs no more than its arithmetic: so none by default.
+inline uint32_t mp_sub_size(int) { return 0u; }
+
+// leaves and levels; with sub != nullptr the leaves and the levels of node
+// size <= 2^lgs[0] run as the subtree kernel first
 hipError_t mp_fft_launch(int L, int P, const LeafArgs &la, const LevelArgs *lv,
-                         int nlev, uint32_t batch, hipStream_t st);
+                         int nlev, uint32_t batch, hipStream_t st,
+                         const SubArgs *sub = nullptr);
 // in place: swap the real and imaginary halves and divide by 2^lgN (exact:
 // an exponent shift), the closing step of the inverse by the swap trick
 hipError_t mp_unswap_scale_launch(int L, uint32_t *data, uint64_t stride, uint32_t N,

@@ -84,15 +84,11 @@ mp_to_aos_kernel(LeafArgs a)
 // operands, so recomputing y and a per lane changes no result bit.
 // Inputs are gathered as whole elements from the element-major copy.
 template <int L, bool FULL>
-__global__ void __launch_bounds__(256, kMpBlocks<L>)
-mp_leaf_kernel(LeafArgs a)
+__device__ __forceinline__ void mp_leaf_value(Mp<L> &res, const LeafArgs &a, uint32_t b,
+                                              uint32_t h, uint32_t p)
 {
-    const uint32_t N = a.N, M = 2 * N;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= M) return;
-    const uint32_t h = t >= N ? 1u : 0u, p = t - h * N;
-    const uint32_t *in = a.aos + a.aos_stride * blockIdx.y;
-    uint32_t *out = a.data + a.stride * blockIdx.y;
+    const uint32_t N = a.N;
+    const uint32_t *in = a.aos + a.aos_stride * b;
     const int P = a.P;
     const int32_t sn = a.pos_sn[p];
     const uint32_t nr = a.pos_nr[p], n = nr & 0xffu, r = nr >> 8;
@@ -101,7 +97,7 @@ mp_leaf_kernel(LeafArgs a)
     const uint32_t odd = r & 1u;
     const uint32_t hb = h * N, ha = (four && odd) ? (N - hb) : hb;   // part of a
     const int nit = four ? 3 : 1;
-    Mp<L> u, v, y, res;
+    Mp<L> u, v, y;
 #pragma unroll 1
     for (int it = 0; it < nit; it++) {
         uint32_t fy;
@@ -120,39 +116,47 @@ mp_leaf_kernel(LeafArgs a)
         mp_add<L, FULL>(res, u, 0, v, fy, P);
         if (it == 0) y = res;
     }
+}
+
+template <int L, bool FULL>
+__global__ void __launch_bounds__(256, kMpBlocks<L>)
+mp_leaf_kernel(LeafArgs a)
+{
+    const uint32_t N = a.N, M = 2 * N;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M) return;
+    const uint32_t h = t >= N ? 1u : 0u, p = t - h * N;
+    uint32_t *out = a.data + a.stride * blockIdx.y;
+    Mp<L> res;
+    mp_leaf_value<L, FULL>(res, a, blockIdx.y, h, p);
     mp_store(res, out, M, t);
 }
 
-// Two waves per SIMD (256 registers per lane): one wave per SIMD leaves the
-// dependent multiply chains nothing to overlap with.
+// One butterfly of a size-n node (serial_mp_fft3.c:184-248): the pair of
+// lanes B = 0 (Z+) and B = 1 (Z-) of butterfly k of the node at offset o.
+// The transform's reals sit in `data` as SoA rows of row stride M (hdr
+// row, then limb rows), real parts at positions 0.., imaginary parts at Np
+// + position: the HBM buffer (M = 2N, Np = N) or a subtree held in LDS (M =
+// 2s, Np = s, positions relative to the subtree).  `lds` holds the two
+// parking slots of each of the nt threads of the block.
 template <int L, bool FULL, bool FOUR_M>
-__global__ void __launch_bounds__(256, kMpBlocks<L>)
-mp_level_kernel(LevelArgs a)
+__device__ __forceinline__ void mp_butterfly(uint32_t *data, uint32_t M, uint32_t Np,
+                                             uint32_t o, uint32_t k, uint32_t n, uint32_t B,
+                                             int code, const uint32_t *lut, uint32_t lut_m,
+                                             uint32_t lut_e, uint32_t lut_off, int P,
+                                             uint32_t *lds, uint32_t nt, uint32_t lt)
 {
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t pair = tid >> 1, B = tid & 1u;   // B: 0 -> Z+ lane, 1 -> Z- lane
-    uint32_t *out = a.data + a.stride * blockIdx.y;
-    const uint32_t N = a.N, n = a.n, q = n >> 2, M = 2 * N, Q = N >> 2;
-    const int P = a.P;
-
-    // node-major: all k of a node on neighbouring lanes, so each load or
-    // store instruction covers dense runs of positions; the three kinds
-    // (0: general k, 1: k = n/8, 2: k = 0) diverge only around the products
-    const uint32_t node = pair / q;
-    if (node >= a.cnt) return;
-    const uint32_t k = pair - node * q;
+    uint32_t *const out = data;
+    const uint32_t q = n >> 2, N = Np;
     // CODE 1 (serial_mp_fft1.c:181-218) has no special butterflies
-    const uint32_t kind = a.code == 1 ? 0u : (k == 0 ? 2u : (k == (n >> 3) ? 1u : 0u));
-    const uint32_t o = a.node_off[node];
+    const uint32_t kind = code == 1 ? 0u : (k == 0 ? 2u : (k == (n >> 3) ? 1u : 0u));
     const uint32_t zi = o + k + (B ? 3 * q : 2 * q);  // Z+ or Z-
     const uint32_t ui = o + k + (B ? q : 0);          // U0 or U1
-    const uint32_t kt = a.lut_off + k;                // this level's twiddle entry k
+    const uint32_t kt = lut_off + k;                  // this level's twiddle entry k
 
     // o1, o2: m1 = Z+ * w^k on the Z+ lane, m2 = Z- * conj(w^k) on the Z- lane.
     // Intermediates that must survive a multiplication wait in LDS slots
     // 0 and 1, so only the multiplication's own state is live across it.
-    extern __shared__ uint32_t lds[];
-    const uint32_t lt = threadIdx.x, nt = blockDim.x;
     Mp<L> o1, o2;
     mp_load(o1, out, M, zi);
     mp_load(o2, out, M, N + zi);
@@ -185,11 +189,11 @@ mp_level_kernel(LevelArgs a)
             if (it == 0 && !m4) f = r;
             else if (kind == 1) lds_get(f, lds, nt, lt, 1);
             else mp_load(f, out, M, (m4 ? (it == 0 || it == 3) : it == 2) ? zi : N + zi);
-            const uint32_t E = a.lut_e;
+            const uint32_t E = lut_e;
             if (kind == 1) row = 0;
             else if (m4) row = (it & 1) ? E : 0;
             else row = it == 0 ? 0 : (it == 1 ? 2 * E : E);
-            mp_load(w, a.lut, a.lut_m, row + kt);
+            mp_load(w, lut, lut_m, row + kt);
             mp_mul<L, FULL>(r, f, w, P);
             if (m4 ? (it & 1) : (kind == 0 && it > 0)) {
                 Mp<L> T;
@@ -231,6 +235,81 @@ mp_level_kernel(LevelArgs a)
         const uint32_t fy = (uint32_t)(it & 1) ^ (im & B);
         mp_add<L, FULL>(r, U, 0, x, fy, P);
         mp_store(r, out, M, (im ? N : 0) + ((it & 1) ? s0 : f0));
+    }
+}
+
+// Two waves per SIMD (256 registers per lane): one wave per SIMD leaves the
+// dependent multiply chains nothing to overlap with.  Butterflies are laid
+// out node-major: all k of a node on neighbouring lanes, so each load or
+// store instruction covers dense runs of positions; the three kinds
+// (general k, k = n/8, k = 0) diverge only around the products.
+template <int L, bool FULL, bool FOUR_M>
+__global__ void __launch_bounds__(256, kMpBlocks<L>)
+mp_level_kernel(LevelArgs a)
+{
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t pair = tid >> 1, B = tid & 1u;   // B: 0 -> Z+ lane, 1 -> Z- lane
+    const uint32_t q = a.n >> 2;
+    const uint32_t node = pair / q;
+    if (node >= a.cnt) return;
+    extern __shared__ uint32_t lds[];
+    mp_butterfly<L, FULL, FOUR_M>(a.data + a.stride * blockIdx.y, 2 * a.N, a.N,
+                                  a.node_off[node], pair - node * q, a.n, B, a.code, a.lut,
+                                  a.lut_m, a.lut_e, a.lut_off, a.P, lds, blockDim.x,
+                                  threadIdx.x);
+}
+
+// A subtree finished in LDS (serial_mp_fft3.c:180-182 below node size s):
+// workgroup (subtree, transform).  Its 2s reals live in LDS as SoA rows of
+// 2s words (real parts, then imaginary parts, positions relative to the
+// subtree); the leaves are computed from the element-major input copy
+// straight into them, then each combine level n = 8 .. s runs on them with
+// a barrier between levels (the same butterflies, operand for operand, as
+// the level kernel), and the subtree is written back once.  The levels of a
+// subtree are the node offsets of one subtree of its size (the tree below a
+// node depends only on the node's size).
+template <int L, bool FULL, bool FOUR_M>
+__global__ void __launch_bounds__(256)
+mp_sub_kernel(SubArgs a)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t nt = a.nthr, lt = threadIdx.x;
+    uint32_t *const park = lds;                              // 2 (L+1) nt words
+    uint32_t *const buf = lds + 2 * (L + 1) * nt;            // (L+1) x 2s words
+    const uint32_t sd = a.sub[blockIdx.x];
+    const uint32_t kind = sd & 1u, o = sd >> 1;
+    const uint32_t lgs = a.lgs[kind], s = 1u << lgs, Ml = 2 * s;
+    const uint32_t N = a.leaf.N, M = 2 * N;
+    const int P = a.leaf.P;
+    uint32_t *const out = a.leaf.data + a.leaf.stride * blockIdx.y;
+    for (uint32_t idx = lt; idx < Ml; idx += nt) {
+        const uint32_t h = idx >= s ? 1u : 0u;
+        Mp<L> res;
+        mp_leaf_value<L, FULL>(res, a.leaf, blockIdx.y, h, o + idx - h * s);
+        lds_put(buf, Ml, idx, 0, res);
+    }
+    __syncthreads();
+    for (uint32_t l = 3; l <= lgs; l++) {
+        const uint32_t n = 1u << l, q = n >> 2;
+        const uint32_t lanes = a.rel_cnt[kind][l] * (n >> 1);
+        const uint32_t *const no = a.rel + a.rel_first[kind][l];
+        for (uint32_t base = 0; base < lanes; base += nt) {
+            // lanes come in pairs (nt and lanes are even): a pair is active
+            // or idle as a whole, as the pair exchange needs
+            const uint32_t tid = base + lt;
+            if (tid < lanes) {
+                const uint32_t pair = tid >> 1, node = pair / q;
+                mp_butterfly<L, FULL, FOUR_M>(buf, Ml, s, no[node], pair - node * q, n,
+                                              tid & 1u, a.code, a.lut, a.lut_m, a.lut_e,
+                                              a.lut_off[l], P, park, nt, lt);
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t idx = lt; idx < Ml; idx += nt) {
+        const uint32_t h = idx >= s ? 1u : 0u, g = h * N + o + idx - h * s;
+#pragma unroll
+        for (int j = 0; j <= L; j++) out[(size_t)j * M + g] = buf[j * Ml + idx];
     }
 }
 
@@ -368,12 +447,25 @@ mp_err_kernel(const uint32_t *x, const uint32_t *y, uint64_t stride, uint32_t N,
 
 template <int L, bool FULL>
 static hipError_t launch_fft(const LeafArgs &la, const LevelArgs *lv, int nlev,
-                             uint32_t batch, hipStream_t st)
+                             uint32_t batch, hipStream_t st, const SubArgs *sub)
 {
-    {
-        if (la.npos) {
-            dim3 grid((2 * la.N + 255) / 256, batch);
-            hipLaunchKernelGGL((mp_to_aos_kernel<L>), grid, dim3(256), 0, st, la);
+    if (la.npos) {
+        dim3 grid((2 * la.N + 255) / 256, batch);
+        hipLaunchKernelGGL((mp_to_aos_kernel<L>), grid, dim3(256), 0, st, la);
+        if (L <= 16 && sub) {
+            SubArgs sa = *sub;
+            sa.leaf = la;
+            const size_t sh = sizeof(uint32_t) * (L + 1) *
+                              (2 * (size_t)sa.nthr + 2 * ((size_t)1 << sa.lgs[0]));
+            if constexpr (L <= 16) {
+                if (sa.code == 3)
+                    hipLaunchKernelGGL((mp_sub_kernel<L, FULL, false>), dim3(sa.nsub, batch),
+                                       dim3(sa.nthr), sh, st, sa);
+                else
+                    hipLaunchKernelGGL((mp_sub_kernel<L, FULL, true>), dim3(sa.nsub, batch),
+                                       dim3(sa.nthr), sh, st, sa);
+            }
+        } else {
             hipLaunchKernelGGL((mp_leaf_kernel<L, FULL>), grid, dim3(256), 0, st, la);
         }
     }
@@ -414,10 +506,10 @@ int limbs_for_precision(long P)
 }
 
 hipError_t mp_fft_launch(int L, int P, const LeafArgs &la, const LevelArgs *lv,
-                         int nlev, uint32_t batch, hipStream_t st)
+                         int nlev, uint32_t batch, hipStream_t st, const SubArgs *sub)
 {
     const bool full = (P == 32 * L);
-#define CALLF(Lv, F) launch_fft<Lv, F>(la, lv, nlev, batch, st)
+#define CALLF(Lv, F) launch_fft<Lv, F>(la, lv, nlev, batch, st, sub)
     switch (L) {
     case 1: return full ? CALLF(1, true) : CALLF(1, false);
     case 2: return full ? CALLF(2, true) : CALLF(2, false);

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -333,6 +334,9 @@ struct mpfft_plan_s {
     DevBuf lut, pos_sn, pos_nr, nodes, aos;
     LeafArgs leaf{};
     std::vector<LevelArgs> levels;
+    DevBuf sub_list, sub_rel;            // subtrees finished in LDS
+    SubArgs sub{};
+    bool use_sub = false;
     // fixed form
     DevBuf fx_tw, fx_buf0, fx_buf1, fx_flag;
     DevBuf fx_eoff;          // block exponents: batch, then batch x fx_eoff_blocks(N) partials
@@ -413,8 +417,55 @@ int build_mp_plan(mpfft_plan_s *p)
     la.npos = N >= 2 ? (uint32_t)N : 0u;
     la.N = (uint32_t)N; la.P = (int)p->P; la.in_swap = 0;
     la.in_stride = la.stride = p->words;
+    // subtrees finished in LDS: the levels up to node size S run inside
+    // them (MPFFT_SUB_SIZE overrides S; 0 keeps every level on HBM)
+    uint32_t S = N >= 8 ? mp_sub_size((int)p->L) : 0u;
+    if (const char *e = getenv("MPFFT_SUB_SIZE")) S = (uint32_t)strtoul(e, nullptr, 10);
+    if (S && (!is_pow2(S) || S < 8)) S = 0;
+    if (S > N) S = (uint32_t)N;
+    const uint32_t nthr = S ? std::min(256u, std::max(64u, S / 2)) : 0u;
+    if (S && sizeof(uint32_t) * (p->L + 1) * (2 * (size_t)nthr + 2 * (size_t)S) > 160 * 1024)
+        S = 0;                                    // does not fit a workgroup's LDS
+    p->use_sub = S != 0;
+    uint32_t lgS = 0;
+    while ((1u << lgS) < S) lgS++;
+    if (p->use_sub) {
+        // maximal subtrees of size <= S: sizes S and S/2 when N > S
+        std::vector<uint32_t> list;
+        std::function<void(uint32_t, uint32_t)> front = [&](uint32_t n, uint32_t off) {
+            if (n <= S) { list.push_back(off << 1 | (n == S ? 0u : 1u)); return; }
+            front(n / 2, off);
+            front(n / 4, off + n / 2);
+            front(n / 4, off + 3 * n / 4);
+        };
+        front((uint32_t)N, 0);
+        std::vector<uint32_t> rel;
+        SubArgs &sa = p->sub;
+        sa = SubArgs{};
+        for (int kind = 0; kind < 2; kind++) {
+            const uint32_t s = kind ? S / 2 : S;
+            sa.lgs[kind] = kind ? lgS - 1 : lgS;
+            if (s < 8) continue;
+            Tree ts(s, 4);
+            for (uint32_t l = 3; l <= ts.lgN && l < (uint32_t)kSubMaxLevel; l++) {
+                sa.rel_first[kind][l] = (uint32_t)rel.size();
+                sa.rel_cnt[kind][l] = (uint32_t)ts.level[l].size();
+                rel.insert(rel.end(), ts.level[l].begin(), ts.level[l].end());
+            }
+        }
+        if ((rc = p->sub_list.upload(list)) || (rc = p->sub_rel.upload(rel))) return rc;
+        sa.lut = p->lut.as<uint32_t>();
+        sa.lut_e = (uint32_t)E;
+        sa.lut_m = (uint32_t)(rows * E);
+        sa.code = mp_code(p->form);
+        sa.sub = p->sub_list.as<uint32_t>();
+        sa.nsub = (uint32_t)list.size();
+        sa.rel = p->sub_rel.as<uint32_t>();
+        for (uint32_t l = 3; l <= lgS && l < (uint32_t)kSubMaxLevel; l++) sa.lut_off[l] = lvl_off[l];
+        sa.nthr = nthr;
+    }
     p->levels.clear();
-    for (uint32_t l = 3; l <= t.lgN; l++) {
+    for (uint32_t l = p->use_sub ? lgS + 1 : 3; l <= t.lgN; l++) {
         LevelArgs a{};
         const uint32_t n = 1u << l, cnt = (uint32_t)t.level[l].size();
         a.lut = p->lut.as<uint32_t>();
@@ -739,7 +790,8 @@ int build_d_plan(mpfft_plan_s *p)
     return MPFFT_OK;
 }
 
-// stages [s0, s1): 0 = leaves, i >= 1 = combine level of node size 2^(i+2)
+// stages [s0, s1): 0 = leaves (and the levels finished in LDS), i >= 1 =
+// the combine levels on HBM, smallest node size first
 // Stages [s0, s1) of the forward transform; with `inverse` (whole range
 // only) the inverse by the reference's swap trick (serial_mp_fft3.c:367-385):
 // out = swap(FFT(swap(in))) / N, swap exchanging real and imaginary parts.
@@ -764,7 +816,7 @@ int run_mp(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t s
     std::vector<LevelArgs> lv(p->levels.begin() + l0, p->levels.begin() + (l1 > l0 ? l1 : l0));
     for (auto &a : lv) a.data = d_out;
     hipError_t e = mp_fft_launch((int)p->L, (int)p->P, la, lv.data(), (int)lv.size(),
-                                 p->batch, st);
+                                 p->batch, st, p->use_sub ? &p->sub : nullptr);
     if (e == hipSuccess && inverse) {
         int32_t lg = 0;
         while ((1UL << lg) < p->N) lg++;

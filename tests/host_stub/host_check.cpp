@@ -13,6 +13,8 @@
 #include <vector>
 
 void stub_fail_malloc_in(int k);      // stub.cpp
+#include <atomic>
+namespace mpfft { extern std::atomic<int> stub_sub_checked, stub_sub_errors; }   // stub.cpp
 
 static int fails = 0;
 #define CHECK(c, ...)                                   \
@@ -219,6 +221,35 @@ static void batches_concurrently()
     c.join();
 }
 
+// the plain form's subtrees finished in LDS (MPFFT_SUB_SIZE): their tables,
+// checked by the stand-in launcher against the recursion, at every
+// transform size and subtree size, at 2, 8, 16 and 32 limbs (a size whose
+// workgroup would not fit the LDS falls back to the level kernels)
+static void subtrees()
+{
+    using mpfft::stub_sub_checked;
+    using mpfft::stub_sub_errors;
+    const int before = stub_sub_checked;
+    for (const char *S : {(const char *)nullptr, "8", "16", "64", "256", "512", "2048"}) {
+        if (S) setenv("MPFFT_SUB_SIZE", S, 1); else unsetenv("MPFFT_SUB_SIZE");
+        for (long prec : {64L, 256L, 512L, 1024L})
+            for (int lg = 3; lg <= (prec >= 512 ? 12 : 16); lg++) {
+                mpfft_plan p;
+                const unsigned long N = 1UL << lg;
+                if (mpfft_plan_create(&p, N, prec, lg % 3 == 0 ? MPFFT_FORM_MP_C2 : MPFFT_FORM_MP, 2)) {
+                    CHECK(false, "plan N=%lu P=%ld", N, prec);
+                    continue;
+                }
+                std::vector<uint32_t> buf(2 * 2 * N * ((prec + 31) / 32 + 1), 0);
+                CHECK(mpfft_execute_limbs_host(p, buf.data(), buf.data()) == 0, "run N=%lu", N);
+                mpfft_plan_destroy(p);
+            }
+    }
+    unsetenv("MPFFT_SUB_SIZE");
+    CHECK(stub_sub_errors == 0, "%d subtree table errors", (int)stub_sub_errors);
+    CHECK(stub_sub_checked > before + 100, "subtree launches checked: %d", stub_sub_checked - before);
+}
+
 int main(int argc, char **argv)
 {
     if (argc > 1 && std::string(argv[1]) == "threads") {
@@ -228,6 +259,7 @@ int main(int argc, char **argv)
     }
     if (argc > 1) return legacy(argv[1]);
     plans();
+    subtrees();
     alloc_failure_then_retry();
     noalloc_stats();
     batch(MPFFT_FORM_MP, {1, 2, 4, 8, 16, 64, 512, 512, 64, 32, 256, 128, 8, 512},

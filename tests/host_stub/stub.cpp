@@ -65,10 +65,63 @@ int limbs_for_precision(long P) { long need = (P + 31) / 32; for (int L : kL) if
 static thread_local volatile uint32_t sink;   // keeps the read-touches (one per thread)
 static void touch_r(const uint32_t *p, size_t n) { uint32_t s = 0; for (size_t i = 0; i < n; i++) s += p[i]; sink = s; }
 static void touch_w(uint32_t *p, size_t n) { for (size_t i = 0; i < n; i++) p[i] = 0x5a5a5a5au; }
-hipError_t mp_fft_launch(int L, int, const LeafArgs &la, const LevelArgs *lv, int nlev, uint32_t batch, hipStream_t) {
+// The subtree kernel's tables against the recursion itself
+// (serial_mp_fft3.c:180-182, restated here): the subtrees tile the N
+// positions, each is a node of the tree (or a size-4 leaf), the nodes of
+// every level up to the subtree size are exactly the subtrees' offsets plus
+// the relative offsets, the level launches start at the next size, and the
+// workgroup fits the LDS.
+std::atomic<int> stub_sub_checked{0}, stub_sub_errors{0};
+static void sub_walk(uint32_t n, uint32_t off, std::vector<std::set<uint32_t>> &lv,
+                     std::set<uint32_t> &leaf4)
+{
+    if (n <= 4) { if (n == 4) leaf4.insert(off); return; }
+    uint32_t l = 0; while ((1u << l) < n) l++;
+    lv[l].insert(off);
+    sub_walk(n / 2, off, lv, leaf4);
+    sub_walk(n / 4, off + n / 2, lv, leaf4);
+    sub_walk(n / 4, off + 3 * n / 4, lv, leaf4);
+}
+static void sub_check(int L, uint32_t N, const SubArgs &sa, const LevelArgs *lv, int nlev)
+{
+    int bad = 0;
+    std::vector<std::set<uint32_t>> glv(40);
+    std::set<uint32_t> leaf4;
+    sub_walk(N, 0, glv, leaf4);
+    std::vector<int> cover(N, 0);
+    std::vector<std::set<uint32_t>> got(40);
+    for (uint32_t i = 0; i < sa.nsub; i++) {
+        const uint32_t kind = sa.sub[i] & 1u, o = sa.sub[i] >> 1, lg = sa.lgs[kind], sz = 1u << lg;
+        if (o + sz > N) { bad++; continue; }
+        for (uint32_t j = 0; j < sz; j++) cover[o + j]++;
+        if (sz == N ? o != 0 : sz >= 8 ? !glv[lg].count(o) : !(sz == 4 && leaf4.count(o))) bad++;
+        for (uint32_t l = 3; l <= lg; l++)
+            for (uint32_t r = 0; r < sa.rel_cnt[kind][l]; r++)
+                got[l].insert(o + sa.rel[sa.rel_first[kind][l] + r]);
+    }
+    for (uint32_t j = 0; j < N; j++) bad += cover[j] != 1;
+    for (uint32_t l = 3; l <= sa.lgs[0]; l++) bad += got[l] != glv[l];
+    if (nlev > 0 && lv[0].n != (2u << sa.lgs[0])) bad++;
+    if (sizeof(uint32_t) * (L + 1) * (2 * (size_t)sa.nthr + (2u << sa.lgs[0])) > 160 * 1024) bad++;
+    if (sa.nthr % 64 || sa.nthr > 256) bad++;
+    stub_sub_checked++;
+    if (bad) { stub_sub_errors++; printf("sub tables: N=%u L=%d S=%u: %d mismatches\n", N, L, 1u << sa.lgs[0], bad); }
+}
+hipError_t mp_fft_launch(int L, int, const LeafArgs &la, const LevelArgs *lv, int nlev, uint32_t batch, hipStream_t, const SubArgs *sub) {
+    if (sub && la.npos) sub_check(L, la.N, *sub, lv, nlev);
     const size_t w = 2 * (size_t)la.N * (L + 1);
     LeafArgs a = la; std::vector<LevelArgs> v(lv, lv + nlev);
+    const bool has_sub = sub != nullptr;
+    SubArgs sa = has_sub ? *sub : SubArgs{};
     run_or_record([=] {
+        // the subtree kernel reads its subtree list and every relative node
+        // offset of its levels
+        if (a.npos && has_sub) {
+            touch_r(sa.sub, sa.nsub);
+            for (int k = 0; k < 2; k++)
+                for (uint32_t l = 3; l <= sa.lgs[k] && l < (uint32_t)kSubMaxLevel; l++)
+                    if (sa.rel_cnt[k][l]) touch_r(sa.rel + sa.rel_first[k][l], sa.rel_cnt[k][l]);
+        }
         for (uint32_t b = 0; b < batch; b++) {
             if (a.npos) { touch_r(a.in + a.in_stride * b, w); touch_w(a.data + a.stride * b, w); touch_w(a.aos + a.aos_stride * b, (size_t)2 * a.N * a.aos_words); }
             for (auto &x : v) { touch_w(x.data + x.stride * b, w); touch_r(x.lut, 1); }

@@ -113,6 +113,48 @@ def test_fft_matches_oracle(n, P, seed):
     assert np.array_equal(run_fft(n, P, x), O.mp_fft(n, P, x))
 
 
+# subtree size S finished in LDS (MPFFT_SUB_SIZE; 0: every level on HBM,
+# None: the default), sizes below, at and above the transform size, both
+# operation orders (CODE 3, CODE 1)
+@pytest.mark.parametrize("n,P,S", [(4096, 256, "0"), (4096, 256, None), (4096, 256, "8"),
+                                   (4096, 256, "16"), (4096, 256, "64"), (4096, 256, "8192"),
+                                   (2048, 512, "256"), (2048, 512, "32"), (1 << 14, 64, "2048"),
+                                   (1024, 200, "128"), (8, 128, "512"), (16, 128, "8"),
+                                   (32, 300, "512"), (1 << 15, 128, "512")])
+def test_subtrees_in_lds_any_size(monkeypatch, n, P, S):
+    if S is None:
+        monkeypatch.delenv("MPFFT_SUB_SIZE", raising=False)
+    else:
+        monkeypatch.setenv("MPFFT_SUB_SIZE", S)
+    x = I.mp_random(n, P, seed=9000 + n + P)
+    print(f"case n={n} P={P} S={S}", flush=True)
+    for code, form in ((3, M.FORM_MP), (1, M.FORM_MP_C1)):
+        assert np.array_equal(run_fft(n, P, x, form=form), O.mp_fft(n, P, x, code=code)), code
+
+
+def test_subtree_stages(monkeypatch):
+    # stage 0 holds the leaves and the levels finished in LDS; the rest are
+    # one combine level each, and running them one by one gives the whole
+    # transform
+    monkeypatch.setenv("MPFFT_SUB_SIZE", "512")
+    n, P = 1 << 14, 256
+    plan = M.Plan(n, P, M.FORM_MP, 2)
+    assert plan.stages() == 1 + (14 - 9)          # levels 2^10 .. 2^14 on HBM
+    x = np.concatenate([I.relimb(I.mp_random(n, P, seed=77 + i), n, O.limbs(P), plan.L)
+                        for i in range(2)])
+    whole = plan.execute_host(x)
+    import torch
+    dx = torch.from_numpy(x.view(np.int32)).cuda()
+    dy = torch.empty_like(dx)
+    st = torch.cuda.current_stream().cuda_stream
+    torch.cuda.synchronize()
+    for s in range(plan.stages()):
+        plan.execute_range(dx.data_ptr(), dy.data_ptr(), s, s + 1, st)
+    torch.cuda.synchronize()
+    assert np.array_equal(dy.cpu().numpy().view(np.uint32), whole)
+    plan.close()
+
+
 @pytest.mark.parametrize("key", ["N65536_P256", "N65536_P1024", "N1048576_P128"])
 def test_fft_large_matches_reference_digest(key, digests):
     n = int(key.split("_")[0][1:]); P = int(key.split("_")[1][1:])
