@@ -185,11 +185,21 @@ int mpfft_execute_limbs(mpfft_plan plan, const uint32_t *d_in, uint32_t *d_out,
  * forms stage i is the i-th radix pass (the first also converts the input,
  * the last rounds the output); for the fixed form on panels stage 0
  * converts the input, the last stage the output, the ones between are the
- * passes.
+ * passes (the four-step plan, mpfft_plan_panel_plan 2: stage i is its i-th
+ * pass, the first also converting the input, the last rounding the output).
  * Running stages [0, mpfft_plan_stages) equals mpfft_execute_limbs. */
 int mpfft_execute_limbs_range(mpfft_plan plan, const uint32_t *d_in, uint32_t *d_out,
                               void *stream, int stage_begin, int stage_end);
 int mpfft_plan_stages(mpfft_plan plan);
+
+/* The pass plan of the fixed form on panels (no counterpart in the
+ * reference): 1 radix passes between an input and an output conversion
+ * (the default), 2 the four-step plan (MPFFT_PANEL_PLAN=fs in the
+ * environment at plan creation, for n = 16^m >= 4096 and at most 1024 bits:
+ * n = n1 256, radix-16 passes over n1 and then over 256 points with the
+ * twiddles w^(k1 n2) between, the input converted in the first pass and the
+ * output in the last), 0 not a panel plan. */
+int mpfft_plan_panel_plan(mpfft_plan plan);
 
 /* Inverse transform, the way the reference's test() computes it
  * (serial_mp_fft3.c:367-385): out = swap(FFT(swap(in))) / N, where swap

@@ -36,6 +36,7 @@ EXPORTS = [
     "mpfft_last_error", "mpfft_strerror", "mpfft_plan_create", "mpfft_plan_destroy",
     "mpfft_plan_info", "mpfft_plan_form", "mpfft_execute_mpfr", "mpfft_execute_limbs",
     "mpfft_execute_limbs_host", "mpfft_execute_limbs_range", "mpfft_plan_stages",
+    "mpfft_plan_panel_plan",
     "mpfft_execute_f64", "mpfft_execute_f64_host",
     "mpfft_plan_sync", "mpfft_mpfr_to_limbs", "mpfft_limbs_to_mpfr",
     "mpfft_mp_binop_host", "mpfft_limbs_for_precision",
@@ -80,6 +81,7 @@ def lib():
     L.mpfft_execute_limbs_host.argtypes = [VP, P, P]
     L.mpfft_execute_limbs_range.argtypes = [VP, P, P, P, I, I]
     L.mpfft_plan_stages.argtypes = [VP]
+    L.mpfft_plan_panel_plan.argtypes = [VP]
     L.mpfft_plan_form.argtypes = [VP]
     L.mpfft_execute_f64.argtypes = [VP, P, P, P]
     L.mpfft_execute_f64_host.argtypes = [VP, P, P]
@@ -176,6 +178,11 @@ class Plan:
 
     def stages(self):
         return int(lib().mpfft_plan_stages(self.h))
+
+    def panel_plan(self):
+        """2: the four-step panel plan (conversions inside the first and
+        last passes), 1: panel passes between conversion stages, 0: none"""
+        return int(lib().mpfft_plan_panel_plan(self.h))
 
     def resolved_form(self):
         """the form the plan runs (FORM_AUTO resolved, FORM_FIXED as its

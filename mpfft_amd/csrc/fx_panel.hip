@@ -1074,6 +1074,387 @@ fxp_out_kernel(FxArgs a, const uint32_t *src)
     }
 }
 
+// ---- four-step plan (kernels.h, fxs_plan) -----------------------------------
+
+// The tiled buffers between the four-step passes (kernels.h, FxsPass): word
+// pair k (k < W/2 real words 2k, 2k+1; then the imaginary ones) of element E
+// of transform t.  A tile of shape (Tt, Te, Se) holds Tt transforms x Te
+// elements Se apart (E = row + Se m, m < Te); slice k of a tile is Tt Te
+// consecutive pairs, transform fastest.  Returns the word offset of pair 0;
+// pair k is k << (lgTt + lgTe + 1) words further.
+template <uint32_t lgTt, uint32_t lgTe, uint32_t lgSe>
+__device__ __forceinline__ size_t tile_at(uint32_t t, uint32_t E, uint32_t N, int W)
+{
+    const uint32_t rowE = ((E >> (lgSe + lgTe)) << lgSe) | (E & ((1u << lgSe) - 1u));
+    const uint32_t m = (E >> lgSe) & ((1u << lgTe) - 1u);
+    const size_t R = (size_t)(t >> lgTt) * (N >> lgTe) + rowE;
+    return ((R * (size_t)W) << (lgTt + lgTe + 1)) +
+           ((size_t)((m << lgTt) | (t & ((1u << lgTt) - 1u))) << 1);
+}
+
+// The shapes of a four-step pass (kernels.h, fxs_plan), fixed per kind:
+// T48 marks a step-1 pass that writes the 4 x 8 tiles the twist pass reads.
+template <uint32_t MODE, bool T48>
+struct FxsShape {
+    static constexpr bool step1 = MODE == kFxsFirst || MODE == kFxsShared;
+    // columns: 2^lgCt transforms x 2^(5 - lgCt) column indices
+    static constexpr uint32_t lgCt = MODE == kFxsTwist ? 5 : (MODE == kFxsLast || T48) ? 2 : 0;
+    static constexpr uint32_t iTt = step1 ? 0 : 2, iTe = step1 ? 5 : 3;
+    static constexpr uint32_t iSe = MODE == kFxsLast ? 8 : 0;
+    static constexpr uint32_t oTt = (MODE == kFxsTwist || T48) ? 2 : 0;
+    static constexpr uint32_t oTe = (MODE == kFxsTwist || T48) ? 3 : 5;
+    static constexpr uint32_t oSe = MODE == kFxsTwist ? 8 : 0;
+    // element strides of the column index u and the position (N2 = 256);
+    // the last pass's output stride of its position is N1 (run time)
+    static constexpr uint32_t ics = step1 ? 1 : 256, ies = step1 ? 256 : 1;
+    static constexpr uint32_t ocs = MODE == kFxsTwist ? 256 : 1;
+    static constexpr uint32_t oes = step1 ? 256 : 1;
+};
+
+// every tile shape of the plan holds Tt Te = 32 pairs per slice: slices are
+// 64 words apart
+constexpr uint32_t kTileSlice = 64;
+
+template <int W>
+__device__ __forceinline__ void tile_get(Fx<W> &re, Fx<W> &im, const uint32_t *p)
+{
+    static_assert(W % 2 == 0, "tiled buffers hold word pairs");
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(p + k * kTileSlice);
+        if (k < W / 2) { re.w[2 * k] = v.x; re.w[2 * k + 1] = v.y; }
+        else { im.w[2 * k - W] = v.x; im.w[2 * k - W + 1] = v.y; }
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void tile_put(uint32_t *p, const Fx<W> &re, const Fx<W> &im)
+{
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const uint2 v = k < W / 2 ? make_uint2(re.w[2 * k], re.w[2 * k + 1])
+                                  : make_uint2(im.w[2 * k - W], im.w[2 * k - W + 1]);
+        *reinterpret_cast<uint2 *>(p + k * kTileSlice) = v;
+    }
+}
+
+// A radix-16 pass of the four-step plan.  The DFT16, the LDS slots and the
+// products are those of fxp_r16_kernel (one item = one butterfly index j of
+// 32 columns; eight waves hold the 16 positions, position w + 8 h in lane
+// half h of wave w; stage A, internal twiddles, stage B); what the mode sets
+// is which butterflies form an item, where their elements are and which
+// twiddle each position takes (kernels.h, FxsPass).  Columns beyond the
+// batch load zeros and store nothing.  first: the loads convert the limb
+// input (block exponent a.eoff); last: the stores round into the limb
+// output (and the next item's loads follow them, so the rounding does not
+// share the registers with them).
+template <int W, int L, uint32_t MODE, bool FULL, bool T48>
+__global__ void __launch_bounds__(512, 1)
+fxs_r16_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, FxsPass ps, uint32_t c16e)
+{
+    using S = FxsShape<MODE, T48>;
+    extern __shared__ uint32_t lds[];
+    constexpr bool EXT = MODE != kFxsFirst;
+    constexpr MfGeom G = mf_geom(W);
+    constexpr uint32_t SLOT = 2 * W * 64, TWN = 3 * G.RW, TWL = (TWN + 63) / 64;
+    constexpr int SETS = kR16Sets(W);
+    const uint32_t N = a.N, M2 = 2 * N, J = 1u << ps.lgJ;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    const uint32_t hl = lane >> 5, col = lane & 31u, h = 2 * wv + hl;
+    uint32_t *const setA = lds, *const setB = lds + (SETS - 1) * 8 * SLOT;
+    uint32_t *const tabE = lds + SETS * 8 * SLOT + wv * 2 * TWN;     // this wave's two tables
+    uint32_t *const w16t = lds + SETS * 8 * SLOT + 16 * TWN;         // the internal tables
+    for (uint32_t i = threadIdx.x; i < 5 * TWN; i += blockDim.x)
+        w16t[i] = a.tw[(size_t)(c16e + 1) * TWN + i];
+    __syncthreads();                                   // constant tables staged
+    uint32_t it = blockIdx.x;
+    const uint32_t nitems = ps.nitems;
+    if (it >= nitems) return;
+    const uint32_t r2 = kR2[h], m1 = kM1[h];
+    uint32_t sa[4], sb[4];                             // word offsets in lds
+#pragma unroll
+    for (uint32_t s_ = 0; s_ < 4; s_++) {
+        const uint32_t r = r2 + 4 * s_;
+        sa[s_] = (r & 7u) * SLOT + 32 * (r >> 3) + col;
+        uint32_t g = 0;
+#pragma unroll
+        for (uint32_t x = 0; x < 16; x++)
+            if (kR2[x] == s_ && kM1[x] == m1) g = x;
+        sb[s_] = (uint32_t)(SETS - 1) * 8 * SLOT + (g >> 1) * SLOT + 32 * (g & 1u) + col;
+    }
+    const uint32_t q = m1 + 4 * r2;                    // stage B output of this lane-half
+    const uint32_t pos = wv + 8 * hl;                  // input position of this lane-half
+    // blocks of 32 columns: Ct transforms x (32 / Ct) column indices u
+    constexpr uint32_t lgCe = 5 - S::lgCt;
+    const uint32_t lgNub = ps.lgU - lgCe, nub = 1u << lgNub;
+    const uint32_t ntg = (a.batch + (1u << S::lgCt) - 1) >> S::lgCt;
+    // item i: butterfly j, column block blk and (twist) the k1 of the item.
+    // Twist order: 32-transform blocks fastest, then j and k1 in 8 x 8 blocks
+    // (the items that read one input tile, j..j+7, and those that write one
+    // output tile, k1..k1+7, run side by side)
+    // Slot i of the persistent loop (i = block + grid n) takes item
+    // xcd_item(i): the grid's blocks are dealt to the 8 XCDs round robin, so
+    // the slots of one XCD in a round take consecutive items, and items that
+    // share cache lines share an L2 (the last, partial round keeps i)
+    const uint32_t ng = gridDim.x, full = (nitems / ng) * ng;
+    auto xcd_item = [&](uint32_t i) {
+        if ((ng & 7u) != 0 || i >= full) return i;
+        const uint32_t b = i % ng;
+        return i - b + (b & 7u) * (ng >> 3) + (b >> 3);
+    };
+    auto item = [&](uint32_t i0, uint32_t &j, uint32_t &blk, uint32_t &ki) {
+        const uint32_t i = xcd_item(i0);
+        if constexpr (MODE == kFxsTwist) {
+            // 4 x 4 blocks of (j, k1) (a 128-byte line of a 4 x 8 tile holds
+            // four j of the input, four k1 of the output), in 8 x 8 blocks
+            blk = i % ntg;
+            const uint32_t r1 = i / ntg;
+            const uint32_t jl = (r1 & 3u) | ((r1 >> 2) & 4u), kl = ((r1 >> 2) & 3u) | ((r1 >> 3) & 4u);
+            const uint32_t r3 = r1 >> 6, nj8 = J >> 3;
+            j = (r3 % nj8) * 8 + jl;
+            ki = (r3 / nj8) * 8 + kl;
+        } else {
+            // column blocks of one transform group fastest, then j: the
+            // items that run side by side sweep whole sub-transforms of a
+            // few transforms (long runs in every buffer)
+            ki = 0;
+            const uint32_t nb = i & (nub - 1u), rest = i >> lgNub;
+            j = rest & (J - 1u);
+            blk = ((rest >> ps.lgJ) << lgNub) | nb;
+        }
+    };
+    // this lane's transform t and column index u; false beyond the batch
+    auto column = [&](uint32_t blk, uint32_t ki, uint32_t &t, uint32_t &u) {
+        if constexpr (MODE == kFxsTwist) {
+            t = 32 * blk + col;
+            u = ki;
+        } else {
+            const uint32_t tg = blk >> lgNub, ub = blk & (nub - 1u);
+            t = (tg << S::lgCt) + (col & ((1u << S::lgCt) - 1u));
+            u = (ub << lgCe) + (col >> S::lgCt);
+        }
+        return t < a.batch;
+    };
+    auto fetch = [&](uint32_t i, Fx<W> &re, Fx<W> &im) {
+        uint32_t j, blk, ki, t, u;
+        item(i, j, blk, ki);
+        if (!column(blk, ki, t, u)) {
+#pragma unroll
+            for (int w = 0; w < W; w++) { re.w[w] = 0u; im.w[w] = 0u; }
+            return;
+        }
+        const uint32_t E = u * S::ics + (j + pos * J) * S::ies;
+        if constexpr (MODE == kFxsFirst) {
+            const uint32_t *b = a.in + a.in_stride * t;
+            const int32_t eo = a.eoff[t];
+            Mp<L> x;
+            mp_load(x, b, M2, E);
+            uint32_t bad = fx_from_mp<L, W>(re, x, a.lim, eo);
+            mp_load(x, b, M2, N + E);
+            bad |= fx_from_mp<L, W>(im, x, a.lim, eo);
+            if (bad) atomicOr(a.flag, 1u);
+        } else {
+            tile_get<W>(re, im, src + tile_at<S::iTt, S::iTe, S::iSe>(t, E, N, W));
+        }
+    };
+    // the wave's two twiddle tables: positions wv and wv + 8
+    auto fetch_tw = [&](uint32_t i, uint32_t (&ta)[TWL], uint32_t (&tb)[TWL]) {
+        uint32_t j, blk, ki;
+        item(i, j, blk, ki);
+        uint32_t ea, eb;
+        if constexpr (MODE == kFxsTwist) {
+            // w^(k1 n2), n2 = j + r J: entry toff + (k1 J + j) 16 + r (the
+            // table in the order the items read it)
+            ea = ps.toff + ((ki * J + j) << 4) + wv;
+            eb = ea + 8;
+        } else {
+            const uint32_t k = j & (ps.Ns - 1u);
+            ea = (wv == 0 || k == 0) ? c16e : ps.toff + 15 * k + wv - 1;
+            eb = k == 0 ? c16e : ps.toff + 15 * k + wv + 7;
+        }
+        const uint32_t *t0 = a.tw + (size_t)ea * TWN, *t1 = a.tw + (size_t)eb * TWN;
+#pragma unroll
+        for (uint32_t u = 0; u < TWL; u++) {
+            ta[u] = (lane + 64 * u < TWN) ? (t0 + 64 * u)[lane] : 0u;
+            tb[u] = (lane + 64 * u < TWN) ? (t1 + 64 * u)[lane] : 0u;
+        }
+    };
+    Fx<W> re, im;
+    uint32_t ta[TWL], tb[TWL];
+    if constexpr (EXT) fetch_tw(it, ta, tb);
+    fetch(it, re, im);
+    for (uint32_t n = 0;; n++) {
+        uint32_t j, blk, ki;
+        item(it, j, blk, ki);
+        const uint32_t nx = it + gridDim.x;
+        uint32_t *const slotA = setA + wv * SLOT, *const slotB = setB + wv * SLOT;
+        // one set: the slots were last read in the previous item's stage B
+        if (SETS == 1 && n > 0) __syncthreads();
+        if constexpr (!EXT) {
+#pragma unroll
+            for (int i = 0; i < W; i++) slotA[i * 64 + lane] = re.w[i];
+#pragma unroll
+            for (int i = 0; i < W; i++) slotA[(W + i) * 64 + lane] = im.w[i];
+        } else {
+#pragma unroll
+            for (uint32_t u = 0; u < TWL; u++)
+                if (lane + 64 * u < TWN) {
+                    tabE[lane + 64 * u] = ta[u];
+                    tabE[TWN + lane + 64 * u] = tb[u];
+                }
+            MfOperand<W> Bx;
+            mf_operand<W>(Bx, re, im);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            mf_cmul<W, false, true, r16_am(W)>(Bx, tabE, tabE + G.RW, tabE + 2 * G.RW, lane, slotA,
+                                          tabE + TWN, tabE + TWN + G.RW, tabE + TWN + 2 * G.RW);
+        }
+        __syncthreads();
+        // stage A
+        Fx<W> yr, yi;
+        dft4_rt<W>(yr, yi, lds + sa[0], lds + sa[1], lds + sa[2], lds + sa[3], m1);
+        if (wv == 7 && hl == 1) {
+            // -i (a + ib) = b - ia
+            Fx<W> z, t_;
+#pragma unroll
+            for (int i = 0; i < W; i++) z.w[i] = 0;
+            fx_sub(t_, z, yr);
+            yr = yi;
+            yi = t_;
+        }
+        if (SETS == 1) __syncthreads();               // stage A's reads of the one set
+        if (wv < 4) {
+            if (wv == 1 && hl == 1) {                  // w16^9 Y = w16^1 (-Y)
+                Fx<W> z;
+#pragma unroll
+                for (int i = 0; i < W; i++) z.w[i] = 0;
+                fx_sub(yr, z, yr);
+                fx_sub(yi, z, yi);
+            }
+            MfOperand<W> B;
+            mf_operand<W>(B, yr, yi);
+            const uint32_t *t0 = w16t + r16_int_table(wv) * TWN;
+            mf_cmul<W, false, false, r16_am(W)>(B, t0, t0 + G.RW, t0 + 2 * G.RW, lane, slotB);
+        } else {
+#pragma unroll
+            for (int i = 0; i < W; i++) slotB[i * 64 + lane] = yr.w[i];
+#pragma unroll
+            for (int i = 0; i < W; i++) slotB[(W + i) * 64 + lane] = yi.w[i];
+        }
+        // the next item's operands and twiddle tables load during stage B
+        // (the last pass: after its stores)
+        if (nx < nitems) {
+            if constexpr (MODE != kFxsLast) fetch(nx, re, im);
+            if constexpr (MODE != kFxsLast && EXT) fetch_tw(nx, ta, tb);
+        }
+        __syncthreads();
+        // stage B: output q = m1 + 4 m2 of this lane-half
+        dft4_rt<W>(yr, yi, lds + sb[0], lds + sb[1], lds + sb[2], lds + sb[3], r2);
+        {
+            uint32_t t, u;
+            if (column(blk, ki, t, u)) {
+                const uint32_t k = j & (ps.Ns - 1u);
+                const uint32_t oes = MODE == kFxsLast ? (N >> 8) : S::oes;
+                const uint32_t E = u * S::ocs + ((j - k) * 16 + k + q * ps.Ns) * oes;
+                if constexpr (MODE == kFxsLast) {
+                    uint32_t *b = a.out + a.out_stride * t;
+                    const int32_t eo = a.eoff[t];
+                    Mp<L> z;
+                    fx_to_mp<W, L, FULL>(z, yr, a.P, eo);
+                    mp_store(z, b, M2, E);
+                    fx_to_mp<W, L, FULL>(z, yi, a.P, eo);
+                    mp_store(z, b, M2, N + E);
+                } else {
+                    tile_put<W>(dst + tile_at<S::oTt, S::oTe, S::oSe>(t, E, N, W), yr, yi);
+                }
+            }
+        }
+        if (nx >= nitems) break;
+        if constexpr (MODE == kFxsLast) {
+            fetch(nx, re, im);
+            fetch_tw(nx, ta, tb);
+        }
+        it = nx;
+    }
+}
+
+template <int L, int W, bool FULL>
+static hipError_t launch_fxs(const FxArgs &a, const FxsPass *ps, int np, uint32_t c16e, int s0,
+                             int s1, hipStream_t st)
+{
+    if constexpr (W > kPanelR16MaxW || W % 2 != 0) {
+        return hipErrorInvalidValue;              // fxs_eligible: even W up to 34
+    } else {
+        static_assert(sizeof(uint32_t) * r16_lds_words<W>() <= 160 * 1024,
+                      "radix-16 pass exceeds the LDS of a CU");
+        const size_t sh = sizeof(uint32_t) * r16_lds_words<W>();
+        // buf1 and buf0 alternate between the passes; the first reads a.in,
+        // the last writes a.out
+        for (int i = 0; i < np; i++) {
+            if (i < s0 || i >= s1) continue;
+            const FxsPass &P = ps[i];
+            const uint32_t *src = (i & 1) ? a.buf0 : a.buf1;
+            uint32_t *dst = (i & 1) ? a.buf1 : a.buf0;
+            const uint32_t nblk = P.nitems < a.ncu ? P.nitems : a.ncu;
+            if (P.nitems == 0) continue;
+            // the pass must have the shapes its kernel is compiled for
+            const bool t48 = P.tout == fxs_tile(2, 3, 0);
+            auto shapes_ok = [&](auto shape) {
+                using Sh = decltype(shape);
+                return P.lgCt == Sh::lgCt && P.ics == Sh::ics && P.ies == Sh::ies &&
+                       P.ocs == Sh::ocs && (P.mode == kFxsLast || P.oes == Sh::oes) &&
+                       (P.mode == kFxsFirst || P.tin == fxs_tile(Sh::iTt, Sh::iTe, Sh::iSe)) &&
+                       (P.mode == kFxsLast || P.tout == fxs_tile(Sh::oTt, Sh::oTe, Sh::oSe)) &&
+                       (P.mode != kFxsLast || P.oes == (a.N >> 8));
+            };
+#define FXS_GO(MD, T)                                                                         \
+    do {                                                                                      \
+        if (!shapes_ok(FxsShape<MD, T>{})) return hipErrorInvalidValue;                       \
+        hipLaunchKernelGGL((fxs_r16_kernel<W, L, MD, FULL, T>), dim3(nblk), dim3(512), sh, st, \
+                           a, src, dst, P, c16e);                                             \
+    } while (0)
+            switch (P.mode) {
+            case kFxsFirst:
+                if (t48) FXS_GO(kFxsFirst, true);
+                else FXS_GO(kFxsFirst, false);
+                break;
+            case kFxsShared:
+                if (t48) FXS_GO(kFxsShared, true);
+                else FXS_GO(kFxsShared, false);
+                break;
+            case kFxsTwist: FXS_GO(kFxsTwist, false); break;
+            default: FXS_GO(kFxsLast, false); break;
+            }
+#undef FXS_GO
+        }
+        return hipGetLastError();
+    }
+}
+
+hipError_t fxs_fft_launch(int L, int P, const FxArgs &a, const FxsPass *ps, int np,
+                          uint32_t c16e, int s0, int s1, hipStream_t st)
+{
+    if (!fxs_eligible(a.N, L + 2) || np < 2 || np > kFxsMaxPasses || ps[0].mode != kFxsFirst ||
+        ps[np - 1].mode != kFxsLast)
+        return hipErrorInvalidValue;
+    const bool full = (P == 32 * L);
+#define CALLS(Lv) (full ? launch_fxs<Lv, Lv + 2, true>(a, ps, np, c16e, s0, s1, st) \
+                        : launch_fxs<Lv, Lv + 2, false>(a, ps, np, c16e, s0, s1, st))
+    switch (L) {
+    case 2: return CALLS(2);
+    case 4: return CALLS(4);
+    case 6: return CALLS(6);
+    case 8: return CALLS(8);
+    case 12: return CALLS(12);
+    case 16: return CALLS(16);
+    case 24: return CALLS(24);
+    case 32: return CALLS(32);
+    default: return hipErrorInvalidValue;
+    }
+#undef CALLS
+}
+
 // ---------------------------------------------------------------------------
 
 template <int L, int W, bool FULL>

@@ -320,6 +320,138 @@ inline int fxp_plan(uint32_t N, int W, int rb, FxpPass (&p)[32], uint32_t *nent)
 // stage per pass, the output conversion (rounding back to floats)
 hipError_t fxp_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st);
 
+// ---- fixed form on panels, four-step plan (fx_panel.hip, fxs_*) ----
+//
+// For N = 16^m (m >= 3) and reals of at most kPanelR16MaxW words: N = N1 N2
+// with N2 = 256, and with x[n1 N2 + n2] the input
+//   step 1: A[k1, n2] = sum_n1 x[n1 N2 + n2] w_N1^(n1 k1)   (held at k1 N2 + n2)
+//   step 2: B[k1, n2] = A[k1, n2] w_N^(k1 n2)
+//   step 3: X[k1 + N1 k2] = sum_n2 B[k1, n2] w_N2^(n2 k2)
+// Every pass is a radix-16 Stockham DIT pass (the radix-16 panel kernel's
+// DFT16) over the sub-transforms of one step.  A work item is one
+// butterfly index j of 32 "columns" that share its twiddles:
+//   first  (step 1, Ns = 1): no twiddles; columns 32 consecutive
+//          (transform, n2); reads the limb input and converts it;
+//   shared (step 1 or 3, Ns > 1): twiddles w_N^(r k N / (16 Ns)), k = j mod
+//          Ns, the same for every column: columns 32 consecutive
+//          (transform, n2) in step 1, (transform, k1) in step 3;
+//   twist  (step 3, Ns = 1): the twiddles of step 2, w_N^(k1 n2) with
+//          n2 = j + r N2/16, one per (k1, n2): columns 32 transforms;
+//   last   (step 3's last pass): as shared, and rounds the output into the
+//          limb layout (X[k1 + N1 k2]: 32 consecutive k1 are 32 consecutive
+//          elements).
+// So the input and output conversions are the first and last passes' loads
+// and stores.  Columns are Ct consecutive transforms times 32/Ct consecutive
+// column indices u (n2 in step 1, k1 in step 3; twist: 32 transforms of one
+// k1), and the buffers between passes are tiled to match: a tile holds
+// Tt transforms x Te elements that are Se apart, as W word-pair slices of
+// Tt Te = 32 pairs (256 bytes), transform fastest inside a slice.
+// A half-wave whose columns match the tile moves whole 256-byte slices; the
+// two transposing transitions (step 1 -> twist -> last) use 4 x 8 tiles, so
+// the twist pass moves eight 32-byte runs per slice on either side:
+//   first, shared (step 1): columns 1 x 32, tiles 1 x 32 (the pass before
+//     the twist: columns 4 x 8, writing 4 x 8 tiles over n2);
+//   twist: columns 32 x 1, reading 4 x 8 tiles over n2, writing 4 x 8 tiles
+//     over k1 (Se = N2); its items in 8 x 8 blocks of (j, k1), so the items
+//     that share a tile run side by side;
+//   last: columns 4 x 8 (8 consecutive k1: 32-byte runs of the limb output).
+// Element addresses: input u ics + i ies, output u ocs + i' oes, with i =
+// j + r M/16 the position read and i' = (j - k) 16 + k + q Ns the one written.
+enum : uint32_t { kFxsFirst = 0, kFxsShared = 1, kFxsTwist = 2, kFxsLast = 3 };
+
+// a tile shape: log2 Tt | log2 Te << 8 | log2 Se << 16
+constexpr uint32_t fxs_tile(uint32_t lgTt, uint32_t lgTe, uint32_t lgSe)
+{
+    return lgTt | lgTe << 8 | lgSe << 16;
+}
+
+struct FxsPass {
+    uint32_t mode;
+    uint32_t Ns;                // sub-transform size before the pass (1: none)
+    uint32_t toff;              // shared, last: entry of (k, r) = toff + 15 k + r - 1;
+                                // twist: entry of (k1, j, r) = toff + (16 k1 + j) 16 + r,
+                                // w^(k1 (j + 16 r))
+    uint32_t lgJ;               // log2 of the butterflies per sub-transform (M / 16)
+    uint32_t lgU;               // log2 of the column indices u per transform (N2 or N1)
+    uint32_t lgCt;              // log2 of the transforms in a block of 32 columns
+    uint32_t ics, ies, ocs, oes;
+    uint32_t tin, tout;         // tile shapes of the input and output buffers
+    uint32_t order;             // 0: column block fastest, then j; 1: the twist order
+    uint32_t nitems;
+};
+
+constexpr int kFxsMaxPasses = 8;
+
+inline bool fxs_eligible(uint32_t N, int W)
+{
+    int lg = 0;
+    while ((1u << lg) < N) lg++;
+    return W <= kPanelR16MaxW && W % 2 == 0 && (1u << lg) == N && lg % 4 == 0 && lg >= 12 &&
+           lg <= 24;
+}
+
+// The four-step plan for N (fxs_eligible) and a batch: the passes and the
+// table entries they use (*nent; the kPanelConstEntries constants follow)
+inline int fxs_plan(uint32_t N, uint32_t batch, FxsPass (&p)[kFxsMaxPasses], uint32_t *nent)
+{
+    const uint32_t xt = 2, xe = 3;           // the transposing tiles: 4 x 8
+    int lg = 0;
+    while ((1u << lg) < N) lg++;
+    const uint32_t N2 = 256, N1 = N / N2, lg1 = (uint32_t)lg - 8;
+    int n = 0;
+    uint32_t toff = 0;
+    auto blocks = [&](uint32_t lgCt, uint32_t lgU) {     // blocks of 32 columns
+        const uint32_t Ct = 1u << lgCt;
+        return ((batch + Ct - 1) / Ct) * ((1u << lgU) / (32 / Ct));
+    };
+    // step 1: N1 = 16^a points over n1, columns (transform, n2)
+    for (uint32_t Ns = 1; Ns < N1; Ns *= 16) {
+        const bool before_twist = Ns * 16 == N1;
+        FxsPass q{};
+        q.mode = Ns == 1 ? kFxsFirst : kFxsShared;
+        q.Ns = Ns; q.toff = toff; q.lgJ = lg1 - 4; q.lgU = 8;
+        q.lgCt = before_twist ? 2 : 0;
+        q.ics = 1; q.ies = N2; q.ocs = 1; q.oes = N2;
+        q.tin = Ns == 1 ? 0u : fxs_tile(0, 5, 0);
+        q.tout = before_twist ? fxs_tile(xt, xe, 0) : fxs_tile(0, 5, 0);
+        q.order = 0;
+        q.nitems = (N1 / 16) * blocks(q.lgCt, q.lgU);
+        if (Ns > 1) toff += 15 * Ns;
+        p[n++] = q;
+    }
+    // step 3 with the step-2 twiddles in its first pass: 256 points over n2
+    {
+        FxsPass q{};
+        q.mode = kFxsTwist;
+        q.Ns = 1; q.toff = toff; q.lgJ = 4; q.lgU = lg1; q.lgCt = 5;
+        q.ics = N2; q.ies = 1; q.ocs = N2; q.oes = 1;
+        q.tin = fxs_tile(xt, xe, 0); q.tout = fxs_tile(xt, xe, 8);
+        q.order = 1;
+        q.nitems = ((batch + 31) / 32) * N1 * 16;
+        toff += N;
+        p[n++] = q;
+    }
+    {
+        FxsPass q{};
+        q.mode = kFxsLast;
+        q.Ns = 16; q.toff = toff; q.lgJ = 4; q.lgU = lg1; q.lgCt = 2;
+        q.ics = N2; q.ies = 1; q.ocs = 1; q.oes = N1;
+        q.tin = fxs_tile(xt, xe, 8); q.tout = 0;
+        q.order = 0;
+        q.nitems = 16 * blocks(q.lgCt, q.lgU);
+        toff += 15 * 16;
+        p[n++] = q;
+    }
+    *nent = toff;
+    return n;
+}
+
+// passes [s0, s1) of the four-step plan (pass 0 reads a.in and needs the
+// block exponents of fx_eoff_launch; the last writes a.out); c16e: the
+// table entry of w^0 = 1, the five internal twiddles of the DFT16 after it
+hipError_t fxs_fft_launch(int L, int P, const FxArgs &a, const FxsPass *ps, int np,
+                          uint32_t c16e, int s0, int s1, hipStream_t st);
+
 // ---- double- and quad-word forms (dd_fft.hip) ----
 //
 // Same arguments as the fixed form, for K = 2 or 4 binary64 words per real:

@@ -345,6 +345,9 @@ struct mpfft_plan_s {
     bool fx_panel = false;   // panel kernels (matrix-core twiddle products)
     int fx_prb = 2;          // panel kernels: log2 of the main radix (fxp_plan)
     int fx_npass = 0;        // passes of the panel plan
+    bool fx_fs = false;      // panel kernels: the four-step plan (fxs_plan)
+    FxsPass fxs[kFxsMaxPasses] = {};
+    uint32_t fxs_c16e = 0;   // its table entry of w^0 = 1
     uint32_t ncu = 0;        // compute units of the plan's device
     // double forms
     DevBuf tw, dleaf_sn, dleaf_off, dleaf_n, dlvl_off, dlvl_begin;
@@ -515,22 +518,53 @@ int build_fx_plan(mpfft_plan_s *p)
         // from the environment where the width allows it (for timing the
         // alternatives)
         p->fx_prb = fxp_radix_default((int)W);
+        bool radix_set = false;
         if (const char *e = getenv("MPFFT_PANEL_RADIX")) {
             const int r = atoi(e), rb = r == 16 ? 4 : r == 8 ? 3 : r == 4 ? 2 : 0;
-            if (rb && fxp_radix_ok((int)W, rb)) p->fx_prb = rb;
+            if (rb && fxp_radix_ok((int)W, rb)) { p->fx_prb = rb; radix_set = true; }
         }
-        FxpPass ps[32];
+        // the four-step plan (its first and last passes convert the input
+        // and output) where N = 16^m, m >= 3, when MPFFT_PANEL_PLAN=fs asks
+        // for it: measured on MI355X at N = 2^16, batch 64, it ties the plan
+        // with conversion passes (1024 bits 14.4 k against 14.4-14.9 k FFT/s,
+        // 256 bits 49.4 k against 50.0 k): its converting passes save what
+        // the conversions cost, and the transposition between its steps
+        // costs about as much again
+        const char *pe = getenv("MPFFT_PANEL_PLAN");
+        p->fx_fs = fxs_eligible((uint32_t)N, (int)W) && !radix_set &&
+                   (pe && strcmp(pe, "fs") == 0);
         uint32_t nent = 0;
-        const int np = fxp_plan((uint32_t)N, (int)W, p->fx_prb, ps, &nent);
-        for (int i = 0; i < np; i++)
-            for (unsigned long k = 0; k < ps[i].Ns; k++)
-                for (unsigned long r = 1; r < ps[i].R; r++)
-                    ent.push_back(r * k * (N / ((unsigned long)ps[i].R * ps[i].Ns)));
+        if (p->fx_fs) {
+            const int np = fxs_plan((uint32_t)N, p->batch, p->fxs, &nent);
+            for (int i = 0; i < np; i++) {
+                const FxsPass &q = p->fxs[i];
+                if (q.mode == kFxsTwist) {
+                    // w^(k1 n2), n2 = j + 16 r, in the order (k1, j, r)
+                    for (unsigned long k1 = 0; k1 < N / 256; k1++)
+                        for (unsigned long j = 0; j < 16; j++)
+                            for (unsigned long r = 0; r < 16; r++)
+                                ent.push_back((k1 * (j + 16 * r)) % N);
+                } else if (q.mode != kFxsFirst) {
+                    for (unsigned long k = 0; k < q.Ns; k++)
+                        for (unsigned long r = 1; r < 16; r++)
+                            ent.push_back(r * k * (N / (16UL * q.Ns)));
+                }
+            }
+            p->fx_npass = np;
+            p->fxs_c16e = nent + 1;
+        } else {
+            FxpPass ps[32];
+            const int np = fxp_plan((uint32_t)N, (int)W, p->fx_prb, ps, &nent);
+            for (int i = 0; i < np; i++)
+                for (unsigned long k = 0; k < ps[i].Ns; k++)
+                    for (unsigned long r = 1; r < ps[i].R; r++)
+                        ent.push_back(r * k * (N / ((unsigned long)ps[i].R * ps[i].Ns)));
+            p->fx_npass = np;
+        }
         if (ent.size() != nent) return MPFFT_ERR_ARG;
         // the butterflies' constants (kPanelConstEntries of them, fxp_plan)
         for (unsigned long t : {N / 8, 0UL, N / 16, 3 * N / 16, 9 * N / 16, N / 8, 3 * N / 8})
             ent.push_back(t);
-        p->fx_npass = np;
     } else {
         for (unsigned long Ns = (lg0 & 1) ? 2 : 1; Ns < N; Ns *= 4)
             for (unsigned long k = 0; k < Ns; k++)
@@ -729,6 +763,8 @@ int run_fx(mpfft_plan_s *p, const uint32_t *d_in, uint32_t *d_out, hipStream_t s
     // stage 0 reads the input: the block exponents come first
     if (s0 == 0 && s1 > 0 && fx_eoff_launch(a, st) != hipSuccess) return MPFFT_ERR_HIP;
     hipError_t e = K            ? dd_fft_launch(K, (int)p->L, (int)p->P, a, s0, s1, st)
+                   : p->fx_fs    ? fxs_fft_launch((int)p->L, (int)p->P, a, p->fxs, p->fx_npass,
+                                                  p->fxs_c16e, s0, s1, st)
                    : p->fx_panel ? fxp_fft_launch((int)p->L, (int)p->P, a, s0, s1, st)
                                  : fx_fft_launch((int)p->L, (int)p->P, a, s0, s1, st);
     return e == hipSuccess ? MPFFT_OK : MPFFT_ERR_HIP;
@@ -1162,12 +1198,19 @@ int mpfft_plan_form(mpfft_plan p)
     return p->form;
 }
 
+int mpfft_plan_panel_plan(mpfft_plan p)
+{
+    if (!p) return MPFFT_ERR_ARG;
+    if (!is_fixed(p->form) || !p->fx_panel) return 0;
+    return p->fx_fs ? 2 : 1;
+}
+
 int mpfft_plan_stages(mpfft_plan p)
 {
     if (!p) return MPFFT_ERR_ARG;
     if (word_count(p->form)) return p->N == 1 ? 1 : p->fx_npass;
     if (is_fixed(p->form))
-        return p->fx_panel ? p->fx_npass + 2 : fx_stages((uint32_t)p->N);
+        return p->fx_fs ? p->fx_npass : p->fx_panel ? p->fx_npass + 2 : fx_stages((uint32_t)p->N);
     return is_mp(p->form) ? 1 + (int)p->levels.size() : MPFFT_ERR_FORM;
 }
 

@@ -157,6 +157,10 @@ hipError_t fx_fft_launch(int L, int, const FxArgs &a, int, int, hipStream_t) { F
 hipError_t fxp_fft_launch(int L, int, const FxArgs &a, int, int, hipStream_t) {
     if (!fxp_eligible(a.N, a.batch)) return hipErrorInvalidValue;
     FxArgs c = a; run_or_record([=] { fx_touch(L, c, true); }); return hipSuccess; }
+hipError_t fxs_fft_launch(int L, int, const FxArgs &a, const FxsPass *, int np, uint32_t, int,
+                          int, hipStream_t) {
+    if (!fxs_eligible(a.N, L + 2) || np < 2 || np > kFxsMaxPasses) return hipErrorInvalidValue;
+    FxArgs c = a; run_or_record([=] { fx_touch(L, c, true); }); return hipSuccess; }
 hipError_t dd_fft_launch(int K, int L, int, const FxArgs &a, int, int, hipStream_t) {
     FxArgs c = a;
     run_or_record([=] {

@@ -166,7 +166,7 @@ def test_panel_large_n():
             h[nz] -= np.int32(2 * sh)
             xs.append(I.join(h, lm))
     plan = M.Plan(n, P, M.FORM_FIXED, 64)
-    assert plan.stages() == 1 + 5 + 1
+    assert plan.panel_plan() == 1 and plan.stages() == 1 + 5 + 1
     plan.close()
     yp = run(n, P, xs, M.FORM_FIXED)
     yl = run(n, P, xs, M.FORM_FIXED_LANE)
@@ -285,3 +285,103 @@ def test_panel_other_radixes(monkeypatch, rad, n, P):
     yl = run(n, P, xs, M.FORM_FIXED_LANE)
     check_against_lane(n, P, yp, yl)
     check_against_oracle(n, P, xs[5], yp[5])
+
+
+# ---- four-step plan (MPFFT_PANEL_PLAN=fs; n = 16^m >= 4096, up to 34 words) --
+#
+# n = n1 256: radix-16 passes over n1 (columns 32 consecutive (transform, n2);
+# the first converts the limb input), the twiddles w^(k1 n2) in the first
+# pass over 256 points (columns 32 transforms), its last pass rounding into
+# the limb output; tiled buffers between the passes.  The same contract and
+# the same cross-check against the per-lane kernels as above; batches that
+# leave a block of 32 columns or transforms partial; the default plan (with
+# conversion passes) against it.
+
+@pytest.fixture
+def four_step(monkeypatch):
+    monkeypatch.setenv("MPFFT_PANEL_PLAN", "fs")
+
+
+@pytest.mark.parametrize("n,P,b", [(4096, 1024, 64), (4096, 256, 70), (4096, 53, 3),
+                                   (4096, 128, 33), (1 << 16, 256, 40), (4096, 160, 64),
+                                   (4096, 768, 32), (1 << 16, 1024, 64)])
+def test_four_step_plan(four_step, n, P, b):
+    print(f"case four-step n={n} P={P} batch={b}", flush=True)
+    plan = M.Plan(n, P, M.FORM_FIXED_PANEL, b)
+    lg = n.bit_length() - 1
+    assert plan.panel_plan() == 2 and plan.stages() == lg // 4
+    plan.close()
+    xs = inputs(n, P, b, seed=12000 + n + P + b)
+    yp = run(n, P, xs, M.FORM_FIXED_PANEL)
+    yl = run(n, P, xs, M.FORM_FIXED_LANE)
+    check_against_lane(n, P, yp, yl)
+    for i in sorted({0, b // 2, b - 1}):
+        check_against_oracle(n, P, xs[i], yp[i])
+
+
+def test_four_step_large_n(four_step):
+    # n = 2^20: three passes over n1 = 4096, then the twist and the last pass
+    n, P, b = 1 << 20, 128, 32
+    print(f"case four-step n={n} P={P} batch={b}", flush=True)
+    plan = M.Plan(n, P, M.FORM_FIXED_PANEL, b)
+    assert plan.panel_plan() == 2 and plan.stages() == 5
+    plan.close()
+    xs = inputs(n, P, b, seed=12077)
+    yp = run(n, P, xs, M.FORM_FIXED_PANEL)
+    yl = run(n, P, xs, M.FORM_FIXED_LANE)
+    check_against_lane(n, P, yp, yl)
+    check_against_oracle(n, P, xs[31], yp[31])
+
+
+def test_four_step_stage_ranges(four_step):
+    # its stages run one at a time give the whole transform, bit for bit
+    import torch
+    n, P, b = 1 << 16, 128, 35
+    xs = inputs(n, P, b, seed=79)
+    plan = M.Plan(n, P, M.FORM_FIXED_PANEL, b)
+    assert plan.panel_plan() == 2
+    x = np.concatenate([I.relimb(t, n, O.limbs(P), plan.L) for t in xs])
+    S = plan.stages()
+    assert S == 4
+    dx = torch.from_numpy(x.view(np.int32)).cuda()
+    d1, d2 = torch.empty_like(dx), torch.empty_like(dx)
+    plan.execute_device(dx.data_ptr(), d1.data_ptr())
+    for s in range(S):
+        plan.execute_range(dx.data_ptr(), d2.data_ptr(), s, s + 1)
+    plan.check()
+    assert torch.equal(d1, d2)
+    plan.close()
+    y = np.split(d1.cpu().numpy().view(np.uint32), b)
+    for i in (0, 34):
+        check_against_oracle(n, P, xs[i], I.relimb(y[i], n, M.limbs_for_precision(P), O.limbs(P)))
+
+
+def test_four_step_against_default_plan(monkeypatch):
+    # the default plan (conversion passes) and the four-step plan on the same
+    # inputs: both within the contract, and within 2 ulp of each other
+    n, P, b = 4096, 1024, 64
+    xs = inputs(n, P, b, seed=81)
+    plan = M.Plan(n, P, M.FORM_FIXED_PANEL, b)
+    assert plan.panel_plan() == 1 and plan.stages() == 1 + 3 + 1
+    plan.close()
+    yd = run(n, P, xs, M.FORM_FIXED_PANEL)
+    monkeypatch.setenv("MPFFT_PANEL_PLAN", "fs")
+    y4 = run(n, P, xs, M.FORM_FIXED_PANEL)
+    check_against_lane(n, P, y4, yd)
+    for i in (0, 63):
+        check_against_oracle(n, P, xs[i], y4[i])
+        check_against_oracle(n, P, xs[i], yd[i])
+
+
+def test_four_step_input_range_checked(four_step):
+    # the first pass converts the input: an exponent beyond the limb
+    # format's range is flagged there
+    n, P = 4096, 256
+    xs = inputs(n, P, 32, seed=15)
+    h, _ = I.split(xs[20], n, O.limbs(P))
+    h[4000] = np.int32(((1 << 29) << 1) | (h[4000] & 1))
+    with pytest.raises(M.MpfftError) as e:
+        run(n, P, xs, M.FORM_FIXED_PANEL)
+    assert e.value.code == -3
+    y = run(n, P, inputs(n, P, 32, seed=16), M.FORM_FIXED_PANEL)
+    assert len(y) == 32
