@@ -260,7 +260,9 @@ int mpfft_plan_check(mpfft_plan plan);
  * group's batch; the groups run on up to four streams inside one HIP graph,
  * captured once per (d_in, d_out) pair and replayed by every execute.  With
  * MPFFT_FORM_FIXED a group runs on the panel or the per-lane kernels as an
- * MPFFT_FORM_FIXED plan of its size and count would.  The
+ * MPFFT_FORM_FIXED plan of its size and count would, except that a group of
+ * more than 64 whose last panel would be mostly padding runs its full
+ * panels on the panel kernels and the rest on the per-lane kernels.  The
  * autotuner times every group alone, deals the groups over 1, 2 and 4
  * streams by those times and keeps the fastest graph.
  * (No counterpart in the reference, which runs one transform per call.) */
@@ -270,7 +272,8 @@ typedef struct mpfft_batch_s *mpfft_batch;
  * bits.  form: MPFFT_FORM_MP, _MP_C1, _MP_C2 (the reference's op orders;
  * every transform bit-identical to a plan of that form), MPFFT_FORM_FIXED
  * (each group on the panel or the per-lane kernels by the rule of a fixed-
- * form plan of its size and count), MPFFT_FORM_FIXED_LANE or _FIXED_PANEL
+ * form plan of its size and count, full panels split off as above),
+ * MPFFT_FORM_FIXED_LANE or _FIXED_PANEL
  * (one kernel family; groups with n < 16 run on the per-lane kernels),
  * MPFFT_FORM_DD or _QD, or MPFFT_FORM_AUTO (per group, the resolution of a
  * plan of that form).  Builds every group's plans (twiddle tables) before

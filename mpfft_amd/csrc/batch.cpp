@@ -11,7 +11,9 @@
 // replayed: one launch per execute, independent groups overlapping on the
 // device.  A fixed-form group takes the panel or the per-lane kernels by the
 // rule MPFFT_FORM_FIXED plans use (mpfft::fxp_preferred, from measured
-// crossovers); the autotuner times every group on the caller's
+// crossovers), a class of more than one panel whose last panel that rule
+// rejects as mostly padding being split into its full panels (panel kernels)
+// and the rest (per-lane kernels); the autotuner times every group on the caller's
 // buffers, then the graph with the groups dealt over 1, 2 and 4 streams, and
 // keeps the fastest.  Built only on the public plan API.
 #include "../../include/mpfft.h"
@@ -232,31 +234,48 @@ int mpfft_batch_create(mpfft_batch *batch, unsigned int count, const unsigned lo
     b->grp_of.resize(count);
     b->rank_of.resize(count);
     size_t off = 0;
-    for (auto &kv : key) {
+    // one group: transforms [r0, r1) of the (n, prec) class kv, its plan form
+    auto add_group = [&](const std::pair<const std::pair<unsigned long, long>,
+                                         std::vector<unsigned>> &kv,
+                         size_t r0, size_t r1, int gform) {
         Group g;
         g.N = kv.first.first;
         g.P = kv.first.second;
         g.L = mpfft_limbs_for_precision(g.P);
         g.words = 2 * g.N * (size_t)(g.L + 1);
         g.off = off;
-        g.idx = kv.second;
+        g.idx.assign(kv.second.begin() + (long)r0, kv.second.begin() + (long)r1);
         for (size_t r = 0; r < g.idx.size(); r++) {
             b->grp_of[g.idx[r]] = (unsigned)b->groups.size();
             b->rank_of[g.idx[r]] = (unsigned)r;
         }
         off += g.words * g.idx.size();
-        const uint32_t cnt = (uint32_t)g.idx.size();
+        g.form = gform;
+        b->groups.push_back(std::move(g));
+    };
+    for (auto &kv : key) {
+        const uint32_t cnt = (uint32_t)kv.second.size();
+        const uint32_t N = (uint32_t)kv.first.first;
+        const int W = (int)mpfft_limbs_for_precision(kv.first.second) + 2;
         // MPFFT_FORM_AUTO goes to the plan as it is (mpfft_plan_create
         // resolves it, with the group's count as the batch)
-        if (form == MPFFT_FORM_FIXED)
-            g.form = mpfft::fxp_preferred((uint32_t)g.N, cnt, (int)g.L + 2)
-                         ? MPFFT_FORM_FIXED_PANEL
-                         : MPFFT_FORM_FIXED_LANE;
-        else if (form == MPFFT_FORM_FIXED_PANEL && !mpfft::fxp_eligible((uint32_t)g.N, cnt))
-            g.form = MPFFT_FORM_FIXED_LANE;
-        else
-            g.form = form;
-        b->groups.push_back(std::move(g));
+        if (form == MPFFT_FORM_FIXED) {
+            if (mpfft::fxp_preferred(N, cnt, W)) {
+                add_group(kv, 0, cnt, MPFFT_FORM_FIXED_PANEL);
+            } else if (cnt > mpfft::kPanel && mpfft::fxp_eligible(N, cnt)) {
+                // a class whose last panel would be mostly padding: its full
+                // panels on the panel kernels, the rest on the per-lane ones
+                const uint32_t full = cnt / mpfft::kPanel * mpfft::kPanel;
+                add_group(kv, 0, full, MPFFT_FORM_FIXED_PANEL);
+                add_group(kv, full, cnt, MPFFT_FORM_FIXED_LANE);
+            } else {
+                add_group(kv, 0, cnt, MPFFT_FORM_FIXED_LANE);
+            }
+        } else if (form == MPFFT_FORM_FIXED_PANEL && !mpfft::fxp_eligible(N, cnt)) {
+            add_group(kv, 0, cnt, MPFFT_FORM_FIXED_LANE);
+        } else {
+            add_group(kv, 0, cnt, form);
+        }
     }
     b->words = off;
     HIP_TRY(hipGetDevice(&b->device));

@@ -140,6 +140,43 @@ def test_batch_fixed_small_sizes():
     b.close()
 
 
+def test_batch_fixed_splits_partial_panels():
+    # classes of 65..127 transforms, whose second panel would be mostly
+    # padding: the full panel on the panel kernels, the rest on the per-lane
+    # kernels (in input order); a class of 64 and one of 40 stay whole.
+    # Every transform within 2 ulp of the per-lane kernels, some against the
+    # 2P-bit oracle
+    rng = np.random.default_rng(44)
+    cls = [(256, 256, 80), (64, 128, 101), (512, 512, 64), (128, 1024, 40)]
+    ns = np.concatenate([np.full(c, n) for n, _, c in cls]).astype(np.int64)
+    Ps = np.concatenate([np.full(c, P) for _, P, c in cls]).astype(np.int64)
+    perm = rng.permutation(len(ns))                  # interleaved in the input
+    ns, Ps = ns[perm], Ps[perm]
+    xs = [I.mp_random(int(n), int(P), seed=2100 + i) for i, (n, P) in enumerate(zip(ns, Ps))]
+    b = M.Batch(ns, Ps, M.FORM_FIXED)
+    ys = run_batch(b, xs)
+    forms = b.forms()
+    for n, P, c in cls:
+        idx = [i for i in range(len(ns)) if ns[i] == n and Ps[i] == P]   # input order
+        f = forms[idx]
+        W = M.limbs_for_precision(P) + 2
+        whole = 4 * (-c % 64) <= c or (W >= 34 and n <= 1024)     # the plan rule
+        if whole:
+            assert (f == M.FORM_FIXED_PANEL).all(), (n, P, c)
+        else:
+            assert c > 64
+            assert (f[:64] == M.FORM_FIXED_PANEL).all() and (f[64:] == M.FORM_FIXED_LANE).all(), \
+                (n, P, c)
+    bl = M.Batch(ns, Ps, M.FORM_FIXED_LANE)
+    yl = run_batch(bl, xs)
+    for i in range(len(xs)):
+        lane_check(int(ns[i]), int(Ps[i]), ys[i], yl[i])
+    for i in range(0, len(xs), 37):
+        oracle_check(int(ns[i]), int(Ps[i]), xs[i], ys[i])
+    b.close()
+    bl.close()
+
+
 def test_fixed_plan_partial_panel():
     # MPFFT_FORM_FIXED takes the panels at 120 of 128 (padding 8 <= 120 / 8)
     n, P, B = 1024, 128, 120
