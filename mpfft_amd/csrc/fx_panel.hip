@@ -849,13 +849,24 @@ __device__ __forceinline__ uint32_t r16_int_table(uint32_t w)
     return w == 0 ? 1u : w == 1 ? 0u : w + 1;
 }
 
-constexpr int kR16Sets(int W) { return W <= 18 ? 2 : 1; }
+// blocks per CU of the radix-16 pass: two up to FXP_R16_OCC2_MAXW words
+// per real (one slot set each, 128 registers per wave, four waves per SIMD),
+// else one.  Measured at N = 2^16, batch 64: at 6 words (128 bits) the pass
+// with products takes 109-111 us against 133 us at one block, the first
+// 84-86 against 99; at 10 words the product spills 35 registers at 128 and
+// the pass takes 312 us against 249
+#ifndef FXP_R16_OCC2_MAXW
+#define FXP_R16_OCC2_MAXW 8
+#endif
+constexpr int kR16Occ(int W) { return W <= FXP_R16_OCC2_MAXW ? 2 : 1; }
+
+constexpr int kR16Sets(int W) { return (W <= 18 && kR16Occ(W) == 1) ? 2 : 1; }
 
 // A-tile sourcing of the radix-16 products (mf_cmul's AM): the diagonal
 // cache up to 18 words per real (measured 6 % faster passes at W = 18); at
 // wider reals it does not fit beside the operands in 256 registers (the
 // spills cost more than the LDS reads saved), nor do the other forms
-constexpr int r16_am(int W) { return W <= 18 ? 1 : 0; }
+constexpr int r16_am(int W) { return (W <= 18 && kR16Occ(W) == 1) ? 1 : 0; }
 
 // LDS words of the radix-16 pass: the slot sets (eight slots of 2 W words
 // x 64 lanes each), two external twiddle tables per wave, and the five
@@ -879,7 +890,7 @@ constexpr uint32_t r16_lds_words()
 // write, stage B reads, all in the one set); two sets: two barriers (stage A
 // reads set A, the internal products write set B, stage B reads set B).
 template <int W, bool EXT>
-__global__ void __launch_bounds__(512, 1)
+__global__ void __launch_bounds__(512, 2 * kR16Occ(W))
 fxp_r16_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t toff,
                uint32_t nitems, uint32_t c16e)
 {
@@ -1493,7 +1504,8 @@ static hipError_t launch_fxp(const FxArgs &a, int s0, int s1, hipStream_t st)
                                   "radix-16 pass exceeds the LDS of a CU");
                     const size_t sh = sizeof(uint32_t) * r16_lds_words<W>();
                     const uint32_t nitems = (N / 16) * NG * 2;
-                    const uint32_t nblk = nitems < a.ncu ? nitems : a.ncu;
+                    const uint32_t cap = kR16Occ(W) * a.ncu;
+                    const uint32_t nblk = nitems < cap ? nitems : cap;
                     if (P.Ns == 1)
                         hipLaunchKernelGGL((fxp_r16_kernel<W, false>), dim3(nblk), dim3(512), sh,
                                            st, a, (const uint32_t *)src, dst, P.Ns, P.toff,
