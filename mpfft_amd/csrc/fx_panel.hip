@@ -307,7 +307,9 @@ __device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *R
     constexpr int KT = G.KT, MT = G.MT, ND = G.DHI - G.DLO + 1;
     // |S| <= 2 (4W - 3) 2^14 (two products of balanced digits): the 32-bit
     // form of the carry step below holds up to W = 34; wider reals take
-    // 64-bit sums and keep the output-tile loop rolled
+    // 64-bit sums (the output-tile loop is unrolled at every width: at
+    // W = 66 the radix-4 pass takes 1.62 ms against 2.08 ms rolled, N = 2^16,
+    // batch 64, with 256 VGPRs and about 200 AGPRs and no spill)
     constexpr bool kNarrow = 257LL * 2 * (4 * W - 3) * 16384 + (1LL << 16) < (1LL << 31);
     constexpr int kAM = kNarrow ? AM : 0;
     constexpr int NTAB = REAL ? 1 : TWO ? 4 : 2;
@@ -361,7 +363,7 @@ __device__ __forceinline__ void mf_cmul(const MfOperand<W> &B, const uint32_t *R
             constexpr int f = mf_next(W, 0, -1);
             fetch(rc, f / KT - f % KT - G.DLO);
         }
-#pragma unroll (kNarrow ? MT : 1)
+#pragma unroll (MT)
         for (int p = 0; p < MT; p++) {
             if constexpr (kAM == 1) {
 #pragma unroll
