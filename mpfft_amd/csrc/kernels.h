@@ -92,6 +92,12 @@ hipError_t mp_err_launch(int L, int P, const uint32_t *x, const uint32_t *y, uin
 hipError_t mp_binop_launch(int L, int P, int op, uint32_t count, const uint32_t *x,
                            const uint32_t *y, uint32_t *z, hipStream_t st);
 
+// One transform of N points between chunk-major staging (chunk c = points
+// [c cl, (c + 1) cl): R rows of 2 cl words, real parts then imaginary parts)
+// and SoA rows (R rows of 2N words); to_soa picks the direction; cl divides N.
+hipError_t mp_chunk_relayout_launch(const uint32_t *src, uint32_t *dst, uint32_t N, uint32_t R,
+                                    uint32_t cl, bool to_soa, hipStream_t st);
+
 // ---- fixed form (fx_fft.hip) ----
 
 struct FxArgs {

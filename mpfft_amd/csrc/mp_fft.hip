@@ -612,3 +612,36 @@ hipError_t mp_binop_launch(int L, int P, int op, uint32_t count, const uint32_t 
 }
 
 }  // namespace mpfft
+
+namespace mpfft {
+
+// Chunk-major staging <-> SoA rows (one transform of N complex points, R =
+// L + 1 rows).  Chunk c holds points [c cl, (c + 1) cl) as R rows of 2 cl
+// words (real parts, then imaginary parts), so the host fills and uploads
+// each chunk as one contiguous block while it converts the next.  Thread
+// (row j, position e) moves one word; consecutive positions of one part of
+// one chunk are consecutive on both sides.
+__global__ void __launch_bounds__(256)
+mp_chunk_relayout_kernel(const uint32_t *src, uint32_t *dst, uint32_t N, uint32_t R, uint32_t cl,
+                         uint32_t to_soa)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (e >= 2 * N || j >= R) return;
+    const uint32_t im = e >= N ? 1u : 0u, k = e - im * N;
+    const uint32_t c = k / cl, loc = k - c * cl + im * cl;
+    const size_t chunk = (size_t)c * R * 2 * cl + (size_t)j * 2 * cl + loc;
+    const size_t soa = (size_t)j * 2 * N + e;
+    if (to_soa) dst[soa] = src[chunk];
+    else dst[chunk] = src[soa];
+}
+
+hipError_t mp_chunk_relayout_launch(const uint32_t *src, uint32_t *dst, uint32_t N, uint32_t R,
+                                    uint32_t cl, bool to_soa, hipStream_t st)
+{
+    if (!N || !R || !cl || N % cl) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mp_chunk_relayout_kernel, dim3((2 * N + 255) / 256, R), dim3(256), 0, st,
+                       src, dst, N, R, cl, to_soa ? 1u : 0u);
+    return hipGetLastError();
+}
+
+}  // namespace mpfft

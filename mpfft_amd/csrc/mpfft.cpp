@@ -19,6 +19,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <condition_variable>
 #include <mutex>
 #include <thread>
 #include <tuple>
@@ -186,6 +187,70 @@ struct PinnedBuf {
     template <class T> T *as() const { return (T *)p; }
 };
 
+// ---- host worker pool -------------------------------------------------------
+
+// The threads that marshal mpfr_t arrays, started on first use and kept for
+// the process (a thread spawned per call costs tens of microseconds, the
+// order of a whole small call).  One job at a time: start(n, f) runs f(w)
+// for w < n on n of the pool's threads and returns; wait() returns when all
+// n have finished.  Callers hold user() across start .. wait.
+class HostPool {
+public:
+    static HostPool &get()
+    {
+        // never destroyed: its threads end with the process
+        static HostPool *pool = new HostPool();
+        return *pool;
+    }
+    unsigned size() const { return (unsigned)th_.size(); }
+    std::mutex &user() { return user_; }
+    void start(unsigned n, std::function<void(unsigned)> f)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        job_ = std::move(f);
+        njob_ = std::min(n, size());
+        left_ = njob_;
+        gen_++;
+        cv_.notify_all();
+    }
+    void wait()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return left_ == 0; });
+        job_ = nullptr;
+    }
+
+private:
+    HostPool()
+    {
+        unsigned n = std::thread::hardware_concurrency();
+        n = std::max(1u, std::min(n, 16u));
+        for (unsigned i = 0; i < n; i++) th_.emplace_back([this, i] { loop(i); });
+    }
+    void loop(unsigned i)
+    {
+        unsigned seen = 0;
+        for (;;) {
+            std::function<void(unsigned)> f;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (i >= njob_) continue;
+                f = job_;
+            }
+            f(i);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--left_ == 0) done_.notify_all();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_, user_;
+    std::condition_variable cv_, done_;
+    std::function<void(unsigned)> job_;
+    unsigned njob_ = 0, left_ = 0, gen_ = 0;
+};
+
 // ---- mpfr_t <-> SoA limbs --------------------------------------------------
 
 int mpfr_put(mpfr_srcptr x, uint32_t *hdr, uint32_t *limb, size_t M, size_t i, long L)
@@ -270,6 +335,39 @@ void mpfr_take(mpfr_ptr x, const uint32_t *hdr, const uint32_t *limb, size_t M, 
     }
     const mpfr_exp_t e = (mpfr_exp_t)((int32_t)h >> 1);
     mpfr_custom_init_set(x, neg ? -MPFR_REGULAR_KIND : MPFR_REGULAR_KIND, e, prec, d);
+}
+
+// mpfr_take for the points k0 .. k1-1 of one array (SoA positions base + k)
+// in blocks of 32 points: the limb rows read one at a time along the row
+// (32 words each), then each point's significand assembled
+void mpfr_take_range(mpfr_t *x, unsigned long k0, unsigned long k1, size_t base,
+                     const uint32_t *hdr, const uint32_t *limb, size_t M, long L)
+{
+    constexpr unsigned kBlk = 32;
+    std::vector<uint32_t> tmp((size_t)L * kBlk);
+    for (unsigned long b0 = k0; b0 < k1; b0 += kBlk) {
+        const unsigned nb = (unsigned)std::min<unsigned long>(kBlk, k1 - b0);
+        for (long j = 0; j < L; j++) {
+            const uint32_t *row = limb + j * M + base + b0;
+            for (unsigned u = 0; u < nb; u++) tmp[(size_t)j * kBlk + u] = row[u];
+        }
+        for (unsigned u = 0; u < nb; u++) {
+            mpfr_ptr v = x[b0 + u];
+            const uint32_t h = hdr[base + b0 + u], neg = h & 1u;
+            if ((h & ~1u) == 0x80000000u) { mpfr_set_zero(v, neg ? -1 : 1); continue; }
+            const mpfr_prec_t prec = mpfr_get_prec(v);
+            const long n64 = (prec + 63) / 64;
+            mp_limb_t *d = (mp_limb_t *)mpfr_custom_get_significand(v);
+            for (long t = 0; t < n64; t++) {
+                const long j = 2 * t;
+                const uint64_t hi = j < L ? tmp[(size_t)j * kBlk + u] : 0;
+                const uint64_t lo = j + 1 < L ? tmp[(size_t)(j + 1) * kBlk + u] : 0;
+                d[n64 - 1 - t] = (mp_limb_t)((hi << 32) | lo);
+            }
+            const mpfr_exp_t e = (mpfr_exp_t)((int32_t)h >> 1);
+            mpfr_custom_init_set(v, neg ? -MPFR_REGULAR_KIND : MPFR_REGULAR_KIND, e, prec, d);
+        }
+    }
 }
 
 // ---- twiddles: the reference's lut() (serial_mp_fft3.c:71-82) ---------------
@@ -357,9 +455,12 @@ struct mpfft_plan_s {
     DevBuf errbuf;                       // per-block partial sums of the error statistics
     PinnedBuf h_stage_in, h_stage_out;
     double timing[4] = {0, 0, 0, 0};     // last mpfr_t call: marshal in, compute, marshal out, total (ms)
+    hipEvent_t time_ev[3] = {};          // mpfr_t calls: upload start, upload end, compute end
 
     ~mpfft_plan_s()
     {
+        for (auto e : time_ev)
+            if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -1269,86 +1370,109 @@ int mpfft_execute_mpfr(mpfft_plan p, mpfr_t *inr, mpfr_t *ini, mpfr_t *outr, mpf
     if ((rc = ensure_stage(p))) return rc;
     const size_t bytes = sizeof(uint32_t) * p->words;
     if ((rc = p->h_stage_in.alloc(bytes)) || (rc = p->h_stage_out.alloc(bytes))) return rc;
+    if (!p->d_aux.holds(bytes) && (rc = p->d_aux.alloc(bytes))) return rc;
 
-    // Marshalling in chunks of points on worker threads into page-locked
-    // staging; one contiguous copy each way.
+    // The input is converted in chunks of cl points (8 chunks from 8192
+    // points on, else one): the pool's threads convert every chunk in turn,
+    // each its share of the chunk's points, into page-locked chunk-major
+    // staging (each chunk L + 1 rows of 2 cl words, real parts then
+    // imaginary parts), while this thread uploads every chunk as soon as all
+    // its shares are in; on the device the chunks are laid out as SoA rows
+    // and the transform runs; the result comes back in one copy and the pool
+    // converts it.  (Chunked downloads overlapped with the conversion out
+    // measured slower: waiting threads, polling or blocked on events, slowed
+    // the runtime's own copy handling on the host.)
     const long L = p->L;
+    const uint32_t R = (uint32_t)L + 1;
+    const unsigned long cl = n >= 8192 ? n / 8 : n;
+    const unsigned C = (unsigned)(n / cl);
+    const size_t cw = (size_t)R * 2 * cl;                        // words per chunk
     const size_t M = 2 * n;
-    unsigned nthr = std::thread::hardware_concurrency();
-    nthr = std::max(1u, std::min({nthr, 16u, (unsigned)(n / 512 ? n / 512 : 1)}));
-    const unsigned C = n >= 4096 ? 4 * nthr : 1;              // chunks
-    const unsigned long cl = (n + C - 1) / C;
     uint32_t *hi = p->h_stage_in.as<uint32_t>(), *ho = p->h_stage_out.as<uint32_t>();
     uint32_t *di = p->d_in.as<uint32_t>(), *dout = p->d_out.as<uint32_t>();
+    uint32_t *ds = p->d_aux.as<uint32_t>();
+    for (auto &e : p->time_ev)
+        if (!e) HIP_TRY(hipEventCreate(&e));
+    HostPool &pool = HostPool::get();
+    std::lock_guard<std::mutex> pg(pool.user());
+    const unsigned nthr =
+        (unsigned)std::max(1UL, std::min((unsigned long)pool.size(), (cl + 255) / 256));
     auto t0 = std::chrono::steady_clock::now();
 
-    std::vector<std::atomic<int>> state(C);                    // 0 pending, 1 done, <0 error
-    for (auto &x : state) x.store(0);
-    std::vector<std::thread> th;
-    for (unsigned w = 0; w < nthr; w++)
-        th.emplace_back([&, w] {
-            for (unsigned c = w; c < C; c += nthr) {
-                const unsigned long k0 = c * cl, k1 = std::min(n, k0 + cl);
-                int r = k1 > k0 ? mpfr_put_range(inr, k0, k1, 0, hi, hi + M, M, L) : MPFFT_OK;
-                if (!r && k1 > k0) r = mpfr_put_range(ini, k0, k1, n, hi, hi + M, M, L);
-                state[c].store(r ? r : 1, std::memory_order_release);
+    // in: conversions on the pool, uploads here in chunk order
+    std::vector<int> parts(C, 0);                                // shares converted
+    std::mutex pm;
+    std::condition_variable pcv;
+    std::atomic<int> err{0};
+    pool.start(nthr, [&](unsigned w) {
+        // share w of a chunk: its points [w cl / nthr, (w + 1) cl / nthr)
+        const unsigned long s0 = (unsigned long)w * cl / nthr;
+        const unsigned long s1 = (unsigned long)(w + 1) * cl / nthr;
+        for (unsigned c = 0; c < C; c++) {
+            const unsigned long k0 = c * cl;
+            uint32_t *blk = hi + c * cw;
+            // position of point k in the chunk: k - k0 (real), cl + k - k0
+            // (imaginary), as offsets modulo 2^64
+            int r = err.load() ? MPFFT_OK
+                               : mpfr_put_range(inr, k0 + s0, k0 + s1, (size_t)0 - k0, blk,
+                                                blk + 2 * cl, 2 * cl, L);
+            if (!r && !err.load())
+                r = mpfr_put_range(ini, k0 + s0, k0 + s1, (size_t)cl - k0, blk, blk + 2 * cl,
+                                   2 * cl, L);
+            if (r) {
+                int z = 0;
+                err.compare_exchange_strong(z, r);
             }
-        });
-    for (auto &x : th) x.join();
-    th.clear();
-    for (unsigned c = 0; c < C && !rc; c++) {
-        const int v = state[c].load(std::memory_order_acquire);
-        if (v < 0) rc = v;
+            std::lock_guard<std::mutex> lk(pm);
+            if (++parts[c] == (int)nthr) pcv.notify_one();
+        }
+    });
+    hipError_t he = hipEventRecord(p->time_ev[0], p->stream);
+    for (unsigned c = 0; c < C && he == hipSuccess; c++) {
+        {
+            std::unique_lock<std::mutex> lk(pm);
+            pcv.wait(lk, [&] { return parts[c] == (int)nthr; });
+        }
+        if (err.load()) break;
+        he = hipMemcpyAsync(ds + c * cw, hi + c * cw, sizeof(uint32_t) * cw, hipMemcpyHostToDevice,
+                            p->stream);
     }
-    if (rc) return rc;
-    auto t1 = std::chrono::steady_clock::now();
-
-    // upload (timed by events, counted as marshalling in), transform,
-    // download: one contiguous copy each way (measured faster than strided
-    // copies per chunk overlapped with the conversions)
-    hipEvent_t cu, c0, c1;
-    HIP_TRY(hipEventCreate(&cu));
-    HIP_TRY(hipEventCreate(&c0));
-    HIP_TRY(hipEventCreate(&c1));
-    auto drop = [&] { (void)hipEventDestroy(cu); (void)hipEventDestroy(c0); (void)hipEventDestroy(c1); };
-    if (hipEventRecord(cu, p->stream) != hipSuccess ||
-        hipMemcpyAsync(di, hi, sizeof(uint32_t) * M * ((size_t)L + 1), hipMemcpyHostToDevice,
-                       p->stream) != hipSuccess ||
-        hipEventRecord(c0, p->stream) != hipSuccess) {
+    pool.wait();
+    rc = err.load();
+    if (!rc && he != hipSuccess) rc = MPFFT_ERR_HIP;
+    if (rc) {
         (void)hipStreamSynchronize(p->stream);
-        drop();
-        return MPFFT_ERR_HIP;
+        return rc;
     }
-    rc = run_limbs(p, di, dout, p->stream);
-    if (!rc && hipEventRecord(c1, p->stream) != hipSuccess) rc = MPFFT_ERR_HIP;
-    if (rc) { (void)hipStreamSynchronize(p->stream); drop(); return rc; }
-    if (hipMemcpyAsync(ho, dout, sizeof(uint32_t) * M * ((size_t)L + 1), hipMemcpyDeviceToHost,
-                       p->stream) != hipSuccess ||
-        hipStreamSynchronize(p->stream) != hipSuccess) {
-        drop();
-        return MPFFT_ERR_HIP;
-    }
-    for (unsigned w = 0; w < nthr; w++)
-        th.emplace_back([&, w] {
-            for (unsigned c = w; c < C; c += nthr) {
-                const unsigned long k0 = c * cl, k1 = std::min(n, k0 + cl);
-                for (unsigned long k = k0; k < k1; k++) {
-                    mpfr_take(outr[k], ho, ho + M, M, k, L);
-                    mpfr_take(outi[k], ho, ho + M, M, n + k, L);
-                }
-            }
-        });
-    for (auto &x : th) x.join();
+    // transform, download
+    if (hipEventRecord(p->time_ev[1], p->stream) != hipSuccess ||
+        mp_chunk_relayout_launch(ds, di, (uint32_t)n, R, (uint32_t)cl, true, p->stream) != hipSuccess)
+        rc = MPFFT_ERR_HIP;
+    if (!rc) rc = run_limbs(p, di, dout, p->stream);
+    if (!rc && (hipEventRecord(p->time_ev[2], p->stream) != hipSuccess ||
+                hipMemcpyAsync(ho, dout, sizeof(uint32_t) * M * R, hipMemcpyDeviceToHost,
+                               p->stream) != hipSuccess))
+        rc = MPFFT_ERR_HIP;
+    if (hipStreamSynchronize(p->stream) != hipSuccess && !rc) rc = MPFFT_ERR_HIP;
+    if (rc) return rc;
+    // out: the pool converts its shares of the points
+    pool.start(nthr, [&](unsigned w) {
+        const unsigned long k0 = (unsigned long)w * n / nthr, k1 = (unsigned long)(w + 1) * n / nthr;
+        mpfr_take_range(outr, k0, k1, 0, ho, ho + M, M, L);
+        mpfr_take_range(outi, k0, k1, n, ho, ho + M, M, L);
+    });
+    pool.wait();
     auto t2 = std::chrono::steady_clock::now();
-    float ms = 0, up = 0;
-    (void)hipEventElapsedTime(&ms, c0, c1);
-    (void)hipEventElapsedTime(&up, cu, c0);
-    drop();
-    const double tin = std::chrono::duration<double, std::milli>(t1 - t0).count() + up;
+    // in: from the call's start to the last upload's end (the upload-start
+    // event completes at once on the idle stream); compute: the device's
+    // layout change and the transform; out: the rest
+    float tin = 0, tgpu = 0;
+    (void)hipEventElapsedTime(&tin, p->time_ev[0], p->time_ev[1]);
+    (void)hipEventElapsedTime(&tgpu, p->time_ev[1], p->time_ev[2]);
     const double ttot = std::chrono::duration<double, std::milli>(t2 - t0).count();
     p->timing[0] = tin;
-    p->timing[1] = ms;
-    p->timing[2] = ttot - tin - ms;
+    p->timing[1] = tgpu;
+    p->timing[2] = ttot - tin - tgpu;
     p->timing[3] = ttot;
     return fx_check(p);
 }

@@ -13,7 +13,7 @@
 // a symbol's "device address" on the stand-in device is its host address
 #define HIP_SYMBOL(x) (&(x))
 typedef int hipError_t;
-enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorUnknown = 999 };
+enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorNotReady = 600, hipErrorUnknown = 999 };
 struct ihipStream_t; typedef ihipStream_t *hipStream_t;
 struct ihipEvent_t; typedef ihipEvent_t *hipEvent_t;
 struct ihipGraph; typedef ihipGraph *hipGraph_t;
@@ -42,6 +42,7 @@ hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned f);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
 hipError_t hipEventSynchronize(hipEvent_t e);
+hipError_t hipEventQuery(hipEvent_t e);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipSetDevice(int d);
 hipError_t hipGetDevice(int *d);

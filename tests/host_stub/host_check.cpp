@@ -8,13 +8,14 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
 
 void stub_fail_malloc_in(int k);      // stub.cpp
 #include <atomic>
-namespace mpfft { extern std::atomic<int> stub_sub_checked, stub_sub_errors; }   // stub.cpp
+namespace mpfft { extern std::atomic<int> stub_sub_checked, stub_sub_errors, stub_identity_mp; }   // stub.cpp
 
 static int fails = 0;
 #define CHECK(c, ...)                                   \
@@ -221,6 +222,58 @@ static void batches_concurrently()
     c.join();
 }
 
+// The drop-in mpfr_t call's pipeline (chunked conversion on the host pool,
+// uploads, the device's chunk <-> SoA layout changes, chunked downloads and
+// conversions out) with the plain transform as the identity: the outputs
+// must equal the inputs bit for bit, at one chunk (n = 16) and at 8 and 64
+// chunks, precisions of one and several 64-bit limbs, signed zeros.
+static void mpfr_pipeline_one(unsigned long n, long P, int seed)
+{
+    mpfft_plan p;
+    CHECK(mpfft_plan_create(&p, n, P, MPFFT_FORM_MP, 1) == 0, "mpfr plan n=%lu P=%ld", n, P);
+    std::unique_ptr<mpfr_t[]> a(new mpfr_t[4 * n]);
+    gmp_randstate_t st;
+    gmp_randinit_default(st);
+    gmp_randseed_ui(st, (unsigned long)seed);
+    for (unsigned long i = 0; i < 4 * n; i++) {
+        mpfr_init2(a[i], P);
+        mpfr_urandomb(a[i], st);
+        if (i % 7 == 3) mpfr_neg(a[i], a[i], MPFR_RNDN);
+        if (i % 97 == 5) mpfr_set_zero(a[i], (i & 1) ? -1 : 1);
+        mpfr_mul_2si(a[i], a[i], (long)(i % 13) - 6, MPFR_RNDN);
+    }
+    mpfr_t *inr = a.get(), *ini = a.get() + n, *outr = a.get() + 2 * n, *outi = a.get() + 3 * n;
+    CHECK(mpfft_execute_mpfr(p, inr, ini, outr, outi) == 0, "mpfr exec n=%lu", n);
+    int bad = 0;
+    for (unsigned long k = 0; k < n; k++) {
+        bad += mpfr_cmp(outr[k], inr[k]) != 0 || mpfr_signbit(outr[k]) != mpfr_signbit(inr[k]);
+        bad += mpfr_cmp(outi[k], ini[k]) != 0 || mpfr_signbit(outi[k]) != mpfr_signbit(ini[k]);
+    }
+    CHECK(bad == 0, "mpfr pipeline n=%lu P=%ld: %d mismatches", n, P, bad);
+    double ms[4];
+    mpfft_plan_last_timing(p, ms);
+    CHECK(ms[3] >= 0, "timing");
+    for (unsigned long i = 0; i < 4 * n; i++) mpfr_clear(a[i]);
+    gmp_randclear(st);
+    mpfft_plan_destroy(p);
+}
+
+static void mpfr_pipeline(bool threads)
+{
+    mpfft::stub_identity_mp.store(1);
+    if (threads) {
+        std::thread a([] { mpfr_pipeline_one(4096, 256, 1); mpfr_pipeline_one(16, 64, 2); });
+        std::thread b([] { mpfr_pipeline_one(1UL << 15, 100, 3); });
+        a.join();
+        b.join();
+    } else {
+        mpfr_pipeline_one(16, 53, 4);
+        mpfr_pipeline_one(4096, 256, 5);
+        mpfr_pipeline_one(1UL << 15, 1100, 6);
+    }
+    mpfft::stub_identity_mp.store(0);
+}
+
 // the plain form's subtrees finished in LDS (MPFFT_SUB_SIZE): their tables,
 // checked by the stand-in launcher against the recursion, at every
 // transform size and subtree size, at 2, 8, 16 and 32 limbs (a size whose
@@ -254,11 +307,13 @@ int main(int argc, char **argv)
 {
     if (argc > 1 && std::string(argv[1]) == "threads") {
         batches_concurrently();
+        mpfr_pipeline(true);
         printf(fails ? "FAIL\n" : "PASS\n");
         return fails ? 1 : 0;
     }
     if (argc > 1) return legacy(argv[1]);
     plans();
+    mpfr_pipeline(false);
     subtrees();
     alloc_failure_then_retry();
     noalloc_stats();

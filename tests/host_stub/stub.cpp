@@ -45,6 +45,8 @@ hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { *e = new ihipEvent
 hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s) { (void)e->x; if (s) (void)s->id; return hipSuccess; }
 hipError_t hipEventSynchronize(hipEvent_t e) { (void)e->x; return hipSuccess; }
+// the stand-in runs work at enqueue: an event is complete once recorded
+hipError_t hipEventQuery(hipEvent_t e) { (void)e->x; return hipSuccess; }
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) { (void)a->x; (void)b->x; *ms = (float)(rand() % 100 + 1) * 0.01f; return hipSuccess; }
 hipError_t hipSetDevice(int) { return hipSuccess; }
 hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
@@ -107,7 +109,31 @@ static void sub_check(int L, uint32_t N, const SubArgs &sa, const LevelArgs *lv,
     stub_sub_checked++;
     if (bad) { stub_sub_errors++; printf("sub tables: N=%u L=%d S=%u: %d mismatches\n", N, L, 1u << sa.lgs[0], bad); }
 }
+// identity mode: the plain transform's launch copies its input to its
+// output, so a caller's data movement around it can be checked end to end
+std::atomic<int> stub_identity_mp{0};
+hipError_t mp_chunk_relayout_launch(const uint32_t *src, uint32_t *dst, uint32_t N, uint32_t R,
+                                    uint32_t cl, bool to_soa, hipStream_t) {
+    if (!N || !R || !cl || N % cl) return hipErrorInvalidValue;
+    run_or_record([=] {
+        for (uint32_t j = 0; j < R; j++)
+            for (uint32_t e = 0; e < 2 * N; e++) {
+                const uint32_t im = e >= N, k = e - im * N, c = k / cl, loc = k - c * cl + im * cl;
+                const size_t chunk = (size_t)c * R * 2 * cl + (size_t)j * 2 * cl + loc;
+                const size_t soa = (size_t)j * 2 * N + e;
+                if (to_soa) dst[soa] = src[chunk];
+                else dst[chunk] = src[soa];
+            }
+    });
+    return hipSuccess;
+}
 hipError_t mp_fft_launch(int L, int, const LeafArgs &la, const LevelArgs *lv, int nlev, uint32_t batch, hipStream_t, const SubArgs *sub) {
+    if (stub_identity_mp.load()) {
+        const size_t w = 2 * (size_t)la.N * (L + 1);
+        LeafArgs a = la;
+        run_or_record([=] { for (uint32_t b = 0; b < batch; b++) memcpy(a.data + a.stride * b, a.in + a.in_stride * b, 4 * w); });
+        return hipSuccess;
+    }
     if (sub && la.npos) sub_check(L, la.N, *sub, lv, nlev);
     const size_t w = 2 * (size_t)la.N * (L + 1);
     LeafArgs a = la; std::vector<LevelArgs> v(lv, lv + nlev);
