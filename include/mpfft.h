@@ -162,14 +162,17 @@ int mpfft_plan_info(mpfft_plan plan, long *limbs, size_t *words_per_transform);
 int mpfft_plan_form(mpfft_plan plan);
 
 /* One transform through mpfr_t arrays (marshalling included).  The
- * significands are marshalled on up to 16 host threads into page-locked
- * staging, with one contiguous copy each way. */
+ * significands are marshalled on a pool of up to 16 host threads (kept for
+ * the process) into page-locked staging: the input in chunks, each uploaded
+ * as soon as it is converted, the output in one copy.  Calls on different
+ * plans share the pool one at a time. */
 int mpfft_execute_mpfr(mpfft_plan plan, mpfr_t *inr, mpfr_t *ini,
                        mpfr_t *outr, mpfr_t *outi);
 
 /* Wall times of the plan's last mpfft_execute_mpfr call, in ms:
- * ms[0] marshalling in (conversion and upload), ms[1] the transform on the
- * GPU, ms[2] download and conversion out, ms[3] total. */
+ * ms[0] marshalling in (from the call's start to the end of the last
+ * upload), ms[1] the transform on the GPU (with the device's change of the
+ * staging layout), ms[2] download and conversion out, ms[3] total. */
 int mpfft_plan_last_timing(mpfft_plan plan, double *ms);
 
 /* `batch` transforms on device-resident SoA-limb buffers (d_in and d_out
