@@ -25,6 +25,8 @@
 #include <tuple>
 #include <vector>
 
+#include <unistd.h>
+
 static_assert(sizeof(mp_limb_t) == 8, "64-bit GMP limbs expected");
 
 using namespace mpfft;
@@ -198,8 +200,13 @@ class HostPool {
 public:
     static HostPool &get()
     {
-        // never destroyed: its threads end with the process
-        static HostPool *pool = new HostPool();
+        // never destroyed: its threads end with the process.  A child made
+        // by fork() inherits the pool but not its threads: it starts its own
+        // (the parent's object is left alone)
+        static std::mutex mk;
+        static HostPool *pool = nullptr;
+        std::lock_guard<std::mutex> lk(mk);
+        if (!pool || pool->pid_ != getpid()) pool = new HostPool();
         return *pool;
     }
     unsigned size() const { return (unsigned)th_.size(); }
@@ -221,7 +228,7 @@ public:
     }
 
 private:
-    HostPool()
+    HostPool() : pid_(getpid())
     {
         unsigned n = std::thread::hardware_concurrency();
         n = std::max(1u, std::min(n, 16u));
@@ -244,6 +251,7 @@ private:
             if (--left_ == 0) done_.notify_all();
         }
     }
+    pid_t pid_;
     std::vector<std::thread> th_;
     std::mutex mu_, user_;
     std::condition_variable cv_, done_;

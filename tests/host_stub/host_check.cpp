@@ -13,6 +13,9 @@
 #include <thread>
 #include <vector>
 
+#include <sys/wait.h>
+#include <unistd.h>
+
 void stub_fail_malloc_in(int k);      // stub.cpp
 #include <atomic>
 namespace mpfft { extern std::atomic<int> stub_sub_checked, stub_sub_errors, stub_identity_mp; }   // stub.cpp
@@ -274,6 +277,22 @@ static void mpfr_pipeline(bool threads)
     mpfft::stub_identity_mp.store(0);
 }
 
+// the mpfr_t call in a child made by fork() after the parent's host pool has
+// started: the child has none of the pool's threads and must start its own
+static void mpfr_pipeline_forked()
+{
+    fflush(stdout);
+    const pid_t pid = fork();
+    if (pid == 0) {
+        mpfft::stub_identity_mp.store(1);
+        mpfr_pipeline_one(4096, 256, 7);
+        _exit(fails ? 1 : 0);
+    }
+    int status = 0;
+    CHECK(pid > 0 && waitpid(pid, &status, 0) == pid, "fork");
+    CHECK(WIFEXITED(status) && WEXITSTATUS(status) == 0, "mpfr call in a forked child: status %d", status);
+}
+
 // the plain form's subtrees finished in LDS (MPFFT_SUB_SIZE): their tables,
 // checked by the stand-in launcher against the recursion, at every
 // transform size and subtree size, at 2, 8, 16 and 32 limbs (a size whose
@@ -314,6 +333,7 @@ int main(int argc, char **argv)
     if (argc > 1) return legacy(argv[1]);
     plans();
     mpfr_pipeline(false);
+    mpfr_pipeline_forked();
     subtrees();
     alloc_failure_then_retry();
     noalloc_stats();
