@@ -472,54 +472,98 @@ __device__ __forceinline__ void w_dft4(CW_<K> &x0, CW_<K> &x1, CW_<K> &x2, CW_<K
     x3 = w_add_ib<K>(t1, t3);
 }
 
-// Radix-16 Stockham pass of sub-transform size Ns: thread j (< N/16) reads
-// v_r = x[j + r N/16], multiplies v_r by w^(r k N / (16 Ns)) (entry
-// toff + 15k + r - 1), then the DFT16 as 4 x 4: a DFT4 over r2 for each r1
-// (inputs v_(r1 + 4 r2)), the internal twiddles w16^(r1 q1) (entries cent +
-// r1 q1; w16^4 = -i exactly), a DFT4 over r1 for each q1, output q1 + 4 q2
-// written to (j - k) 16 + k + (q1 + 4 q2) Ns.
+// One complex value of K words in the exchange buffer of the radix-16 pass:
+// K/2 double2 of the real part, then K/2 of the imaginary part, each a row
+// of 64 lanes (a wave reads or writes 1 KB runs).
+template <int K>
+__device__ __forceinline__ void w_xput(double2 (*x)[64], uint32_t l, const CW_<K> &v)
+{
+    if constexpr (K == 2) {
+        x[0][l] = make_double2(v.re.hi, v.re.lo);
+        x[1][l] = make_double2(v.im.hi, v.im.lo);
+    } else {
+        x[0][l] = make_double2(v.re.c[0], v.re.c[1]);
+        x[1][l] = make_double2(v.re.c[2], v.re.c[3]);
+        x[2][l] = make_double2(v.im.c[0], v.im.c[1]);
+        x[3][l] = make_double2(v.im.c[2], v.im.c[3]);
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void w_xget(CW_<K> &v, double2 (*x)[64], uint32_t l)
+{
+    if constexpr (K == 2) {
+        const double2 r = x[0][l], i = x[1][l];
+        v.re = {r.x, r.y};
+        v.im = {i.x, i.y};
+    } else {
+        const double2 r0 = x[0][l], r1 = x[1][l], i0 = x[2][l], i1 = x[3][l];
+        v.re = {{r0.x, r0.y, r1.x, r1.y}};
+        v.im = {{i0.x, i0.y, i1.x, i1.y}};
+    }
+}
+
+// Radix-16 Stockham pass of sub-transform size Ns, one butterfly on four
+// waves.  Butterfly g = 64 blockIdx + lane is j = g mod N/16 of transform
+// g / (N/16) (small transforms share a block).  Wave w reads v_r = x[j + r
+// N/16] for r = w + 4 r2 (r2 = 0 .. 3), multiplies v_r by w^(r k N / (16 Ns)),
+// k = j mod Ns (entry toff + 15k + r - 1), takes the DFT4 over r2 (the DFT16
+// as 4 x 4: U_w(q1) = sum_r2 v_(w + 4 r2) (-i)^(r2 q1)) and the internal
+// twiddles w16^(w q1) (entries cent + w q1; w16^4 = -i exactly), and puts
+// U_w(q1) into the exchange buffer.  After the barrier wave w holds q1 = w:
+// the DFT4 over r1 of U_r1(w) gives the outputs y_(w + 4 q2), written to
+// (j - k) 16 + k + (w + 4 q2) Ns.  A thread holds four complex values, so
+// the pass runs at several waves per SIMD where one thread per butterfly
+// (sixteen values) ran at three, and the first and last passes convert
+// from and to limbs in it.
 template <int K, int L, bool FULL, bool IN, bool OUT>
 __global__ void __launch_bounds__(256)
 mw_r16_kernel(FxArgs a, const double2 *src, double2 *dst, uint32_t Ns, uint32_t toff,
-              uint32_t cent)
+              uint32_t cent, uint32_t lgS)
 {
-    const uint32_t S = a.N >> 4, j = blockIdx.x * 256 + threadIdx.x, b = blockIdx.y;
-    if (j >= S) return;
+    __shared__ double2 xs[16][K][64];          // (r1, q1), then the words
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t g = (uint64_t)blockIdx.x * 64 + lane;
+    const uint32_t S = a.N >> 4;
+    const bool live = g < ((uint64_t)a.batch << lgS);
+    const uint32_t b = live ? (uint32_t)(g >> lgS) : 0u, j = (uint32_t)g & (S - 1);
     const uint32_t k = j & (Ns - 1);
     const double2 *tw = reinterpret_cast<const double2 *>(a.tw);
     auto entry = [&](uint32_t t) {
-        CW_<K> w;
-        Word<K>::get(w.re, tw + (size_t)K * t);
-        Word<K>::get(w.im, tw + (size_t)K * t + K / 2);
-        return w;
+        CW_<K> e;
+        Word<K>::get(e.re, tw + (size_t)K * t);
+        Word<K>::get(e.im, tw + (size_t)K * t + K / 2);
+        return e;
     };
-    CW_<K> v[16];
+    CW_<K> v[4];
+    if (live) {
 #pragma unroll
-    for (int r = 0; r < 16; r++) v[r] = w_load<K, L, IN>(a, src, b, j + r * S);
-    if (k) {
+        for (int r2 = 0; r2 < 4; r2++) v[r2] = w_load<K, L, IN>(a, src, b, j + (w + 4 * r2) * S);
+        if (k) {
 #pragma unroll
-        for (int r = 1; r < 16; r++) v[r] = w_mul<K>(v[r], entry(toff + 15 * k + r - 1));
-    }
-#pragma unroll
-    for (int r1 = 0; r1 < 4; r1++) w_dft4<K>(v[r1], v[r1 + 4], v[r1 + 8], v[r1 + 12]);
-    // v[r1 + 4 q1] now holds U_r1(q1)
-#pragma unroll
-    for (int r1 = 1; r1 < 4; r1++)
+            for (int r2 = 0; r2 < 4; r2++) {
+                const uint32_t r = w + 4 * r2;
+                if (r) v[r2] = w_mul<K>(v[r2], entry(toff + 15 * k + r - 1));
+            }
+        }
+        w_dft4<K>(v[0], v[1], v[2], v[3]);     // v[q1] = U_w(q1)
 #pragma unroll
         for (int q1 = 1; q1 < 4; q1++) {
-            const int m = r1 * q1;
-            CW_<K> &u = v[r1 + 4 * q1];
-            u = m == 4 ? w_neg_i<K>(u) : w_mul<K>(u, entry(cent + m));
+            const uint32_t m = w * q1;
+            if (m == 4) v[q1] = w_neg_i<K>(v[q1]);
+            else if (m) v[q1] = w_mul<K>(v[q1], entry(cent + m));
         }
 #pragma unroll
-    for (int q1 = 0; q1 < 4; q1++) w_dft4<K>(v[4 * q1], v[4 * q1 + 1], v[4 * q1 + 2], v[4 * q1 + 3]);
-    // v[4 q1 + q2] = y_(q1 + 4 q2)
+        for (int q1 = 0; q1 < 4; q1++) w_xput<K>(xs[4 * w + q1], lane, v[q1]);
+    }
+    __syncthreads();
+    if (!live) return;
+#pragma unroll
+    for (int r1 = 0; r1 < 4; r1++) w_xget<K>(v[r1], xs[4 * r1 + w], lane);
+    w_dft4<K>(v[0], v[1], v[2], v[3]);         // v[q2] = y_(w + 4 q2)
     const uint32_t o = (j - k) * 16 + k;
 #pragma unroll
-    for (int q1 = 0; q1 < 4; q1++)
-#pragma unroll
-        for (int q2 = 0; q2 < 4; q2++)
-            w_store<K, L, FULL, OUT>(a, dst, b, o + (q1 + 4 * q2) * Ns, v[4 * q1 + q2]);
+    for (int q2 = 0; q2 < 4; q2++) w_store<K, L, FULL, OUT>(a, dst, b, o + (w + 4 * q2) * Ns, v[q2]);
 }
 
 // N = 1: the transform is the identity; the input split and rounded back
@@ -545,9 +589,14 @@ void launch_pass(const FxArgs &a, const MwPass &ps, uint32_t cent, const double2
     else if (ps.R == 4)
         hipLaunchKernelGGL((mw_r4_kernel<K, L, FULL, IN, OUT>), grid, dim3(256), 0, st, a, src,
                            dst, ps.Ns, ps.toff);
-    else if constexpr (K == 2 && !IN && !OUT)       // (the plan keeps radix 16 inside)
-        hipLaunchKernelGGL((mw_r16_kernel<K, L, FULL, IN, OUT>), grid, dim3(256), 0, st, a, src,
-                           dst, ps.Ns, ps.toff, cent);
+    else if constexpr (K == 2) {
+        // four waves per block, one butterfly per lane
+        uint32_t lgS = 0;
+        while ((2u << lgS) <= a.N >> 4) lgS++;
+        const uint64_t bf = (uint64_t)a.batch << lgS;
+        hipLaunchKernelGGL((mw_r16_kernel<K, L, FULL, IN, OUT>), dim3((uint32_t)((bf + 63) / 64)),
+                           dim3(256), 0, st, a, src, dst, ps.Ns, ps.toff, cent, lgS);
+    }
 }
 
 template <int K, int L, bool FULL>
@@ -565,7 +614,7 @@ hipError_t launch_mw(const FxArgs &a, int s0, int s1, hipStream_t st)
     const int npass = mw_plan(N, K, ps, &cent);
     double2 *src = reinterpret_cast<double2 *>(a.buf1), *dst = reinterpret_cast<double2 *>(a.buf0);
     for (int s = 0; s < npass; s++) {
-        if (ps[s].R == 16 && (K != 2 || s == 0 || s == npass - 1)) return hipErrorInvalidValue;
+        if (ps[s].R == 16 && K != 2) return hipErrorInvalidValue;
         if (s >= s0 && s < s1) {
             const bool first = s == 0, last = s == npass - 1;
             if (first && last) launch_pass<K, L, FULL, true, true>(a, ps[s], cent, src, dst, st);

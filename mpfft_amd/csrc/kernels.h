@@ -468,14 +468,14 @@ hipError_t fxs_fft_launch(int L, int P, const FxArgs &a, const FxsPass *ps, int 
 // input, the last rounds to P bits); N = 1: one.
 hipError_t dd_fft_launch(int K, int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st);
 
-// Passes of the word forms.  Double-word (K = 2): the first pass (which
-// also splits the limb input) radix 2 or 4, the last (which also rounds to
-// limbs) radix 4, and radix 16 in between (one thread holds the 16 inputs
-// of a butterfly), with one more radix-4 pass where log2 N needs it (a
-// converting radix-16 pass ran at one wave per SIMD and took 2.4x the time
-// of a converting radix-4 pass; an inner one 0.82x that of the two radix-4
-// passes it replaces); quad-word (K = 4, twice the registers per value):
-// radix 4 after a radix-2 pass when log2 N is odd.
+// Passes of the word forms.  Double-word (K = 2): radix 16 (four threads per
+// butterfly), with a radix-2 pass first where log2 N is odd and a radix-4
+// pass after the first radix-16 one where bit 1 of log2 N is set (a
+// radix-16 pass with one thread per butterfly ran at three waves per SIMD,
+// and converting at one); quad-word (K = 4, twice the registers per value):
+// radix 4 after a radix-2 pass when log2 N is odd.  The first pass also
+// splits the limb input into words and the last rounds back to limbs; the
+// radix-2 kernel has no twiddles, so it runs only as a first pass (Ns = 1).
 // Pass i has radix R and sub-transform size Ns; its twiddles w^(r k N / (R
 // Ns)) (k < Ns, r = 1 .. R-1) are table entries toff + (R - 1) k + r - 1.
 // With a radix-16 pass, ten entries from *nent on hold w16^m = w^(m N / 16),
@@ -495,12 +495,11 @@ inline int mw_plan(uint32_t N, int K, MwPass (&p)[32], uint32_t *nent)
         toff += (R - 1) * Ns;
         Ns *= R;
     };
-    if (K == 2 && lg >= 5) {
-        add((lg & 1) ? 2 : 4);
-        const int m = lg - ((lg & 1) ? 1 : 2) - 2;   // log2 of the middle passes' product
-        if (m % 4) add(4);
-        for (int i = 0; i < m / 4; i++) add(16);
-        add(4);
+    if (K == 2 && lg >= 4) {
+        if (lg & 1) add(2);
+        add(16);
+        if (lg & 2) add(4);
+        for (int i = 1; i < lg / 4; i++) add(16);
     } else {
         if (lg & 1) add(2);
         while (Ns < N) add(4);

@@ -9,8 +9,9 @@ own error is far below 2^-P):
     component), plus the double-word arithmetic (each radix-4 pass at most
     ~12 u^2 relative, u^2 = 2^-106: DWPlusDW below 3u^2, DWTimesDW below
     5u^2, per Joldes-Muller-Popescu 2017, two levels of additions and one
-    complex product per element), which the measured errors stay far
-    below (P = 106: 1.3 at n = 2^10 .. 2^12, 1.6 at 2^16);
+    complex product per element; a radix-16 pass is two such steps), which
+    the measured errors stay far below (P = 106: 1.3 at n = 2^10 .. 2^12,
+    1.6 at 2^16);
   - no output is nonzero where the exact transform is zero;
   - closed forms: an impulse gives exactly 1 everywhere.
 """
@@ -47,7 +48,7 @@ def check(n, P, x, y):
     return mu, nw
 
 
-@pytest.mark.parametrize("n", [1, 2, 4, 8, 16, 32, 64, 1024, 2048, 4096])
+@pytest.mark.parametrize("n", [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096])
 @pytest.mark.parametrize("P", [53, 64, 100, 106])
 def test_dd_error_bound(n, P):
     xs = [I.mp_random(n, P, seed=1300 + n + P + 7 * i) for i in range(3)]
@@ -112,7 +113,7 @@ def test_dd_batch_and_stages():
         check(n, P, x, y)
     plan = M.Plan(n, P, M.FORM_DD, b)
     S = plan.stages()
-    assert S == 1 + 2 + 1                # log2 n = 11: radix 2, two radix 16, radix 4
+    assert S == 4                        # log2 n = 11: radix 2, 16, 4, 16
     x = np.concatenate([I.relimb(t, n, O.limbs(P), plan.L) for t in xs])
     dx = torch.from_numpy(x.view(np.int32)).cuda()
     d1, d2 = torch.empty_like(dx), torch.empty_like(dx)
