@@ -32,7 +32,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
 SOURCES = ["mp_fft.hip", "fx_fft.hip", "fx_panel.hip", "dd_fft.hip", "d_fft.hip", "mpfft.cpp",
            "batch.cpp",
            "hexio.cpp"]
-HEADERS = ["mp_arith.h", "fx_arith.h", "fx_lane.h", "kernels.h"]
+HEADERS = ["mp_arith.h", "fx_arith.h", "fx_lane.h", "kernels.h", "dd_conv.h"]
 
 
 def _stale(obj, deps):

@@ -17,21 +17,18 @@
 // below).
 //
 // Structure: Stockham passes (mw_plan in kernels.h): radix 16 for the
-// double-word form, radix 4 for the quad-word form.  The first pass reads the SoA-limb
-// input and splits every real into its words exactly; the last pass rounds
-// the words' sum to P bits with the limb adder of mp_arith.h (double-word:
-// both parts are exact P-bit values, so one correctly rounded addition) and
-// writes SoA limbs.  In between, a real is K/2 double2.
+// double-word form, radix 4 for the quad-word form.  The first pass reads
+// the SoA-limb input and splits every real into its words exactly; the last
+// pass rounds the words' sum to P bits and writes SoA limbs (double-word:
+// the correctly rounded sum of two binary64 numbers, dd_conv.h; quad-word:
+// the limb adder of mp_arith.h).  In between, a real is K/2 double2.
+#include "dd_conv.h"
 #include "kernels.h"
 #include "mp_arith.h"
 
 namespace mpfft {
 
 namespace {
-
-struct DD {
-    double hi, lo;
-};
 
 struct CDD {
     DD re, im;
@@ -73,74 +70,6 @@ __device__ __forceinline__ DD dd_mul(const DD &x, const DD &y)
     e = __builtin_fma(x.hi, y.lo, e);
     e = __builtin_fma(x.lo, y.hi, e);
     return fast_two_sum(p, e);
-}
-
-// P-bit limb value (53 <= P <= 106) times 2^-eoff (the transform's block
-// exponent, an exact shift) -> hi + lo, exact.  The significand's top 53
-// bits rounded to nearest (ties to even) give hi; the remainder, at most 53
-// bits wide because P <= 106, is lo.  A scaled exponent e below -lim
-// (binary64 could not hold lo; the value is below 2^-lim of the transform's
-// largest input) is taken as zero; one above lim returns 1.
-template <int L>
-__device__ __forceinline__ uint32_t dd_from_mp(DD &z, const Mp<L> &x, int lim, int32_t eoff)
-{
-    const int32_t e = x.e - eoff;
-    if (x.zero || e < -lim) {
-        z.hi = x.neg ? -0.0 : 0.0;
-        z.lo = 0.0;
-        return 0;
-    }
-    if (e > lim) {
-        z.hi = 0.0;
-        z.lo = 0.0;
-        return 1;
-    }
-    // value = 0.m * 2^e; A = significand bits 1..64, B = bits 65..128
-    const uint64_t A = (uint64_t)x.m[0] << 32 | (L > 1 ? x.m[1] : 0u);
-    const uint64_t B = (uint64_t)(L > 2 ? x.m[2] : 0u) << 32 | (L > 3 ? x.m[3] : 0u);
-    uint64_t h = A >> 11;                                        // bits 1..53
-    int64_t r = (int64_t)(((A & 0x7ffull) << 42) | (B >> 22));  // bits 54..106, < 2^53
-    const int64_t half = 1ll << 52;
-    if (r > half || (r == half && (h & 1u))) {
-        h += 1;
-        r -= 2 * half;
-    }
-    const double hi = __builtin_ldexp((double)h, e - 53);
-    const double lo = __builtin_ldexp((double)r, e - 106);
-    z.hi = x.neg ? -hi : hi;
-    z.lo = x.neg ? -lo : lo;
-    return 0;
-}
-
-// a binary64 value as an exact limb value (53 bits <= 32 L)
-template <int L>
-__device__ __forceinline__ void mp_from_double(Mp<L> &z, double v)
-{
-    if (v == 0.0) {
-        set_zero(z, __builtin_signbit(v) ? 1u : 0u);
-        return;
-    }
-    int ex = 0;
-    const double f = __builtin_frexp(__builtin_fabs(v), &ex);  // [0.5, 1)
-    const uint64_t m = (uint64_t)__builtin_ldexp(f, 53) << 11;  // 53 bits at the top
-#pragma unroll
-    for (int j = 0; j < L; j++) z.m[j] = 0u;
-    z.m[0] = (uint32_t)(m >> 32);
-    z.m[1] = (uint32_t)m;
-    z.e = ex;
-    z.neg = v < 0.0 ? 1u : 0u;
-    z.zero = 0u;
-}
-
-// (hi + lo) 2^eoff rounded once to P bits
-template <int L, bool FULL>
-__device__ __forceinline__ void dd_to_mp(Mp<L> &z, const DD &x, int P, int32_t eoff)
-{
-    Mp<L> a, b;
-    mp_from_double(a, x.hi);
-    mp_from_double(b, x.lo);
-    mp_add<L, FULL>(z, a, 0u, b, 0u, P);
-    if (!z.zero) z.e += eoff;
 }
 
 // ---- quad-word numbers (K = 4) ---------------------------------------------
@@ -355,7 +284,7 @@ __device__ __forceinline__ uint32_t w_from_mp(RW_<K> &z, const Mp<L> &x, int lim
 template <int K, int L, bool FULL>
 __device__ __forceinline__ void w_to_mp(Mp<L> &z, const RW_<K> &x, int P, int32_t eoff)
 {
-    if constexpr (K == 2) dd_to_mp<L, FULL>(z, x, P, eoff);
+    if constexpr (K == 2) dd_to_mp<L>(z, x, P, eoff);
     else qd_to_mp<L, FULL>(z, x, P, eoff);
 }
 
