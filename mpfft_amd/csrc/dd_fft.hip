@@ -293,28 +293,37 @@ __device__ __forceinline__ void w_to_mp(Mp<L> &z, const RW_<K> &x, int P, int32_
 // Element e of transform b (re: e < N, im: N + e) is K/2 double2 at
 // b * stride + (K/2) e.
 
-// element e of transform b: from the limb input (IN) or a word buffer
-template <int K, int L, bool IN>
-__device__ __forceinline__ CW_<K> w_load(const FxArgs &a, const double2 *src, uint32_t b,
-                                         uint32_t e)
+// elements e[0..R) of transform b: from the limb input (IN) or a word
+// buffer.  The limb input is loaded for all R elements before any is
+// converted (a conversion branches on its value, and a load behind a branch
+// waits for it), and an input beyond the range sets `bad`, which the caller
+// reports once.
+template <int K, int L, bool IN, int R>
+__device__ __forceinline__ void w_load(const FxArgs &a, const double2 *src, uint32_t b,
+                                       const uint32_t (&e)[R], CW_<K> (&v)[R], uint32_t &bad)
 {
-    CW_<K> v;
     if constexpr (IN) {
         const uint32_t *in = a.in + a.in_stride * b;
         const int32_t eo = a.eoff[b];
-        Mp<L> x;
-        uint32_t bad = 0;
-        mp_load(x, in, 2 * a.N, e);
-        bad |= w_from_mp<K, L>(v.re, x, a.lim, eo);
-        mp_load(x, in, 2 * a.N, a.N + e);
-        bad |= w_from_mp<K, L>(v.im, x, a.lim, eo);
-        if (bad) atomicOr(a.flag, 1u);
+        Mp<L> xr[R], xi[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            mp_load(xr[r], in, 2 * a.N, e[r]);
+            mp_load(xi[r], in, 2 * a.N, a.N + e[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            bad |= w_from_mp<K, L>(v[r].re, xr[r], a.lim, eo);
+            bad |= w_from_mp<K, L>(v[r].im, xi[r], a.lim, eo);
+        }
     } else {
         const double2 *p = src + a.stride * b;
-        Word<K>::get(v.re, p + (K / 2) * e);
-        Word<K>::get(v.im, p + (K / 2) * (a.N + e));
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            Word<K>::get(v[r].re, p + (K / 2) * e[r]);
+            Word<K>::get(v[r].im, p + (K / 2) * (a.N + e[r]));
+        }
     }
-    return v;
 }
 
 template <int K, int L, bool FULL, bool OUT>
@@ -343,9 +352,12 @@ mw_r2_kernel(FxArgs a, const double2 *src, double2 *dst)
 {
     const uint32_t H = a.N >> 1, j = blockIdx.x * 256 + threadIdx.x, b = blockIdx.y;
     if (j >= H) return;
-    const CW_<K> v0 = w_load<K, L, IN>(a, src, b, j), v1 = w_load<K, L, IN>(a, src, b, j + H);
-    w_store<K, L, FULL, OUT>(a, dst, b, 2 * j, w_add<K>(v0, v1));
-    w_store<K, L, FULL, OUT>(a, dst, b, 2 * j + 1, w_sub<K>(v0, v1));
+    uint32_t bad = 0;
+    CW_<K> v[2];
+    w_load<K, L, IN>(a, src, b, {j, j + H}, v, bad);
+    if (bad) atomicOr(a.flag, 1u);
+    w_store<K, L, FULL, OUT>(a, dst, b, 2 * j, w_add<K>(v[0], v[1]));
+    w_store<K, L, FULL, OUT>(a, dst, b, 2 * j + 1, w_sub<K>(v[0], v[1]));
 }
 
 // Radix-4 Stockham pass of sub-transform size Ns: thread j (< N/4) reads
@@ -361,8 +373,9 @@ mw_r4_kernel(FxArgs a, const double2 *src, double2 *dst, uint32_t Ns, uint32_t t
     if (j >= Q) return;
     const uint32_t k = j & (Ns - 1);
     CW_<K> v[4];
-#pragma unroll
-    for (int r = 0; r < 4; r++) v[r] = w_load<K, L, IN>(a, src, b, j + r * Q);
+    uint32_t bad = 0;
+    w_load<K, L, IN>(a, src, b, {j, j + Q, j + 2 * Q, j + 3 * Q}, v, bad);
+    if (bad) atomicOr(a.flag, 1u);
     if (k) {
         const double2 *tw = reinterpret_cast<const double2 *>(a.tw);
 #pragma unroll
@@ -466,8 +479,10 @@ mw_r16_kernel(FxArgs a, const double2 *src, double2 *dst, uint32_t Ns, uint32_t 
     };
     CW_<K> v[4];
     if (live) {
-#pragma unroll
-        for (int r2 = 0; r2 < 4; r2++) v[r2] = w_load<K, L, IN>(a, src, b, j + (w + 4 * r2) * S);
+        uint32_t bad = 0;
+        w_load<K, L, IN>(a, src, b, {j + w * S, j + (w + 4) * S, j + (w + 8) * S, j + (w + 12) * S},
+                         v, bad);
+        if (bad) atomicOr(a.flag, 1u);
         if (k) {
 #pragma unroll
             for (int r2 = 0; r2 < 4; r2++) {
@@ -503,7 +518,11 @@ mw_copy_kernel(FxArgs a)
 {
     const uint32_t b = blockIdx.x * 256 + threadIdx.x;
     if (b >= a.batch) return;
-    w_store<K, L, FULL, true>(a, nullptr, b, 0, w_load<K, L, true>(a, nullptr, b, 0));
+    uint32_t bad = 0;
+    CW_<K> v[1];
+    w_load<K, L, true>(a, nullptr, b, {0u}, v, bad);
+    if (bad) atomicOr(a.flag, 1u);
+    w_store<K, L, FULL, true>(a, nullptr, b, 0, v[0]);
 }
 
 template <int K, int L, bool FULL, bool IN, bool OUT>
