@@ -191,13 +191,11 @@ def word_fp64_ops(n, K):
     dft4 = 8 * cadd
     lg = n.bit_length() - 1
     plan = []
-    if K == 2 and lg >= 5:
-        plan.append(2 if lg & 1 else 4)
-        m = lg - (1 if lg & 1 else 2) - 2
-        if m % 4:
-            plan.append(4)
-        plan += [16] * (m // 4)
-        plan.append(4)
+    if K == 2 and lg >= 4:
+        plan += [2] if lg & 1 else []
+        plan.append(16)
+        plan += [4] if lg & 2 else []
+        plan += [16] * (lg // 4 - 1)
     else:
         if lg & 1:
             plan.append(2)
