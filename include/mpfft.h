@@ -121,18 +121,21 @@ typedef struct mpfft_plan_s *mpfft_plan;
                               operation an error-free transformation on the
                               FP64 units (TwoSum, FMA residual products);
                               same SoA-limb I/O, the result rounded once to
-                              prec bits; |exponent| of the input <= 900,
-                              checked by mpfft_plan_check (MPFFT_ERR_VALUE);
-                              normwise error against the exact transform
-                              below 2^-prec + (2 + 3 log2 n) 2^-104       */
+                              prec bits; a block exponent per transform as
+                              in the fixed form (a real more than 2^900
+                              below the transform's largest counts as
+                              zero); normwise error against the exact
+                              transform below 2^-prec + (2 + 3 log2 n)
+                              2^-104                                      */
 #define MPFFT_FORM_QD 12  /* quad-word form, 53 <= prec <= 212: every real
                               c0 + c1 + c2 + c3 (four binary64), sums and
                               products by error-free transformations with
                               the order-3 terms rounded and renormalised;
                               same SoA-limb I/O, result rounded to prec
-                              bits; |exponent| of the input <= 800
-                              (MPFFT_ERR_VALUE); normwise error below
-                              2^-prec + (8 + 32 log2 n) 2^-212            */
+                              bits; a block exponent per transform (a real
+                              more than 2^800 below the transform's
+                              largest counts as zero); normwise error
+                              below 2^-prec + (8 + 32 log2 n) 2^-212      */
 #define MPFFT_FORM_AUTO 13 /* the chosen form: resolved at plan creation to
                               the fastest form whose error against a
                               2P-bit oracle measured no worse than the plain
