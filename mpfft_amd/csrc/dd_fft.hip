@@ -362,7 +362,7 @@ mw_r2_kernel(FxArgs a, const double2 *src, double2 *dst)
 
 // Radix-4 Stockham pass of sub-transform size Ns: thread j (< N/4) reads
 // v_r = x[j + r N/4], multiplies v_r by w^(r k N / (4 Ns)), k = j mod Ns
-// (table entry t = toff + 3k + r - 1: K double2, the real part's K/2 then
+// (table entry t = toff + (r - 1) Ns + k: K double2, the real part's K/2 then
 // the imaginary part's), and writes y_q = sum_r v_r (-i)^(r q) to
 // (j - k) 4 + k + q Ns.
 template <int K, int L, bool FULL, bool IN, bool OUT>
@@ -380,7 +380,7 @@ mw_r4_kernel(FxArgs a, const double2 *src, double2 *dst, uint32_t Ns, uint32_t t
         const double2 *tw = reinterpret_cast<const double2 *>(a.tw);
 #pragma unroll
         for (int r = 1; r < 4; r++) {
-            const double2 *t = tw + (size_t)K * (toff + 3 * k + r - 1);
+            const double2 *t = tw + (size_t)K * (toff + (r - 1) * Ns + k);
             CW_<K> w;
             Word<K>::get(w.re, t);
             Word<K>::get(w.im, t + K / 2);
@@ -449,7 +449,7 @@ __device__ __forceinline__ void w_xget(CW_<K> &v, double2 (*x)[64], uint32_t l)
 // waves.  Butterfly g = 64 blockIdx + lane is j = g mod N/16 of transform
 // g / (N/16) (small transforms share a block).  Wave w reads v_r = x[j + r
 // N/16] for r = w + 4 r2 (r2 = 0 .. 3), multiplies v_r by w^(r k N / (16 Ns)),
-// k = j mod Ns (entry toff + 15k + r - 1), takes the DFT4 over r2 (the DFT16
+// k = j mod Ns (entry toff + (r - 1) Ns + k), takes the DFT4 over r2 (the DFT16
 // as 4 x 4: U_w(q1) = sum_r2 v_(w + 4 r2) (-i)^(r2 q1)) and the internal
 // twiddles w16^(w q1) (entries cent + w q1; w16^4 = -i exactly), and puts
 // U_w(q1) into the exchange buffer.  After the barrier wave w holds q1 = w:
@@ -487,7 +487,7 @@ mw_r16_kernel(FxArgs a, const double2 *src, double2 *dst, uint32_t Ns, uint32_t 
 #pragma unroll
             for (int r2 = 0; r2 < 4; r2++) {
                 const uint32_t r = w + 4 * r2;
-                if (r) v[r2] = w_mul<K>(v[r2], entry(toff + 15 * k + r - 1));
+                if (r) v[r2] = w_mul<K>(v[r2], entry(toff + (r - 1) * Ns + k));
             }
         }
         w_dft4<K>(v[0], v[1], v[2], v[3]);     // v[q1] = U_w(q1)

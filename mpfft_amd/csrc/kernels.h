@@ -477,7 +477,8 @@ hipError_t dd_fft_launch(int K, int L, int P, const FxArgs &a, int s0, int s1, h
 // splits the limb input into words and the last rounds back to limbs; the
 // radix-2 kernel has no twiddles, so it runs only as a first pass (Ns = 1).
 // Pass i has radix R and sub-transform size Ns; its twiddles w^(r k N / (R
-// Ns)) (k < Ns, r = 1 .. R-1) are table entries toff + (R - 1) k + r - 1.
+// Ns)) (k < Ns, r = 1 .. R-1) are table entries toff + (r - 1) Ns + k (k
+// fastest: the lanes of a wave, consecutive k, read consecutive entries).
 // With a radix-16 pass, ten entries from *nent on hold w16^m = w^(m N / 16),
 // m = 0 .. 9, the internal twiddles of the 4 x 4 butterfly.
 struct MwPass {

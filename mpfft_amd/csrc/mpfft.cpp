@@ -780,11 +780,11 @@ int build_fx_plan(mpfft_plan_s *p)
     return MPFFT_OK;
 }
 
-// Double- and quad-word forms: twiddles of the radix-4 passes (the per-lane
-// fixed form's entry order: pass after pass, entry 3k + r - 1 =
-// w^(r k N / (4 Ns))) as K words of the real part, then K of the imaginary
-// part, each word the binary64 nearest to what the words before it leave of
-// the value (MPFR at 512 bits), and the word ping-pong buffers.
+// Double- and quad-word forms: twiddles of the passes (pass after pass,
+// entry (r - 1) Ns + k = w^(r k N / (R Ns)), k fastest) as K words of the
+// real part, then K of the imaginary part, each word the binary64 nearest to
+// what the words before it leave of the value (MPFR at 512 bits), and the
+// word ping-pong buffers.
 int build_dd_plan(mpfft_plan_s *p)
 {
     const int K = word_count(p->form);
@@ -798,8 +798,8 @@ int build_dd_plan(mpfft_plan_s *p)
     std::vector<unsigned long> ent;
     bool r16 = false;
     for (int i = 0; i < np; i++) {
-        for (unsigned long k = 0; k < ps[i].Ns; k++)
-            for (unsigned long r = 1; r < ps[i].R; r++)
+        for (unsigned long r = 1; r < ps[i].R; r++)
+            for (unsigned long k = 0; k < ps[i].Ns; k++)
                 ent.push_back(r * k * (N / ((unsigned long)ps[i].R * ps[i].Ns)));
         r16 |= ps[i].R == 16;
     }

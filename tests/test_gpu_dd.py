@@ -125,6 +125,21 @@ def test_dd_batch_and_stages():
     plan.close()
 
 
+@pytest.mark.parametrize("n", [16, 32, 64, 128])
+def test_dd_many_small_transforms(n):
+    # small transforms share the radix-16 pass's blocks (64 butterflies per
+    # block, butterfly g of the grid is butterfly g mod n/16 of transform
+    # g / (n/16)): 100 transforms fill several blocks and leave the last one
+    # partial; each result is within the bound, and equals its transform run
+    # alone bit for bit
+    P, b = 106, 100
+    xs = [I.mp_random(n, P, seed=9100 + 7 * i + n) for i in range(b)]
+    ys = run(n, P, xs)
+    for i in (0, 1, 37, 63, 64, 99):
+        check(n, P, xs[i], ys[i])
+        assert np.array_equal(ys[i], run(n, P, [xs[i]])[0]), i
+
+
 def test_dd_precision_and_range_checked():
     for P in (52, 107, 128):
         with pytest.raises(M.MpfftError) as e:
