@@ -191,7 +191,7 @@ def word_fp64_ops(n, K):
     dft4 = 8 * cadd
     lg = n.bit_length() - 1
     plan = []
-    if K == 2 and lg >= 4:
+    if lg >= 4:
         plan += [2] if lg & 1 else []
         plan.append(16)
         plan += [4] if lg & 2 else []

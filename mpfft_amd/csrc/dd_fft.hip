@@ -16,8 +16,8 @@
 // Quad-word numbers c0 + c1 + c2 + c3 follow the same principles (see
 // below).
 //
-// Structure: Stockham passes (mw_plan in kernels.h): radix 16 for the
-// double-word form, radix 4 for the quad-word form.  The first pass reads
+// Structure: Stockham passes (mw_plan in kernels.h), radix 16 on four waves
+// per butterfly for both forms.  The first pass reads
 // the SoA-limb input and splits every real into its words exactly; the last
 // pass rounds the words' sum to P bits and writes SoA limbs (double-word:
 // the correctly rounded sum of two binary64 numbers, dd_conv.h; quad-word:
@@ -537,7 +537,7 @@ void launch_pass(const FxArgs &a, const MwPass &ps, uint32_t cent, const double2
     else if (ps.R == 4)
         hipLaunchKernelGGL((mw_r4_kernel<K, L, FULL, IN, OUT>), grid, dim3(256), 0, st, a, src,
                            dst, ps.Ns, ps.toff);
-    else if constexpr (K == 2) {
+    else {
         // four waves per block, one butterfly per lane
         uint32_t lgS = 0;
         while ((2u << lgS) <= a.N >> 4) lgS++;
@@ -562,7 +562,6 @@ hipError_t launch_mw(const FxArgs &a, int s0, int s1, hipStream_t st)
     const int npass = mw_plan(N, K, ps, &cent);
     double2 *src = reinterpret_cast<double2 *>(a.buf1), *dst = reinterpret_cast<double2 *>(a.buf0);
     for (int s = 0; s < npass; s++) {
-        if (ps[s].R == 16 && K != 2) return hipErrorInvalidValue;
         if (s >= s0 && s < s1) {
             const bool first = s == 0, last = s == npass - 1;
             if (first && last) launch_pass<K, L, FULL, true, true>(a, ps[s], cent, src, dst, st);

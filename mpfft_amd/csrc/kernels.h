@@ -468,14 +468,15 @@ hipError_t fxs_fft_launch(int L, int P, const FxArgs &a, const FxsPass *ps, int 
 // input, the last rounds to P bits); N = 1: one.
 hipError_t dd_fft_launch(int K, int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st);
 
-// Passes of the word forms.  Double-word (K = 2): radix 16 (four threads per
-// butterfly), with a radix-2 pass first where log2 N is odd and a radix-4
-// pass after the first radix-16 one where bit 1 of log2 N is set (a
-// radix-16 pass with one thread per butterfly ran at three waves per SIMD,
-// and converting at one); quad-word (K = 4, twice the registers per value):
-// radix 4 after a radix-2 pass when log2 N is odd.  The first pass also
-// splits the limb input into words and the last rounds back to limbs; the
-// radix-2 kernel has no twiddles, so it runs only as a first pass (Ns = 1).
+// Passes of the word forms (double and quad words alike): radix 16, four
+// threads per butterfly, with a radix-2 pass first where log2 N is odd and a
+// radix-4 pass after the first radix-16 one where bit 1 of log2 N is set;
+// radix 2 and 4 alone below N = 16.  (A radix-16 pass with one thread per
+// butterfly ran at three waves per SIMD for double words and converting at
+// one; the quad-word form ran radix 4 throughout, 16 % slower at N = 2^16.)
+// The first pass also splits the limb input into words and the last rounds
+// back to limbs; the radix-2 kernel has no twiddles, so it runs only as a
+// first pass (Ns = 1).
 // Pass i has radix R and sub-transform size Ns; its twiddles w^(r k N / (R
 // Ns)) (k < Ns, r = 1 .. R-1) are table entries toff + (r - 1) Ns + k (k
 // fastest: the lanes of a wave, consecutive k, read consecutive entries).
@@ -496,7 +497,8 @@ inline int mw_plan(uint32_t N, int K, MwPass (&p)[32], uint32_t *nent)
         toff += (R - 1) * Ns;
         Ns *= R;
     };
-    if (K == 2 && lg >= 4) {
+    (void)K;                                         // the same plan for K = 2 and 4
+    if (lg >= 4) {
         if (lg & 1) add(2);
         add(16);
         if (lg & 2) add(4);

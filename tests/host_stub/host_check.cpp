@@ -71,21 +71,20 @@ static void plans()
             const int np = mw_plan(N, K, p, &nent);
             for (int i = 0; i < np; i++) {
                 CHECK(p[i].Ns == prod && p[i].toff == ent, "mw K=%d lg=%d pass %d", K, lg, i);
-                // radix 16 only for double words
-                CHECK(p[i].R != 16 || (K == 2 && N >= 16), "mw K=%d lg=%d r16", K, lg);
+                CHECK(p[i].R != 16 || N >= 16, "mw K=%d lg=%d r16", K, lg);
                 // the radix-2 kernel has no twiddles: first pass only
                 CHECK(p[i].R != 2 || (i == 0 && p[i].Ns == 1), "mw K=%d lg=%d r2 at %d", K, lg, i);
-                // double words: radix 4 only right after the first radix 16
-                CHECK(K != 2 || lg < 4 || p[i].R != 4 ||
+                // radix 4 only right after the first radix 16 (from N = 16 on)
+                CHECK(lg < 4 || p[i].R != 4 ||
                           (i > 0 && p[i - 1].R == 16 && p[i - 1].Ns == (p[0].R == 2 ? 2u : 1u)),
                       "mw K=%d lg=%d r4 at %d", K, lg, i);
                 ent += (p[i].R - 1) * p[i].Ns;
                 prod *= p[i].R;
             }
             CHECK(prod == N && ent == nent, "mw K=%d lg=%d product", K, lg);
-            // double words: one pass per radix-16 factor, plus one for each of
-            // bits 0 and 1 of log2 N
-            CHECK(K != 2 || lg < 4 || np == lg / 4 + (lg & 1) + ((lg >> 1) & 1),
+            // one pass per radix-16 factor, plus one for each of bits 0 and 1
+            // of log2 N
+            CHECK(lg < 4 || np == lg / 4 + (lg & 1) + ((lg >> 1) & 1),
                   "mw K=%d lg=%d passes %d", K, lg, np);
         }
     // panels: a partial last panel is padded; the preferred rule

@@ -215,7 +215,7 @@ def test_qd_batch_stages_and_checks():
         qcheck(n, P, x, y)
     plan = M.Plan(n, P, M.FORM_QD, b)
     S = plan.stages()
-    assert S == 1 + 4                    # log2 n = 9: radix 2, then four radix-4 passes
+    assert S == 3                        # log2 n = 9: radix 2, 16, 16
     x = np.concatenate([I.relimb(t, n, O.limbs(P), plan.L) for t in xs])
     dx = torch.from_numpy(x.view(np.int32)).cuda()
     d1, d2 = torch.empty_like(dx), torch.empty_like(dx)
