@@ -132,8 +132,22 @@ __device__ __forceinline__ int32_t wave_max(int32_t v)
     return v;
 }
 
+// eoff[b] from the largest exponent m of transform b's inputs: 0 when all
+// are zero; an m outside the limb format's input range is flagged
+__device__ __forceinline__ void eoff_store(const FxArgs &a, uint32_t b, int32_t m)
+{
+    if (m == kNoExp) m = 0;                                  // all zero
+    if (m >= kInputExpLimit || m <= -kInputExpLimit) {
+        atomicOr(a.flag, 1u);
+        m = 0;
+    }
+    a.eoff[b] = m;
+}
+
 // grid (eoff_nblk, batch): block c of transform b writes the largest
-// exponent of its part of the 2N header words to eoff_part[b nblk + c]
+// exponent of its part of the 2N header words to eoff_part[b nblk + c];
+// with one block per transform (2N <= 8192) it writes eoff[b] itself and
+// the final kernel is not launched
 __global__ void __launch_bounds__(256)
 fx_eoff_part_kernel(FxArgs a)
 {
@@ -161,7 +175,8 @@ fx_eoff_part_kernel(FxArgs a)
         m = red[0];
 #pragma unroll
         for (int w = 1; w < 4; w++) m = red[w] > m ? red[w] : m;
-        a.eoff_part[(size_t)b * nb + c] = m;
+        if (nb == 1) eoff_store(a, b, m);
+        else a.eoff_part[(size_t)b * nb + c] = m;
     }
 }
 
@@ -175,19 +190,16 @@ fx_eoff_final_kernel(FxArgs a)
     int32_t m = kNoExp;
 #pragma unroll 1
     for (uint32_t c = 0; c < a.eoff_nblk; c++) m = p[c] > m ? p[c] : m;
-    if (m == kNoExp) m = 0;                                  // all zero
-    if (m >= kInputExpLimit || m <= -kInputExpLimit) {
-        atomicOr(a.flag, 1u);
-        m = 0;
-    }
-    a.eoff[b] = m;
+    eoff_store(a, b, m);
 }
 
 hipError_t fx_eoff_launch(const FxArgs &a, hipStream_t st)
 {
     if (a.eoff_nblk != fx_eoff_blocks(a.N)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(fx_eoff_part_kernel, dim3(a.eoff_nblk, a.batch), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(fx_eoff_final_kernel, dim3((a.batch + 255) / 256), dim3(256), 0, st, a);
+    if (a.eoff_nblk > 1)
+        hipLaunchKernelGGL(fx_eoff_final_kernel, dim3((a.batch + 255) / 256), dim3(256), 0, st,
+                           a);
     return hipGetLastError();
 }
 

@@ -142,8 +142,8 @@ inline uint32_t fx_eoff_blocks(uint32_t N)
 }
 
 // The block-exponent scan of the fixed and word forms: eoff[b] for every
-// transform of a.in (two launches, no memset: it can be captured in a
-// graph).  Flags (a.flag) a transform whose largest exponent is outside
+// transform of a.in (two launches, one where 2N <= 8192; no memset: it can
+// be captured in a graph).  Flags (a.flag) a transform whose largest exponent is outside
 // the limb format's input range.
 hipError_t fx_eoff_launch(const FxArgs &a, hipStream_t st);
 
