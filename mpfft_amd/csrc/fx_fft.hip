@@ -306,7 +306,215 @@ fx_r2_kernel(FxArgs a, const uint32_t *src, uint32_t *dst)
     fx_pass_store<L, W, FULL, IO>(re, im, a, dst, b, 2 * j + r);
 }
 
+// (re, im) *= c + i d, table entry q: three products (Gauss), k3 = im (c + d),
+// k2 = re (d - c), k1 = (re + im) c; re' = k1 - k3, im' = k1 + k2
+template <int W>
+__device__ __forceinline__ void fx_cmul_tab(Fx<W> &re, Fx<W> &im, const uint32_t *tw,
+                                            uint32_t TT, uint32_t q)
+{
+    const uint32_t MT = 3 * TT;
+    uint32_t T[FxDig<W>::DT];
+    Fx<W> s, k2, k3;
+    fx_add(s, re, im);
+    uint32_t tn = tw_load<W>(T, tw, MT, 2 * TT + q);
+    fx_mul(k3, im, T, tn);
+    tn = tw_load<W>(T, tw, MT, TT + q);
+    fx_mul(k2, re, T, tn);
+    tn = tw_load<W>(T, tw, MT, q);
+    fx_mul(s, s, T, tn);
+    fx_sub(re, s, k3);
+    fx_add(im, s, k2);
+}
+
+// DFT4 in registers: v_q <- sum_r v_r (-i)^(r q), exact
+template <int W>
+__device__ __forceinline__ void fx_dft4_reg(Fx<W> (&re)[4], Fx<W> (&im)[4])
+{
+    Fx<W> t0r, t0i, t1r, t1i, t2r, t2i, t3r, t3i;
+    fx_add(t0r, re[0], re[2]);
+    fx_add(t0i, im[0], im[2]);
+    fx_sub(t1r, re[0], re[2]);
+    fx_sub(t1i, im[0], im[2]);
+    fx_add(t2r, re[1], re[3]);
+    fx_add(t2i, im[1], im[3]);
+    fx_sub(t3r, re[1], re[3]);
+    fx_sub(t3i, im[1], im[3]);
+    fx_add(re[0], t0r, t2r);                     // t0 + t2
+    fx_add(im[0], t0i, t2i);
+    fx_sub(re[2], t0r, t2r);                     // t0 - t2
+    fx_sub(im[2], t0i, t2i);
+    fx_add(re[1], t1r, t3i);                     // t1 - i t3
+    fx_sub(im[1], t1i, t3r);
+    fx_sub(re[3], t1r, t3i);                     // t1 + i t3
+    fx_add(im[3], t1i, t3r);
+}
+
+// Radix-16 pass of sub-transform size Ns, one butterfly on four waves (up
+// to kFxl16MaxW words per real).  Butterfly g = 64 blockIdx + lane is j = g
+// mod N/16 of transform g / (N/16) (small transforms share a block).  Wave w
+// holds positions r = w + 4 r2 (r2 = 0 .. 3): it loads v_r = x[j + r N/16]
+// (all four before converting any, IO bit 1), multiplies v_r by w^(r k N /
+// (16 Ns)), k = j mod Ns (entry toff + (r - 1) Ns + k), takes the DFT4 over
+// r2 and the internal twiddles w16^(w q1) (entries cent + w q1; w16^4 = -i
+// exactly), and puts U_w(q1) into LDS slot 4 w + q1; after the barrier wave
+// w takes the DFT4 over r1 of U_r1(w) and writes y_(w + 4 q2) to (j - k) 16 +
+// k + (w + 4 q2) Ns (rounding to limbs, IO bit 2).  LDS: 16 slots of 2 W
+// words x 64 lanes.
+template <int L, int W, bool FULL, int IO>
+__global__ void __launch_bounds__(256)
+fx_r16_kernel(FxArgs a, const uint32_t *src, uint32_t *dst, uint32_t Ns, uint32_t toff,
+              uint32_t cent, uint32_t lgS)
+{
+    extern __shared__ uint32_t xs[];
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint64_t g = (uint64_t)blockIdx.x * 64 + lane;
+    const uint32_t N = a.N, M = 2 * N, S = N >> 4;
+    const bool live = g < ((uint64_t)a.batch << lgS);
+    const uint32_t b = live ? (uint32_t)(g >> lgS) : 0u, j = (uint32_t)g & (S - 1);
+    const uint32_t k = j & (Ns - 1);
+    auto slot = [&](uint32_t sl, uint32_t word) { return xs + ((sl * 2 * W + word) << 6) + lane; };
+    Fx<W> re[4], im[4];
+    if (live) {
+        if constexpr ((IO & 1) != 0) {
+            const uint32_t *in = a.in + a.in_stride * b;
+            const int32_t eo = a.eoff[b];
+            Mp<L> xr[4], xi[4];
+#pragma unroll
+            for (int r2 = 0; r2 < 4; r2++) {
+                mp_load(xr[r2], in, M, j + (w + 4 * r2) * S);
+                mp_load(xi[r2], in, M, N + j + (w + 4 * r2) * S);
+            }
+            uint32_t bad = 0;
+#pragma unroll
+            for (int r2 = 0; r2 < 4; r2++) {
+                bad |= fx_from_mp<L, W>(re[r2], xr[r2], a.lim, eo);
+                bad |= fx_from_mp<L, W>(im[r2], xi[r2], a.lim, eo);
+            }
+            if (bad) atomicOr(a.flag, 1u);
+        } else {
+            const uint32_t *sb = src + a.stride * b;
+#pragma unroll
+            for (int r2 = 0; r2 < 4; r2++) {
+                fx_load(re[r2], sb, M, j + (w + 4 * r2) * S);
+                fx_load(im[r2], sb, M, N + j + (w + 4 * r2) * S);
+            }
+        }
+        if (k) {
+#pragma unroll
+            for (int r2 = 0; r2 < 4; r2++) {
+                const uint32_t r = w + 4 * r2;
+                if (r) fx_cmul_tab<W>(re[r2], im[r2], a.tw, a.ntw, toff + (r - 1) * Ns + k);
+            }
+        }
+        fx_dft4_reg<W>(re, im);                    // [q1] = U_w(q1)
+#pragma unroll
+        for (int q1 = 1; q1 < 4; q1++) {
+            const uint32_t m = w * q1;
+            if (m == 4) {                          // -i (x + i y) = y - i x
+                Fx<W> z;
+#pragma unroll
+                for (int t = 0; t < W; t++) z.w[t] = 0u;
+                Fx<W> nre;
+                fx_sub(nre, z, re[q1]);
+                re[q1] = im[q1];
+                im[q1] = nre;
+            } else if (m) {
+                fx_cmul_tab<W>(re[q1], im[q1], a.tw, a.ntw, cent + m);
+            }
+        }
+#pragma unroll
+        for (int q1 = 0; q1 < 4; q1++)
+#pragma unroll
+            for (int t = 0; t < W; t++) {
+                *slot(4 * w + q1, t) = re[q1].w[t];
+                *slot(4 * w + q1, W + t) = im[q1].w[t];
+            }
+    }
+    __syncthreads();
+    if (live) {
+#pragma unroll
+        for (int r1 = 0; r1 < 4; r1++)
+#pragma unroll
+            for (int t = 0; t < W; t++) {
+                re[r1].w[t] = *slot(4 * r1 + w, t);
+                im[r1].w[t] = *slot(4 * r1 + w, W + t);
+            }
+        fx_dft4_reg<W>(re, im);                    // [q2] = y_(w + 4 q2)
+    }
+    if (Ns >= 64) {
+        // a wave's outputs y_q lie on 64 consecutive positions of every row
+        if (!live) return;
+        const uint32_t o = (j - k) * 16 + k;
+        uint32_t *db = dst + a.stride * b;
+#pragma unroll
+        for (int q2 = 0; q2 < 4; q2++)
+            fx_pass_store<L, W, FULL, IO>(re[q2], im[q2], a, db, b, o + (w + 4 * q2) * Ns);
+        return;
+    }
+    // Ns < 64: the block's outputs fill 1024 consecutive positions of the
+    // (transform, position) order, output q of lane l at e = 16 (l - k) + k
+    // + q Ns, but a wave's stores would touch them 4 bytes in 4 Ns: they
+    // go through LDS (row t of the words at t 1024, each 32-word group
+    // rotated by its index so neither side conflicts on the banks) and
+    // leave as 64 consecutive words per store
+    auto swz = [](uint32_t e) { return (e & ~31u) | ((e + (e >> 5)) & 31u); };
+    __syncthreads();                               // the slots have been read
+    if (live) {
+#pragma unroll
+        for (int q2 = 0; q2 < 4; q2++) {
+            const uint32_t e = swz(16 * (lane - k) + k + (w + 4 * q2) * Ns);
+#pragma unroll
+            for (int t = 0; t < W; t++) {
+                xs[t * 1024 + e] = re[q2].w[t];
+                xs[(W + t) * 1024 + e] = im[q2].w[t];
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t lgN = lgS + 4;
+    const uint64_t total = (uint64_t)a.batch << lgN;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const uint32_t e = 256 * m + 64 * w + lane;
+        const uint64_t G = (uint64_t)blockIdx.x * 1024 + e;
+        if (G >= total) break;
+        const uint32_t bb = (uint32_t)(G >> lgN), pos = (uint32_t)G & (N - 1), es = swz(e);
+        Fx<W> yr, yi;
+#pragma unroll
+        for (int t = 0; t < W; t++) {
+            yr.w[t] = xs[t * 1024 + es];
+            yi.w[t] = xs[(W + t) * 1024 + es];
+        }
+        fx_pass_store<L, W, FULL, IO>(yr, yi, a, dst + a.stride * bb, bb, pos);
+    }
+}
+
 // ---------------------------------------------------------------------------
+
+// one pass of the plan (fxl_plan in kernels.h); IO: bit 1 converts the
+// input, bit 2 the output
+template <int L, int W, bool FULL, int IO>
+static void launch_fx_pass(const FxArgs &a, const MwPass &P, uint32_t cent, const uint32_t *src,
+                           uint32_t *dst, hipStream_t st)
+{
+    const uint32_t N = a.N;
+    if (P.R == 2) {
+        dim3 g(((N / 2) * 2 + 255) / 256 * a.batch);
+        hipLaunchKernelGGL((fx_r2_kernel<L, W, FULL, IO>), g, dim3(256), 0, st, a, src, dst);
+    } else if (P.R == 4) {
+        dim3 g(((N / 4) * 4 + 255) / 256 * a.batch);
+        const size_t sh = sizeof(uint32_t) * 2 * W * 256;     // two parking slots per lane
+        hipLaunchKernelGGL((fx_r4_kernel<L, W, FULL, IO>), g, dim3(256), sh, st, a, src, dst,
+                           P.Ns, P.toff);
+    } else if constexpr (W <= kFxl16MaxW) {
+        uint32_t lgS = 0;
+        while ((2u << lgS) <= N >> 4) lgS++;
+        const uint64_t bf = (uint64_t)a.batch << lgS;
+        const size_t sh = sizeof(uint32_t) * 16 * 2 * W * 64;
+        hipLaunchKernelGGL((fx_r16_kernel<L, W, FULL, IO>), dim3((uint32_t)((bf + 63) / 64)),
+                           dim3(256), sh, st, a, src, dst, P.Ns, P.toff, cent, lgS);
+    }
+}
 
 // Stages [s0, s1) of the transform: one per pass (buffer 0 -> 1 -> 0 ...),
 // the first converting the input on the fly, the last the output.  N = 1
@@ -316,9 +524,9 @@ static hipError_t launch_fx(const FxArgs &a, int s0, int s1, hipStream_t st)
 {
     static_assert(FxDig<W>::B == fx_lane_digit_bits(W), "host twiddle digits differ");
     const uint32_t N = a.N, M = 2 * N;
-    uint32_t lg = 0;
-    while ((1u << lg) < N) lg++;
-    const int npass = fx_passes(N);
+    MwPass ps[32];
+    uint32_t nent = 0;
+    const int npass = N > 1 ? fxl_plan(N, W, ps, &nent) : 0;
     auto in_range = [&](int s) { return s >= s0 && s < s1; };
     if (npass == 0) {
         dim3 gc((M + 255) / 256, a.batch);
@@ -329,40 +537,16 @@ static hipError_t launch_fx(const FxArgs &a, int s0, int s1, hipStream_t st)
         return hipGetLastError();
     }
     uint32_t *src = a.buf1, *dst = a.buf0;
-    int stage = 0;
-    uint32_t Ns = 1;
-    if (lg & 1) {
-        dim3 g(((N / 2) * 2 + 255) / 256 * a.batch);
-        if (in_range(stage)) {
-            if (npass == 1)
-                hipLaunchKernelGGL((fx_r2_kernel<L, W, FULL, 3>), g, dim3(256), 0, st, a,
-                                   (const uint32_t *)src, dst);
-            else
-                hipLaunchKernelGGL((fx_r2_kernel<L, W, FULL, 1>), g, dim3(256), 0, st, a,
-                                   (const uint32_t *)src, dst);
-        }
-        std::swap(src, dst);
-        Ns = 2;
-        stage++;
-    }
-    for (uint32_t toff = 0; Ns < N; toff += 3 * Ns, Ns *= 4, stage++) {
-        dim3 g(((N / 4) * 4 + 255) / 256 * a.batch);
-        const size_t sh = sizeof(uint32_t) * 2 * W * 256;     // two parking slots per lane
-        const bool first = stage == 0, last = stage == npass - 1;
-        if (in_range(stage)) {
-            const uint32_t *s_ = src;
-            if (first && last)
-                hipLaunchKernelGGL((fx_r4_kernel<L, W, FULL, 3>), g, dim3(256), sh, st, a, s_,
-                                   dst, Ns, toff);
-            else if (first)
-                hipLaunchKernelGGL((fx_r4_kernel<L, W, FULL, 1>), g, dim3(256), sh, st, a, s_,
-                                   dst, Ns, toff);
-            else if (last)
-                hipLaunchKernelGGL((fx_r4_kernel<L, W, FULL, 2>), g, dim3(256), sh, st, a, s_,
-                                   dst, Ns, toff);
-            else
-                hipLaunchKernelGGL((fx_r4_kernel<L, W, FULL, 0>), g, dim3(256), sh, st, a, s_,
-                                   dst, Ns, toff);
+    for (int s = 0; s < npass; s++) {
+        const MwPass &P = ps[s];
+        // the radix-2 kernel has no twiddles; radix 16 only where the plan puts it
+        if ((P.R == 2 && P.Ns != 1) || (P.R == 16 && W > kFxl16MaxW)) return hipErrorInvalidValue;
+        if (in_range(s)) {
+            const bool first = s == 0, last = s == npass - 1;
+            if (first && last) launch_fx_pass<L, W, FULL, 3>(a, P, nent, src, dst, st);
+            else if (first) launch_fx_pass<L, W, FULL, 1>(a, P, nent, src, dst, st);
+            else if (last) launch_fx_pass<L, W, FULL, 2>(a, P, nent, src, dst, st);
+            else launch_fx_pass<L, W, FULL, 0>(a, P, nent, src, dst, st);
         }
         std::swap(src, dst);
     }

@@ -154,18 +154,6 @@ constexpr int fx_words(int L) { return L + 2; }
 // while a real has at most 63 digits (W <= 57), 28 beyond
 constexpr int fx_lane_digit_bits(int W) { return (32 * W + 28) / 29 <= 63 ? 29 : 28; }
 
-// passes of the fixed form: a radix-2 pass when log2 N is odd, then radix 4
-inline int fx_passes(uint32_t N)
-{
-    int lg = 0;
-    while ((1u << lg) < N) lg++;
-    return (lg & 1) + lg / 2;
-}
-
-// stages of the fixed form: one per pass (the first converts the input, the
-// last the output); N = 1: input and output conversion
-inline int fx_stages(uint32_t N) { return fx_passes(N) > 0 ? fx_passes(N) : 2; }
-
 // stages [s0, s1) of the fixed form
 hipError_t fx_fft_launch(int L, int P, const FxArgs &a, int s0, int s1, hipStream_t st);
 
@@ -507,6 +495,36 @@ inline int mw_plan(uint32_t N, int K, MwPass (&p)[32], uint32_t *nent)
         if (lg & 1) add(2);
         while (Ns < N) add(4);
     }
+    *nent = toff;
+    return n;
+}
+
+// Passes of the fixed form on lanes (fx_fft.hip).  Up to kFxl16MaxW words
+// per real (P <= 256) and from N = 16 on, the word forms' plan: radix 16
+// (four waves per butterfly, a lane holding four positions), a radix-2 pass
+// first where log2 N is odd and a radix-4 pass after the first radix-16 one
+// where bit 1 of log2 N is set; wider reals (four positions per lane would
+// not fit the registers) and smaller N: a radix-2 pass first where log2 N
+// is odd, then radix 4.  Twiddle entries of a pass from toff: radix 16
+// r-major, (r - 1) Ns + k (the lanes of a wave hold consecutive k); radix 4
+// and 2 k-major, (R - 1) k + r - 1 (a DPP quad holds the r of one k); with a
+// radix-16 pass, ten entries from *nent hold w16^m, m = 0 .. 9.
+constexpr int kFxl16MaxW = 10;
+
+inline int fxl_plan(uint32_t N, int W, MwPass (&p)[32], uint32_t *nent)
+{
+    if (W <= kFxl16MaxW) return mw_plan(N, 2, p, nent);
+    int lg = 0;
+    while ((1u << lg) < N) lg++;
+    int n = 0;
+    uint32_t Ns = 1, toff = 0;
+    auto add = [&](uint32_t R) {
+        p[n++] = {R, Ns, toff};
+        toff += (R - 1) * Ns;
+        Ns *= R;
+    };
+    if (lg & 1) add(2);
+    while (Ns < N) add(4);
     *nent = toff;
     return n;
 }

@@ -450,7 +450,7 @@ struct mpfft_plan_s {
     uint32_t fx_ntw = 0;
     bool fx_panel = false;   // panel kernels (matrix-core twiddle products)
     int fx_prb = 2;          // panel kernels: log2 of the main radix (fxp_plan)
-    int fx_npass = 0;        // passes of the panel plan
+    int fx_npass = 0;        // passes of the panel or lane plan
     bool fx_fs = false;      // panel kernels: the four-step plan (fxs_plan)
     FxsPass fxs[kFxsMaxPasses] = {};
     uint32_t fxs_c16e = 0;   // its table entry of w^0 = 1
@@ -675,9 +675,27 @@ int build_fx_plan(mpfft_plan_s *p)
         for (unsigned long t : {N / 8, 0UL, N / 16, 3 * N / 16, 9 * N / 16, N / 8, 3 * N / 8})
             ent.push_back(t);
     } else {
-        for (unsigned long Ns = (lg0 & 1) ? 2 : 1; Ns < N; Ns *= 4)
-            for (unsigned long k = 0; k < Ns; k++)
-                for (unsigned long r = 1; r <= 3; r++) ent.push_back(r * k * (N / (4 * Ns)));
+        // the lane passes (fxl_plan): radix 16 r-major, radix 4 and 2 k-major,
+        // then w16^m, m = 0 .. 9, where a radix-16 pass needs them
+        MwPass ps[32];
+        uint32_t nent = 0;
+        const int np = N > 1 ? fxl_plan((uint32_t)N, (int)W, ps, &nent) : 0;
+        bool r16 = false;
+        for (int i = 0; i < np; i++) {
+            const unsigned long R = ps[i].R, Ns = ps[i].Ns;
+            if (R == 16) {
+                r16 = true;
+                for (unsigned long r = 1; r < R; r++)
+                    for (unsigned long k = 0; k < Ns; k++) ent.push_back(r * k * (N / (R * Ns)));
+            } else {
+                for (unsigned long k = 0; k < Ns; k++)
+                    for (unsigned long r = 1; r < R; r++) ent.push_back(r * k * (N / (R * Ns)));
+            }
+        }
+        if (ent.size() != nent) return MPFFT_ERR_ARG;
+        if (r16)
+            for (unsigned long m = 0; m < 10; m++) ent.push_back(m * (N / 16));
+        p->fx_npass = np;
     }
     const size_t T3 = ent.size(), MT = 3 * T3;
     const size_t tabn = p->fx_panel ? T3 * 3 * (size_t)G.RW : MT * (DT + 1);
@@ -1319,7 +1337,8 @@ int mpfft_plan_stages(mpfft_plan p)
     if (!p) return MPFFT_ERR_ARG;
     if (word_count(p->form)) return p->N == 1 ? 1 : p->fx_npass;
     if (is_fixed(p->form))
-        return p->fx_fs ? p->fx_npass : p->fx_panel ? p->fx_npass + 2 : fx_stages((uint32_t)p->N);
+        return p->fx_fs ? p->fx_npass : p->fx_panel ? p->fx_npass + 2
+                                      : (p->fx_npass > 0 ? p->fx_npass : 2);
     return is_mp(p->form) ? 1 + (int)p->levels.size() : MPFFT_ERR_FORM;
 }
 

@@ -87,6 +87,25 @@ static void plans()
             CHECK(lg < 4 || np == lg / 4 + (lg & 1) + ((lg >> 1) & 1),
                   "mw K=%d lg=%d passes %d", K, lg, np);
         }
+    // the fixed form on lanes: radix 16 up to kFxl16MaxW words (the word
+    // forms' plan), radix 2 (first only) and 4 beyond
+    for (int W : {3, 6, 10, 12, 34, 66})
+        for (int lg = 1; lg <= 24; lg++) {
+            const uint32_t N = 1u << lg;
+            MwPass p[32];
+            uint32_t nent = 0, prod = 1, ent = 0;
+            const int np = fxl_plan(N, W, p, &nent);
+            for (int i = 0; i < np; i++) {
+                CHECK(p[i].Ns == prod && p[i].toff == ent, "fxl W=%d lg=%d pass %d", W, lg, i);
+                CHECK(p[i].R != 16 || (W <= kFxl16MaxW && N >= 16), "fxl W=%d lg=%d r16", W, lg);
+                CHECK(p[i].R != 2 || (i == 0 && p[i].Ns == 1), "fxl W=%d lg=%d r2 at %d", W, lg, i);
+                ent += (p[i].R - 1) * p[i].Ns;
+                prod *= p[i].R;
+            }
+            CHECK(prod == N && ent == nent, "fxl W=%d lg=%d product", W, lg);
+            CHECK(W > kFxl16MaxW || lg < 4 || np == lg / 4 + (lg & 1) + ((lg >> 1) & 1),
+                  "fxl W=%d lg=%d passes %d", W, lg, np);
+        }
     // panels: a partial last panel is padded; the preferred rule
     CHECK(fxp_groups(1) == 1 && fxp_groups(64) == 1 && fxp_groups(65) == 2, "groups");
     CHECK(fxp_preferred(1024, 64, 10) && fxp_preferred(1024, 120, 10) &&
