@@ -91,22 +91,25 @@ typedef struct mpfft_plan_s *mpfft_plan;
                               butterfly for every k (serial_mp_fft1.c)      */
 #define MPFFT_FORM_MP_C2 2 /* multiprecision, CODE 2 op order: 4m2a, special
                               butterflies k = 0 and n/8 (serial_mp_fft2.c)  */
-#define MPFFT_FORM_FIXED 8 /* multiprecision "fixed" form: radix-4 Stockham on
-                              fixed-point reals with 32 (L+1) fraction bits;
-                              same SoA-limb I/O as MPFFT_FORM_MP, results at
-                              the plain form's accuracy or better instead of
-                              the reference's bit pattern; |input| < 2^(29 -
-                              log2 n), checked by mpfft_plan_check.  At
-                              n >= 16 and a batch that fills its panels of
-                              64 transforms to within a quarter (any
-                              multiple of 64, or 48 of 64, ...; above 768
-                              bits with n <= 1024: any batch) as
-                              MPFFT_FORM_FIXED_PANEL; otherwise as
+#define MPFFT_FORM_FIXED 8 /* multiprecision "fixed" form: Stockham passes on
+                              fixed-point reals with 32 (L+1) fraction bits
+                              and a block exponent per transform; same
+                              SoA-limb I/O as MPFFT_FORM_MP, results at the
+                              plain form's accuracy or better instead of the
+                              reference's bit pattern.  Up to 256 bits at
+                              n >= 4096 (up to 64 bits at any n) as
+                              MPFFT_FORM_FIXED_LANE; otherwise, at n >= 16
+                              and a batch that fills its panels of 64
+                              transforms to within a quarter (any multiple
+                              of 64, or 48 of 64, ...; above 768 bits with
+                              n <= 1024: any batch) as
+                              MPFFT_FORM_FIXED_PANEL, else as
                               MPFFT_FORM_FIXED_LANE                        */
 #define MPFFT_FORM_FIXED_LANE 9 /* the fixed form with per-lane twiddle
                               products (29-bit digit multiply-adds; 28-bit
-                              at 2048 bits) at any batch; same format and
-                              error bound                                 */
+                              at 2048 bits) at any batch: radix-16 passes
+                              (four waves per butterfly) up to 256 bits,
+                              radix 4 beyond; same format and error bound */
 #define MPFFT_FORM_FIXED_PANEL 10 /* the fixed form with transforms
                               interleaved in panels of 64 and the twiddle
                               products on the int8 matrix cores; any batch

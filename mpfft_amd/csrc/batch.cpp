@@ -262,7 +262,8 @@ int mpfft_batch_create(mpfft_batch *batch, unsigned int count, const unsigned lo
         if (form == MPFFT_FORM_FIXED) {
             if (mpfft::fxp_preferred(N, cnt, W)) {
                 add_group(kv, 0, cnt, MPFFT_FORM_FIXED_PANEL);
-            } else if (cnt > mpfft::kPanel && mpfft::fxp_eligible(N, cnt)) {
+            } else if (cnt > mpfft::kPanel && mpfft::fxp_eligible(N, cnt) &&
+                       !mpfft::fxl_preferred(N, W)) {
                 // a class whose last panel would be mostly padding: its full
                 // panels on the panel kernels, the rest on the per-lane ones
                 const uint32_t full = cnt / mpfft::kPanel * mpfft::kPanel;

@@ -239,16 +239,31 @@ inline bool fxp_eligible(uint32_t N, uint32_t batch)
     return N >= 16 && batch > 0;
 }
 
-// MPFFT_FORM_FIXED takes the panels (reals of W words) when the padding of
-// the last panel is at most a quarter of the batch, and at W >= 34 for N <=
-// 1024 at any batch.  Measured on MI355X, time per transform of the per-lane
-// kernels over the panels at N = 2^6 .. 2^14, P = 128 .. 1024, batches 16 ..
-// 128: 1.1-1.3 at full panels, falling with the padding to ~1 near a quarter
-// of the batch for N >= 2^12; at W = 34 and N <= 2^10 above 1 at every batch
-// (the per-lane kernels run few waves of very wide reals there).
+// The per-lane kernels run radix-16 passes up to this many words per real
+// (fxl_plan).
+constexpr int kFxl16MaxW = 10;
+
+// Where those beat the panels at any batch: measured on MI355X, time per
+// transform of the per-lane radix-16 passes over the panels at batch 64:
+// 0.64-0.92 for N >= 2^14 at 3 to 10 words per real, 0.81-1.06 at N = 2^12,
+// and for N <= 2^10 0.81-1.07 at 3 and 4 words but 0.92-1.6 at 5 to 10
+// (N = 2^16, 256 bits: 63.4 k against 47.2 k FFT/s).
+inline bool fxl_preferred(uint32_t N, int W)
+{
+    return W <= kFxl16MaxW && (N >= 4096 || W <= 4);
+}
+
+// MPFFT_FORM_FIXED takes the panels (reals of W words) unless fxl_preferred,
+// when the padding of the last panel is at most a quarter of the batch, and
+// at W >= 34 for N <= 1024 at any batch.  Measured on MI355X, time per
+// transform of the per-lane radix-4 kernels over the panels at N = 2^6 ..
+// 2^14, P = 128 .. 1024, batches 16 .. 128: 1.1-1.3 at full panels, falling
+// with the padding to ~1 near a quarter of the batch for N >= 2^12; at W =
+// 34 and N <= 2^10 above 1 at every batch (the per-lane kernels run few
+// waves of very wide reals there).
 inline bool fxp_preferred(uint32_t N, uint32_t batch, int W)
 {
-    if (!fxp_eligible(N, batch)) return false;
+    if (!fxp_eligible(N, batch) || fxl_preferred(N, W)) return false;
     if (W >= 34 && N <= 1024) return true;
     return 4 * (fxp_groups(batch) * kPanel - batch) <= batch;
 }
@@ -509,8 +524,6 @@ inline int mw_plan(uint32_t N, int K, MwPass (&p)[32], uint32_t *nent)
 // r-major, (r - 1) Ns + k (the lanes of a wave hold consecutive k); radix 4
 // and 2 k-major, (R - 1) k + r - 1 (a DPP quad holds the r of one k); with a
 // radix-16 pass, ten entries from *nent hold w16^m, m = 0 .. 9.
-constexpr int kFxl16MaxW = 10;
-
 inline int fxl_plan(uint32_t N, int W, MwPass (&p)[32], uint32_t *nent)
 {
     if (W <= kFxl16MaxW) return mw_plan(N, 2, p, nent);

@@ -110,7 +110,9 @@ static void plans()
     CHECK(fxp_groups(1) == 1 && fxp_groups(64) == 1 && fxp_groups(65) == 2, "groups");
     CHECK(fxp_preferred(1024, 64, 10) && fxp_preferred(1024, 120, 10) &&
               !fxp_preferred(1 << 16, 80, 10) && fxp_preferred(256, 3, 34) &&
-              !fxp_preferred(8, 64, 10),
+              !fxp_preferred(8, 64, 10) && !fxp_preferred(4096, 64, 10) &&
+              !fxp_preferred(1024, 64, 4) && fxp_preferred(4096, 64, 12) &&
+              fxl_preferred(1 << 16, 10) && !fxl_preferred(1 << 16, 12),
           "preferred rule");
 }
 

@@ -160,6 +160,7 @@ def test_batch_fixed_splits_partial_panels():
         idx = [i for i in range(len(ns)) if ns[i] == n and Ps[i] == P]   # input order
         f = forms[idx]
         W = M.limbs_for_precision(P) + 2
+        assert not (W <= 10 and (n >= 4096 or W <= 4))           # (lanes preferred)
         whole = 4 * (-c % 64) <= c or (W >= 34 and n <= 1024)     # the plan rule
         if whole:
             assert (f == M.FORM_FIXED_PANEL).all(), (n, P, c)

@@ -60,7 +60,7 @@ def check_against_oracle(n, P, x, y):
                                  (256, 768)])
 def test_panel_matches_lane_and_oracle(n, P):
     xs = inputs(n, P, 64, seed=9000 + n + P)
-    yp = run(n, P, xs, M.FORM_FIXED)
+    yp = run(n, P, xs, M.FORM_FIXED_PANEL)
     yl = run(n, P, xs, M.FORM_FIXED_LANE)
     check_against_lane(n, P, yp, yl)
     for i in (0, 21, 63):
@@ -71,7 +71,7 @@ def test_panel_two_groups():
     # batch 128: two groups of 64 transforms per pass
     n, P = 256, 256
     xs = inputs(n, P, 128, seed=4242)
-    yp = run(n, P, xs, M.FORM_FIXED)
+    yp = run(n, P, xs, M.FORM_FIXED_PANEL)
     yl = run(n, P, xs, M.FORM_FIXED_LANE)
     check_against_lane(n, P, yp, yl)
     check_against_oracle(n, P, xs[100], yp[100])
@@ -82,14 +82,14 @@ def test_panel_stage_ranges():
     import torch
     n, P, b = 1024, 256, 64
     xs = inputs(n, P, b, seed=77)
-    plan = M.Plan(n, P, M.FORM_FIXED, b)
+    plan = M.Plan(n, P, M.FORM_FIXED_PANEL, b)
     x = np.concatenate([I.relimb(t, n, O.limbs(P), plan.L) for t in xs])
     S = plan.stages()
     # input conversion, a radix-4 pass (log2 n = 10 = 2 + 2 * 4), two radix-16
     # passes, output conversion
     assert S == 1 + 3 + 1
-    assert M.Plan(n, 1024, M.FORM_FIXED, b).stages() == 1 + 3 + 1   # W = 34 likewise
-    assert M.Plan(n, 2048, M.FORM_FIXED, b).stages() == 1 + 5 + 1   # W = 66: radix 4
+    assert M.Plan(n, 1024, M.FORM_FIXED_PANEL, b).stages() == 1 + 3 + 1   # W = 34 likewise
+    assert M.Plan(n, 2048, M.FORM_FIXED_PANEL, b).stages() == 1 + 5 + 1   # W = 66: radix 4
     dx = torch.from_numpy(x.view(np.int32)).cuda()
     d1, d2 = torch.empty_like(dx), torch.empty_like(dx)
     plan.execute_device(dx.data_ptr(), d1.data_ptr())
@@ -106,9 +106,9 @@ def test_panel_input_range_checked():
     h, _ = I.split(xs[40], n, O.limbs(P))
     h[17] = np.int32(((1 << 29) << 1) | (h[17] & 1))   # beyond the limb format's range
     with pytest.raises(M.MpfftError) as e:
-        run(n, P, xs, M.FORM_FIXED)
+        run(n, P, xs, M.FORM_FIXED_PANEL)
     assert e.value.code == -3
-    y = run(n, P, inputs(n, P, 64, seed=6), M.FORM_FIXED)
+    y = run(n, P, inputs(n, P, 64, seed=6), M.FORM_FIXED_PANEL)
     assert len(y) == 64
 
 
@@ -165,10 +165,10 @@ def test_panel_large_n():
             nz = (h & ~1) != np.int32(-2**31)
             h[nz] -= np.int32(2 * sh)
             xs.append(I.join(h, lm))
-    plan = M.Plan(n, P, M.FORM_FIXED, 64)
+    plan = M.Plan(n, P, M.FORM_FIXED_PANEL, 64)
     assert plan.panel_plan() == 1 and plan.stages() == 1 + 5 + 1
     plan.close()
-    yp = run(n, P, xs, M.FORM_FIXED)
+    yp = run(n, P, xs, M.FORM_FIXED_PANEL)
     yl = run(n, P, xs, M.FORM_FIXED_LANE)
     check_against_lane(n, P, yp, yl)
     for i in (0, 63):
