@@ -242,7 +242,7 @@ inline bool fxp_eligible(uint32_t N, uint32_t batch)
 // The per-lane kernels run radix-16 passes up to this many words per real
 // (fxl_plan).  Beyond, the exchange of a butterfly's 16 values (16 x 2 W
 // words x 64 lanes of LDS) leaves one block per CU: at 14 and 18 words the
-// radix-16 lanes measured 1.4-1.8 times the radix-4 lanes' time and 1.7-2.7
+// radix-16 lanes measured 1.4-2.5 times the radix-4 lanes' time and 2.0-2.7
 // times the panels' at N = 2^10 .. 2^16, batch 64.
 constexpr int kFxl16MaxW = 10;
 
