@@ -115,6 +115,24 @@ def test_fixed_batch():
         assert np.array_equal(got[i], run_fixed(n, P, xs[i]))
 
 
+@pytest.mark.parametrize("n,P", [(16, 256), (32, 64), (64, 256), (256, 128), (512, 200),
+                                 (1024, 256), (2048, 64)])
+def test_fixed_lane_radix16_many_transforms(n, P):
+    # the per-lane radix-16 passes (up to 256 bits) with 100 transforms: small
+    # transforms share a block (64 butterflies), passes with Ns < 64 restage
+    # their outputs through LDS, the last block is partial; each result is
+    # the same transform run alone, bit for bit, and some are within the
+    # fixed form's bound against the 2P-bit oracle
+    b = 100
+    xs = [I.mp_random(n, P, seed=4300 + 3 * i + n + P) for i in range(b)]
+    got = np.split(run_fixed(n, P, np.concatenate(xs), batch=b, form=M.FORM_FIXED_LANE), b)
+    for i in (0, 1, 37, 63, 64, 99):
+        assert np.array_equal(got[i], run_fixed(n, P, xs[i], form=M.FORM_FIXED_LANE)), i
+    for i in (0, 99):
+        star = O.mp_fft(n, P, xs[i], Pw=2 * P)
+        O.fixed_check(n, P, M.limbs_for_precision(P), xs[i], got[i], star)
+
+
 def test_fixed_input_range_checked():
     # the limb format's exponent range: |exponent| < 2^29 (a real 2^40 times
     # the others is accepted since the block exponent follows the largest)
