@@ -1,10 +1,14 @@
-// The "fixed" form of the multiprecision FFT: a radix-4 Stockham
-// (self-sorting, out-of-place) transform on fixed-point multiprecision
-// reals (fx_arith.h), for callers that want the transform at the plain
-// form's accuracy or better rather than the reference's bit pattern.
+// The "fixed" form of the multiprecision FFT on its per-lane kernels: a
+// Stockham (self-sorting, out-of-place) transform on fixed-point
+// multiprecision reals (fx_arith.h), for callers that want the transform at
+// the plain form's accuracy or better rather than the reference's bit
+// pattern.  Passes (fxl_plan in kernels.h): radix 16 on four waves per
+// butterfly up to kFxl16MaxW words per real (fx_r16_kernel), radix 4 with a
+// DPP quad per butterfly beyond (fx_r4_kernel), a radix-2 pass first where
+// log2 N is odd.
 //
 // Why it is cheaper than the plain form: every radix-4 pass covers two
-// levels with one read and one write of the data (the plain form reads and
+// levels (a radix-16 pass four) with one read and one write of the data (the plain form reads and
 // writes every element once per level), additions are exact W-word carry
 // chains instead of aligned, normalised and rounded floating-point sums, and
 // a product needs no normalisation or rounding test.  The cost is an
@@ -17,10 +21,11 @@
 // the last pass rounds back to it at P bits, so the conversions cost no
 // extra pass over the data.
 //
-// Twiddles: one table per radix-4 pass, entry 3k + r - 1 = w^(r k) of that
-// pass (k < Ns, r = 1..3), so the lanes of a wave read consecutive entries.
+// Twiddles: one table per pass, for radix 4 entry 3k + r - 1 = w^(r k) of
+// that pass (k < Ns, r = 1..3), so the lanes of a wave read consecutive
+// entries (radix 16: (r - 1) Ns + k, for the same reason).
 //
-// Pass (radix R = 4 or 2, Ns = size of the sub-transforms already done):
+// Radix-4 pass (radix R = 4 or 2, Ns = size of the sub-transforms already done):
 //   for j < N/R, k = j mod Ns:
 //     v_r = x[j + r N/R] * w^(r k), w = exp(-2 pi i / (R Ns))
 //     y = DFT_R(v)
