@@ -445,6 +445,36 @@ __device__ __forceinline__ void w_xget(CW_<K> &v, double2 (*x)[64], uint32_t l)
     }
 }
 
+// The same words in the output staging of the radix-16 pass: word pair h
+// of element e at st[h 1024 + e]
+template <int K>
+__device__ __forceinline__ void w_sput(double2 *st, uint32_t e, const CW_<K> &v)
+{
+    if constexpr (K == 2) {
+        st[e] = make_double2(v.re.hi, v.re.lo);
+        st[1024 + e] = make_double2(v.im.hi, v.im.lo);
+    } else {
+        st[e] = make_double2(v.re.c[0], v.re.c[1]);
+        st[1024 + e] = make_double2(v.re.c[2], v.re.c[3]);
+        st[2048 + e] = make_double2(v.im.c[0], v.im.c[1]);
+        st[3072 + e] = make_double2(v.im.c[2], v.im.c[3]);
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void w_sget(CW_<K> &v, const double2 *st, uint32_t e)
+{
+    if constexpr (K == 2) {
+        const double2 r = st[e], i = st[1024 + e];
+        v.re = {r.x, r.y};
+        v.im = {i.x, i.y};
+    } else {
+        const double2 r0 = st[e], r1 = st[1024 + e], i0 = st[2048 + e], i1 = st[3072 + e];
+        v.re = {{r0.x, r0.y, r1.x, r1.y}};
+        v.im = {{i0.x, i0.y, i1.x, i1.y}};
+    }
+}
+
 // Radix-16 Stockham pass of sub-transform size Ns, one butterfly on four
 // waves.  Butterfly g = 64 blockIdx + lane is j = g mod N/16 of transform
 // g / (N/16) (small transforms share a block).  Wave w reads v_r = x[j + r
@@ -501,13 +531,51 @@ mw_r16_kernel(FxArgs a, const double2 *src, double2 *dst, uint32_t Ns, uint32_t 
         for (int q1 = 0; q1 < 4; q1++) w_xput<K>(xs[4 * w + q1], lane, v[q1]);
     }
     __syncthreads();
-    if (!live) return;
+    if (live) {
 #pragma unroll
-    for (int r1 = 0; r1 < 4; r1++) w_xget<K>(v[r1], xs[4 * r1 + w], lane);
-    w_dft4<K>(v[0], v[1], v[2], v[3]);         // v[q2] = y_(w + 4 q2)
-    const uint32_t o = (j - k) * 16 + k;
+        for (int r1 = 0; r1 < 4; r1++) w_xget<K>(v[r1], xs[4 * r1 + w], lane);
+        w_dft4<K>(v[0], v[1], v[2], v[3]);     // v[q2] = y_(w + 4 q2)
+    }
+    if (Ns >= 16) {
+        // a wave's outputs y_q lie on runs of at least 16 consecutive
+        // positions (256 bytes and more)
+        if (!live) return;
+        const uint32_t o = (j - k) * 16 + k;
 #pragma unroll
-    for (int q2 = 0; q2 < 4; q2++) w_store<K, L, FULL, OUT>(a, dst, b, o + (w + 4 * q2) * Ns, v[q2]);
+        for (int q2 = 0; q2 < 4; q2++)
+            w_store<K, L, FULL, OUT>(a, dst, b, o + (w + 4 * q2) * Ns, v[q2]);
+        return;
+    }
+    // Ns < 16: the block's outputs fill 1024 consecutive positions of the
+    // (transform, position) order, output q of lane l at e = 16 (l - k) + k
+    // + q Ns, but a wave would store them in runs of Ns elements 16 Ns
+    // apart: they go through LDS (row h of the words at h 1024, each group of
+    // 8 elements rotated by e / 16 so neither side conflicts on the banks)
+    // and leave as runs of 64 consecutive elements (the first pass at
+    // N = 2^16, 106 bits: 79 -> 64 us)
+    double2 *st = &xs[0][0][0];
+    auto swz = [](uint32_t e) { return (e & ~7u) | ((e + (e >> 4)) & 7u); };
+    __syncthreads();                           // the slots have been read
+    if (live) {
+#pragma unroll
+        for (int q2 = 0; q2 < 4; q2++) {
+            const uint32_t e = swz(16 * (lane - k) + k + (w + 4 * q2) * Ns);
+            w_sput<K>(st, e, v[q2]);
+        }
+    }
+    __syncthreads();
+    const uint32_t lgN = lgS + 4;
+    const uint64_t total = (uint64_t)a.batch << lgN;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const uint32_t e = 256 * m + 64 * w + lane;
+        const uint64_t G = (uint64_t)blockIdx.x * 1024 + e;
+        if (G >= total) break;
+        const uint32_t bb = (uint32_t)(G >> lgN), pos = (uint32_t)G & (a.N - 1), es = swz(e);
+        CW_<K> y;
+        w_sget<K>(y, st, es);
+        w_store<K, L, FULL, OUT>(a, dst, bb, pos, y);
+    }
 }
 
 // N = 1: the transform is the identity; the input split and rounded back
