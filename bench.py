@@ -745,7 +745,8 @@ def side_leg(line, key, fn):
 def host_cpu():
     """the host CPU: model name, logical CPUs, CPUs usable by this process,
     and the OpenMP thread count of the port leg (OMP_NUM_THREADS if set,
-    else the usable CPUs, at most 16: one GPU's share of a GPU box)"""
+    else the usable CPUs, at most 16: one GPU's share of the host's cores on
+    a machine of several GPUs)"""
     model = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -769,8 +770,8 @@ def cpu_legs(O, I, n, P, seed):
     # OpenMP CODE 2 (4 threads, hard-coded at parallel_mp_fft2.c:262) when
     # its build is present, and the oracle restatement on the host's CPU
     # share and on 4 threads.  The host: CPU model, logical CPUs, and the
-    # CPUs this process may run on (a GPU box gives one GPU's work a share
-    # of 16; OMP_NUM_THREADS states it)
+    # CPUs this process may run on (a machine of several GPUs may give one
+    # GPU's work a share of its cores; OMP_NUM_THREADS states it)
     host = host_cpu()
     nthr = host["omp_threads"]
     xs = I.mp_random(n, P, seed=seed)
