@@ -536,7 +536,7 @@ def main():
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {a.gpus} is not supported "
               "(the multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
 
