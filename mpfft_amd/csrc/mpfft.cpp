@@ -1150,9 +1150,10 @@ int mpfft_plan_create(mpfft_plan *plan, unsigned long n, long prec, int form, un
     *plan = nullptr;
     if (!is_pow2(n) || n > kMaxN) return MPFFT_ERR_SIZE;
     // the chosen form, by the measured rates at matched error: the
-    // double-word form where it applies (2.9x the plain form at 106 bits,
-    // 1.9x the fixed form at 128), the fixed form elsewhere (the quad-word
-    // form matches the fixed form's speed at 212 bits with a looser bound)
+    // double-word form where it applies (at 106 bits 4.5x the plain form
+    // and 1.7x the fixed form at 128 bits, N = 2^16, batch 64), the fixed
+    // form elsewhere (the quad-word form at 212 bits runs at 0.83x the fixed
+    // form at 256 bits, with a looser bound)
     if (form == MPFFT_FORM_AUTO)
         form = (prec >= kDdMinPrec && prec <= kDdMaxPrec) ? MPFFT_FORM_DD : MPFFT_FORM_FIXED;
     if (!is_limb_form(form) && form != MPFFT_FORM_D5 && form != MPFFT_FORM_D6)
